@@ -1,0 +1,249 @@
+/*
+ * refraktor_amd — C ABI of the MI355X-native fractal-flame renderer.
+ *
+ * Every entry point returns an int status (RFK_OK = 0, negative on error) and
+ * never exits the process; the last error text of the calling thread is
+ * available from rfk_last_error(). Handles are opaque; memory behind them is
+ * owned by the library. Device pointers are plain `void*`/typed pointers into
+ * HBM, host pointers are marked "host".
+ *
+ * Each declaration cites the librefrakt interface it replaces
+ * (paths relative to the untrioctium/refraktor tree).
+ */
+#ifndef REFRAKT_AMD_H
+#define REFRAKT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RFK_OK 0
+#define RFK_ERR_INVALID (-1)    /* bad argument or malformed input          */
+#define RFK_ERR_PARSE (-2)      /* flame / DSL / YAML parse failure          */
+#define RFK_ERR_COMPILE (-3)    /* hiprtc compile failure (log filled)       */
+#define RFK_ERR_HIP (-4)        /* HIP runtime failure                       */
+#define RFK_ERR_NODEVICE (-5)   /* no GPU visible                            */
+#define RFK_ERR_OVERFLOW (-6)   /* a fixed-point histogram bin would wrap    */
+
+typedef struct rfk_db rfk_db;           /* variation database                */
+typedef struct rfk_flame rfk_flame;     /* host-side flame description       */
+typedef struct rfk_kernel rfk_kernel;   /* compiled iterate kernel           */
+typedef struct rfk_state rfk_state;     /* walkers + histogram of one render */
+
+/* Text of the last error on this thread ("" if none). */
+const char* rfk_last_error(void);
+/* Library version string. */
+const char* rfk_version(void);
+
+/* ------------------------------------------------------------------ DB --
+ * rfkt::initialize(flamedb&, std::string_view config_path)
+ *   src/librefrakt/include/librefrakt/flame_info.h:159,
+ *   src/librefrakt/src/flame_info.cpp:60-148.
+ * Loads `<dir>/variations_new.yml` and `<dir>/common.yml`; entries that fail to
+ * parse or type-check are skipped and counted in *n_errors (may be NULL).     */
+int rfk_db_load(const char* config_dir, rfk_db** out, int* n_errors);
+int rfk_db_free(rfk_db* db);
+/* Number of variations, the i-th name (sorted), and an error message. */
+int rfk_db_variation_count(const rfk_db* db);
+const char* rfk_db_variation_name(const rfk_db* db, int i);
+const char* rfk_db_error(const rfk_db* db, int i);
+/* Flamedb value hash (flame_info.cpp:210-222 recalc_hash equivalent). */
+uint64_t rfk_db_hash(const rfk_db* db);
+
+/* --------------------------------------------------------------- flame --
+ * rfkt::flame::deserialize (src/librefrakt/src/flame_types.cpp:437-485):
+ * parses the flame JSON document (center_x, center_y, scale, rotate, gamma,
+ * brightness, vibrancy, highlight_power, gamma_threshold, mod_hue, mod_sat,
+ * mod_val, xforms[], final_xform, palette). `palette` is either librefrakt's
+ * zlib+base64 blob of 256 (h,s,v) doubles or a JSON array of [h,s,v] triples;
+ * an optional `chaos` n x n array enables xaos.                              */
+int rfk_flame_from_json(const rfk_db* db, const char* json, rfk_flame** out);
+/* rfkt::import_flam3 (flame_types.cpp:131-239): flam3 XML -> flame. */
+int rfk_flame_from_flam3(const rfk_db* db, const char* xml, rfk_flame** out);
+/* rfkt::flame::serialize (flame_types.cpp:366-397); palette written as [h,s,v]
+ * triples. Writes at most cap bytes (NUL-terminated); *needed gets the size.  */
+int rfk_flame_to_json(const rfk_flame* f, char* buf, size_t cap, size_t* needed);
+int rfk_flame_free(rfk_flame* f);
+/* rfkt::flame::size_reals (flame_types.h:602-613). */
+int rfk_flame_size_reals(const rfk_flame* f, size_t* out);
+/* rfkt::flame::affine_indices (flame_types.h:674-705); *count set always,
+ * at most cap entries written.                                              */
+int rfk_flame_affine_indices(const rfk_flame* f, size_t* out, size_t cap, size_t* count);
+/* rfkt::flame::pack_samples (flame_types.h:707-712): `count` samples at
+ * t = start + offset*i, each size_reals + 768 doubles (host), for a w x h view. */
+int rfk_flame_pack_samples(const rfk_flame* f, double start, double offset, int count, int w, int h,
+                           double* out /* host */, size_t cap);
+/* Number of standard xforms and whether a final xform exists. */
+int rfk_flame_xform_count(const rfk_flame* f, int* n_xforms, int* has_final);
+/* Structural alignment of keyframes for interpolation (the interpolator's
+ * interp_by_weight path, src/refraktor/src/refraktor.cpp:146-198): every
+ * output flame holds the concatenation of all inputs' xforms, with weight 0
+ * on the xforms that belong to other keyframes.                              */
+int rfk_flames_align(const rfk_db* db, rfk_flame* const* in, int n, rfk_flame** out);
+
+/* ------------------------------------------------------------ compiler --
+ * rfkt::flame_compiler::get_flame_kernel / make_source
+ * (src/librefrakt/include/librefrakt/flame_compiler.h:182-183,
+ *  src/librefrakt/src/flame_compiler.cpp:448-656).
+ * Generates gfx950 HIP source for the flame's xforms from the DSL, compiles it
+ * with hiprtc (cached on disk by source hash) and loads it. On compile failure
+ * returns RFK_ERR_COMPILE with the hiprtc log in `log` (may be NULL).        */
+#define RFK_PRECISION_F32 0
+#define RFK_PRECISION_F64 1
+typedef struct rfk_exec_config {
+    uint32_t threads_per_block;  /* walkers per block: 64, 256 or 1024 (0 = 256) */
+    uint32_t atomic_width;       /* 32: four u32 atomics per draw; 64: two u64 (0 = 32) */
+    uint32_t bin_tile_log2;      /* reserved for LDS bin tiles (0 = off)       */
+    uint32_t reserved;
+} rfk_exec_config;
+int rfk_compile(const rfk_db* db, const rfk_flame* f, int precision, const rfk_exec_config* cfg,
+                rfk_kernel** out, char* log, size_t log_cap);
+/* Compile only (no device needed): runs the generator and hiprtc for gfx950
+ * and reports the code-object size; used to check generated sources offline. */
+int rfk_compile_check(const rfk_db* db, const rfk_flame* f, int precision, const rfk_exec_config* cfg,
+                      size_t* code_bytes, char* log, size_t log_cap);
+/* The generated HIP source (make_source); writes at most cap bytes. */
+int rfk_make_source(const rfk_db* db, const rfk_flame* f, int precision, const rfk_exec_config* cfg,
+                    char* buf, size_t cap, size_t* needed);
+int rfk_kernel_free(rfk_kernel* k);
+int rfk_kernel_size_reals(const rfk_kernel* k, size_t* out);
+int rfk_kernel_threads_per_block(const rfk_kernel* k, uint32_t* out);
+double rfk_kernel_compile_ms(const rfk_kernel* k);
+
+/* --------------------------------------------------------------- render --
+ * rfkt::flame_kernel::warmup (flame_compiler.h:99-100, flame_compiler.cpp:1037-1108).
+ * samples: host doubles from rfk_flame_pack_samples (>= 4 samples, n_reals
+ * total). Runs host fix_rotation, the Catmull-Rom coefficient kernel, sample
+ * evaluation, per-sample precalc + anti-alias jitter, walker seeding and
+ * `fuse` un-binned passes. `walkers` is rounded up to a whole number of blocks.
+ * temporal_multiplier multiplies the number of temporal samples (blocks).
+ * stream: a hipStream_t or NULL (library stream). The state owns a zeroed
+ * fixed-point RGBA u32 histogram of width x height.                         */
+int rfk_warmup(rfk_kernel* k, void* stream, const double* samples, size_t n_reals, uint32_t width, uint32_t height,
+               uint32_t seed, uint32_t fuse, uint32_t walkers, int temporal_multiplier, rfk_state** out);
+int rfk_state_free(rfk_state* s);
+
+/* rfkt::flame_kernel::bailout_args (flame_compiler.h:92-96). iters is an exact
+ * iteration budget (rounded up to whole passes of all walkers; 0 = no budget),
+ * millis a device-clock time limit (0 = none), quality a target mean density
+ * in draws per pixel (0 = none). At least one must be set.                  */
+typedef struct rfk_bailout {
+    uint64_t iters;
+    double millis;
+    double quality;
+} rfk_bailout;
+
+/* rfkt::flame_kernel::bin_result (flame_compiler.h:27-35) plus counters. */
+typedef struct rfk_bin_result {
+    double quality;           /* accumulated mean density (draw units) this call */
+    double elapsed_ms;        /* device clock, launch to end                 */
+    double kernel_ms;         /* hipEvent time of the iterate launch(es)     */
+    uint64_t total_passes;    /* iterations (walker steps) done this call    */
+    uint64_t total_draws;     /* in-bounds iterations with opacity > 0       */
+    uint64_t total_bins;      /* width * height                              */
+    double passes_per_thread;
+    double max_density;       /* max bin density after the call (draw units) */
+    uint64_t bad_values;      /* walkers reset after a non-finite / huge value */
+    uint64_t hit_sum;         /* sum of 255*opacity over draws (quality units) */
+} rfk_bin_result;
+
+/* rfkt::flame_kernel::bin (flame_compiler.cpp:866-955); synchronous. */
+int rfk_bin(rfk_kernel* k, rfk_state* s, const rfk_bailout* bo, rfk_bin_result* out);
+/* Asynchronous form: enqueues the work on the state's stream and returns;
+ * rfk_bin_wait collects the result.                                         */
+int rfk_bin_async(rfk_kernel* k, rfk_state* s, const rfk_bailout* bo);
+int rfk_bin_wait(rfk_state* s, rfk_bin_result* out);
+/* rfkt::flame_kernel::saved_state::abort_binning (flame_compiler.h:85-88).
+ * Thread-safe; the running bin launch stops at its next check.             */
+int rfk_abort(rfk_state* s);
+
+/* Oracle entry (no reference counterpart): bins a device list of n points
+ * (x, y, color, opacity) given in plane space, through the state's sample-0
+ * screen transform and palette, with the same draw code as the iterate kernel. */
+int rfk_bin_points(rfk_state* s, const float* xyco /* device, n*4 */, size_t n);
+/* Test entry: runs `passes` passes and records every walker's (x, y, color,
+ * opacity) after each pass and every wave's xform index (device buffers of
+ * passes*walkers*4 floats and passes*(walkers/64) u32).                     */
+int rfk_trace(rfk_kernel* k, rfk_state* s, uint32_t passes, float* trace, uint32_t* xform_ids);
+
+/* State accessors (device pointers remain owned by the state). */
+typedef struct rfk_state_info {
+    uint32_t width, height;
+    uint32_t walkers, threads_per_block, blocks;
+    uint32_t size_reals, param_stride;
+    uint32_t seed;
+    uint64_t passes_done;     /* passes per walker completed (block 0)       */
+    uint32_t* histogram;      /* device, height*width*4 u32 (R,G,B,W)        */
+    float* params;            /* device, blocks*param_stride floats          */
+    uint32_t* palettes;       /* device, blocks*256 packed RGBA8             */
+    float* walker_x;          /* device, walkers                             */
+    float* walker_y;
+    float* walker_c;
+    void* stream;             /* hipStream_t                                 */
+    double quality;           /* accumulated mean density over all bin calls */
+} rfk_state_info;
+int rfk_state_get_info(const rfk_state* s, rfk_state_info* out);
+/* Zero the histogram and counters (walkers keep their positions). */
+int rfk_state_clear_histogram(rfk_state* s);
+
+/* -------------------------------------------------------- image kernels --
+ * Density statistics: calculate_histogram<4> (assets/kernels/density_histo.cu:1-41)
+ * over the fixed-point histogram: 97 log2 bins (granularity 4) and max density. */
+int rfk_density_histogram(rfk_state* s, uint64_t* bins97 /* host, 97 */, double* max_density);
+
+/* Density estimation (no reference counterpart: flam3-style variable-width
+ * filter). Writes a float4 RGBA image in draw units (device, w*h*4 floats).
+ * Radius h = max_radius / n^curve clamped to >= min_radius, n = bin density;
+ * kernel exp(-2 (r/h)^2), r <= h, normalised per source bin.                */
+int rfk_density_estimate(rfk_state* s, float max_radius, float min_radius, float curve, float* out_rgba);
+
+/* rfkt::tonemapper::run (src/librefrakt/include/librefrakt/image/tonemapper.h:18-22,
+ * src/librefrakt/src/image/tonemapper.cpp:37-56; assets/kernels/tonemap.cu:50-105). */
+#define RFK_PIXEL_FLOAT4 0
+#define RFK_PIXEL_HALF4 1
+#define RFK_PIXEL_FLOAT3 2
+#define RFK_PIXEL_HALF3 3
+#define RFK_PIXEL_RGBA8 4
+typedef struct rfk_tonemap_args {
+    double quality;      /* mean density (state quality); <= 0 uses the state's */
+    double gamma;
+    double brightness;
+    double vibrancy;
+    double max_density;  /* informational, as in the reference                */
+} rfk_tonemap_args;
+/* Tonemap the state's fixed-point histogram. */
+int rfk_tonemap(rfk_state* s, void* out /* device */, int pixel_format, const rfk_tonemap_args* a);
+/* Tonemap a float4 RGBA image in draw units (e.g. the density-estimated one). */
+int rfk_tonemap_image(const float* in_rgba, uint32_t width, uint32_t height, void* out, int pixel_format,
+                      const rfk_tonemap_args* a, void* stream);
+
+/* Catmull-Rom coefficients: generate_sample_coefficients
+ * (assets/kernels/catmull.cu:33-42): device samples [nseg+3][sample_size]
+ * -> segments [nseg][sample_size][4] (a,b,c,d), tension 0, alpha 0.5.       */
+int rfk_catmull_coefficients(const double* samples, uint32_t sample_size, uint32_t nseg, double* segments,
+                             void* stream);
+/* Host fix_rotation (flame_compiler.cpp:957-1035) over `count` samples. */
+int rfk_fix_rotation(double* samples /* host */, size_t sample_size, size_t flame_size, const size_t* affine_indices,
+                     size_t n_indices, int count);
+
+/* Microbenchmark of uniform-random RGBA u32 atomic increments into a buffer of
+ * `footprint_bytes`; returns increments per second (the binning roofline).  */
+int rfk_atomic_peak(uint64_t footprint_bytes, uint64_t increments, double* increments_per_s);
+
+/* Device helpers for hosts without a GPU framework. */
+int rfk_device_count(int* n);
+int rfk_malloc(void** p, size_t bytes);
+int rfk_free(void* p);
+int rfk_memcpy_htod(void* dst, const void* src, size_t bytes);
+int rfk_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int rfk_memset(void* dst, int value, size_t bytes);
+int rfk_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REFRAKT_AMD_H */

@@ -1,0 +1,783 @@
+// C ABI implementation: handles, launch plumbing and state ownership.
+// Replaces rfkt::flame_kernel / saved_state / tonemapper host code
+// (src/librefrakt/src/flame_compiler.cpp:866-1108,
+//  src/librefrakt/src/image/tonemapper.cpp:37-56) with explicit HIP calls,
+// status codes instead of exit(), and a pinned host stop flag.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/refrakt.h"
+#include "compiler.hpp"
+#include "device/flame_device.h"
+#include "flame.hpp"
+#include "json.hpp"
+#include "static_kernels.hpp"
+#include "vardb.hpp"
+
+struct rfk_db {
+    rfk::variation_db db;
+};
+
+struct rfk_flame {
+    rfk::flame f;
+};
+
+struct rfk_kernel {
+    rfk::flame f;
+    rfk::flame_layout L;
+    rfk::compile_options opt;
+    hipModule_t mod = nullptr;
+    hipFunction_t f_iterate = nullptr;
+    hipFunction_t f_precalc = nullptr;
+    double compile_ms = 0.0;
+    bool cache_hit = false;
+    int device = 0;
+};
+
+struct rfk_state {
+    rfk_kernel* k = nullptr;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    uint32_t width = 0, height = 0, walkers = 0, tpb = 0, blocks = 0;
+    uint32_t size_reals = 0, param_stride = 0, seed = 0;
+    int precision = 0;
+    void* params = nullptr;
+    uint32_t* palettes = nullptr;
+    float *wx = nullptr, *wy = nullptr, *wc = nullptr;
+    uint32_t* wlast = nullptr;
+    uint32_t* block_pass = nullptr;
+    uint32_t* hist = nullptr;
+    unsigned long long* counters = nullptr;        // device [8]
+    unsigned long long* counters_host = nullptr;   // pinned [8 + 98]
+    unsigned long long* dstats = nullptr;          // device [97 + 1]
+    uint32_t* stop_host = nullptr;                 // pinned, mapped
+    uint32_t* stop_dev = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double quality = 0.0;
+    uint64_t passes_done = 0;
+    bool pending = false;
+    uint64_t pending_passes = 0;
+    double wallclock_khz = 100000.0;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define RFK_HIP(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(RFK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int check_device() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(RFK_ERR_NODEVICE, "no HIP device visible");
+    return RFK_OK;
+}
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T) + 16);
+}
+
+void free_state_buffers(rfk_state* s) {
+    auto f = [](void* p) { if (p) (void)hipFree(p); };
+    f(s->params); f(s->palettes); f(s->wx); f(s->wy); f(s->wc); f(s->wlast); f(s->block_pass); f(s->hist);
+    f(s->counters); f(s->dstats);
+    if (s->counters_host) (void)hipHostFree(s->counters_host);
+    if (s->stop_host) (void)hipHostFree(s->stop_host);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+rfk_iter_args base_args(rfk_state* s) {
+    rfk_iter_args a;
+    std::memset(&a, 0, sizeof a);
+    a.params = static_cast<const float*>(s->params);
+    a.palettes = s->palettes;
+    a.wx = s->wx;
+    a.wy = s->wy;
+    a.wc = s->wc;
+    a.wlast = s->wlast;
+    a.block_pass = s->block_pass;
+    a.hist = s->hist;
+    a.stop_flag = s->stop_dev;
+    a.counters = reinterpret_cast<u64*>(s->counters);
+    a.width = s->width;
+    a.height = s->height;
+    a.param_stride = s->param_stride;
+    a.seed = s->seed;
+    a.stream = 0;
+    a.total_walkers = s->walkers;
+    a.check_every = 16;
+    a.nsamples = s->blocks;
+    return a;
+}
+
+int launch_iterate(rfk_state* s, rfk_iter_args& a) {
+    void* argv[] = {&a};
+    RFK_HIP(hipModuleLaunchKernel(s->k->f_iterate, s->blocks, 1, 1, s->tpb, 1, 1, 0, s->stream, argv, nullptr));
+    return RFK_OK;
+}
+
+int reset_counters(rfk_state* s) {
+    unsigned long long init[8] = {0, 0, 0, 0, ~0ull, 0, 0, 0};
+    std::memcpy(s->counters_host, init, sizeof init);
+    RFK_HIP(hipMemcpyAsync(s->counters, s->counters_host, sizeof init, hipMemcpyHostToDevice, s->stream));
+    return RFK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rfk_last_error(void) { return g_last_error.c_str(); }
+const char* rfk_version(void) { return "refraktor_amd 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------------- DB
+
+int rfk_db_load(const char* config_dir, rfk_db** out, int* n_errors) {
+    if (!config_dir || !out) return fail(RFK_ERR_INVALID, "null argument");
+    auto db = std::make_unique<rfk_db>();
+    std::string err;
+    if (!db->db.load(config_dir, err)) return fail(RFK_ERR_PARSE, err);
+    if (n_errors) *n_errors = int(db->db.errors().size());
+    *out = db.release();
+    return RFK_OK;
+}
+
+int rfk_db_free(rfk_db* db) {
+    delete db;
+    return RFK_OK;
+}
+
+int rfk_db_variation_count(const rfk_db* db) { return db ? int(db->db.variations().size()) : 0; }
+
+const char* rfk_db_variation_name(const rfk_db* db, int i) {
+    if (!db || i < 0 || i >= int(db->db.variations().size())) return nullptr;
+    auto it = db->db.variations().begin();
+    std::advance(it, i);
+    return it->first.c_str();
+}
+
+const char* rfk_db_error(const rfk_db* db, int i) {
+    if (!db || i < 0 || i >= int(db->db.errors().size())) return nullptr;
+    return db->db.errors()[size_t(i)].c_str();
+}
+
+uint64_t rfk_db_hash(const rfk_db* db) { return db ? db->db.hash() : 0; }
+
+// ------------------------------------------------------------------- flame
+
+int rfk_flame_from_json(const rfk_db* db, const char* json, rfk_flame** out) {
+    if (!db || !json || !out) return fail(RFK_ERR_INVALID, "null argument");
+    try {
+        auto js = rfk::json::parse(json);
+        auto f = std::make_unique<rfk_flame>();
+        f->f = rfk::flame_from_json(js, db->db);
+        *out = f.release();
+        return RFK_OK;
+    } catch (const std::exception& e) {
+        return fail(RFK_ERR_PARSE, e.what());
+    }
+}
+
+int rfk_flame_from_flam3(const rfk_db* db, const char* xml, rfk_flame** out) {
+    if (!db || !xml || !out) return fail(RFK_ERR_INVALID, "null argument");
+    try {
+        auto f = std::make_unique<rfk_flame>();
+        f->f = rfk::flame_from_flam3(xml, db->db);
+        *out = f.release();
+        return RFK_OK;
+    } catch (const std::exception& e) {
+        return fail(RFK_ERR_PARSE, e.what());
+    }
+}
+
+int rfk_flame_to_json(const rfk_flame* f, char* buf, size_t cap, size_t* needed) {
+    if (!f) return fail(RFK_ERR_INVALID, "null flame");
+    std::string s = rfk::json::dump(rfk::flame_to_json(f->f));
+    if (needed) *needed = s.size() + 1;
+    if (buf && cap) {
+        size_t n = std::min(cap - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = '\0';
+    }
+    return RFK_OK;
+}
+
+int rfk_flame_free(rfk_flame* f) {
+    delete f;
+    return RFK_OK;
+}
+
+int rfk_flame_size_reals(const rfk_flame* f, size_t* out) {
+    if (!f || !out) return fail(RFK_ERR_INVALID, "null argument");
+    *out = f->f.size_reals();
+    return RFK_OK;
+}
+
+int rfk_flame_affine_indices(const rfk_flame* f, size_t* out, size_t cap, size_t* count) {
+    if (!f) return fail(RFK_ERR_INVALID, "null flame");
+    auto idx = f->f.affine_indices();
+    if (count) *count = idx.size();
+    for (size_t i = 0; out && i < idx.size() && i < cap; i++) out[i] = idx[i];
+    return RFK_OK;
+}
+
+int rfk_flame_pack_samples(const rfk_flame* f, double start, double offset, int count, int w, int h, double* out,
+                           size_t cap) {
+    if (!f || !out || count <= 0 || w <= 0 || h <= 0) return fail(RFK_ERR_INVALID, "bad argument");
+    std::vector<double> v;
+    for (int i = 0; i < count; i++) f->f.pack_sample(v, start + offset * i, w, h);
+    if (v.size() > cap) return fail(RFK_ERR_INVALID, "output buffer too small: need " + std::to_string(v.size()));
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+    return RFK_OK;
+}
+
+int rfk_flame_xform_count(const rfk_flame* f, int* n_xforms, int* has_final) {
+    if (!f) return fail(RFK_ERR_INVALID, "null flame");
+    if (n_xforms) *n_xforms = int(f->f.xforms.size());
+    if (has_final) *has_final = f->f.final_xform ? 1 : 0;
+    return RFK_OK;
+}
+
+int rfk_flames_align(const rfk_db* db, rfk_flame* const* in, int n, rfk_flame** out) {
+    if (!db || !in || !out || n <= 0) return fail(RFK_ERR_INVALID, "bad argument");
+    std::vector<rfk::flame> fl;
+    for (int i = 0; i < n; i++) fl.push_back(in[i]->f);
+    auto aligned = rfk::align_by_weight(fl);
+    for (int i = 0; i < n; i++) {
+        out[i] = new rfk_flame;
+        out[i]->f = std::move(aligned[size_t(i)]);
+    }
+    return RFK_OK;
+}
+
+// ---------------------------------------------------------------- compiler
+
+static int make_options(const rfk_exec_config* cfg, int precision, rfk::compile_options& o) {
+    o.precision = precision;
+    o.tpb = (cfg && cfg->threads_per_block) ? cfg->threads_per_block : 256;
+    o.atomic_width = (cfg && cfg->atomic_width) ? cfg->atomic_width : 32;
+    if (o.tpb != 64 && o.tpb != 128 && o.tpb != 256 && o.tpb != 512 && o.tpb != 1024)
+        return fail(RFK_ERR_INVALID, "threads_per_block must be 64, 128, 256, 512 or 1024");
+    if (o.atomic_width != 32 && o.atomic_width != 64) return fail(RFK_ERR_INVALID, "atomic_width must be 32 or 64");
+    if (precision != RFK_PRECISION_F32 && precision != RFK_PRECISION_F64) return fail(RFK_ERR_INVALID, "bad precision");
+    return RFK_OK;
+}
+
+int rfk_make_source(const rfk_db* db, const rfk_flame* f, int precision, const rfk_exec_config* cfg, char* buf,
+                    size_t cap, size_t* needed) {
+    if (!db || !f) return fail(RFK_ERR_INVALID, "null argument");
+    rfk::compile_options o;
+    if (int rc = make_options(cfg, precision, o)) return rc;
+    try {
+        auto L = rfk::make_layout(f->f, db->db);
+        std::string src = rfk::make_source(db->db, f->f, L, o);
+        if (needed) *needed = src.size() + 1;
+        if (buf && cap) {
+            size_t n = std::min(cap - 1, src.size());
+            std::memcpy(buf, src.data(), n);
+            buf[n] = '\0';
+        }
+        return RFK_OK;
+    } catch (const std::exception& e) {
+        return fail(RFK_ERR_PARSE, e.what());
+    }
+}
+
+int rfk_compile(const rfk_db* db, const rfk_flame* f, int precision, const rfk_exec_config* cfg, rfk_kernel** out,
+                char* log, size_t log_cap) {
+    if (!db || !f || !out) return fail(RFK_ERR_INVALID, "null argument");
+    if (log && log_cap) log[0] = '\0';
+    auto k = std::make_unique<rfk_kernel>();
+    if (int rc = make_options(cfg, precision, k->opt)) return rc;
+    std::string src;
+    try {
+        k->f = f->f;
+        k->L = rfk::make_layout(k->f, db->db);
+        src = rfk::make_source(db->db, k->f, k->L, k->opt);
+    } catch (const std::exception& e) {
+        return fail(RFK_ERR_PARSE, e.what());
+    }
+    auto cr = rfk::compile_hip(src, "rfk_flame", rfk::default_compile_flags());
+    if (log && log_cap) {
+        size_t n = std::min(log_cap - 1, cr.log.size());
+        std::memcpy(log, cr.log.data(), n);
+        log[n] = '\0';
+    }
+    if (!cr.ok) return fail(RFK_ERR_COMPILE, "hiprtc compile failed:\n" + cr.log);
+    k->compile_ms = cr.ms;
+    k->cache_hit = cr.cache_hit;
+    if (int rc = check_device()) return rc;
+    RFK_HIP(hipGetDevice(&k->device));
+    RFK_HIP(hipModuleLoadData(&k->mod, cr.code.data()));
+    RFK_HIP(hipModuleGetFunction(&k->f_iterate, k->mod, "rfk_iterate"));
+    RFK_HIP(hipModuleGetFunction(&k->f_precalc, k->mod, "rfk_precalc"));
+    *out = k.release();
+    return RFK_OK;
+}
+
+int rfk_compile_check(const rfk_db* db, const rfk_flame* f, int precision, const rfk_exec_config* cfg,
+                      size_t* code_bytes, char* log, size_t log_cap) {
+    if (!db || !f) return fail(RFK_ERR_INVALID, "null argument");
+    if (log && log_cap) log[0] = '\0';
+    rfk::compile_options o;
+    if (int rc = make_options(cfg, precision, o)) return rc;
+    std::string src;
+    try {
+        auto L = rfk::make_layout(f->f, db->db);
+        src = rfk::make_source(db->db, f->f, L, o);
+    } catch (const std::exception& e) {
+        return fail(RFK_ERR_PARSE, e.what());
+    }
+    auto cr = rfk::compile_hip(src, "rfk_flame", rfk::default_compile_flags());
+    if (log && log_cap) {
+        size_t n = std::min(log_cap - 1, cr.log.size());
+        std::memcpy(log, cr.log.data(), n);
+        log[n] = '\0';
+    }
+    if (!cr.ok) return fail(RFK_ERR_COMPILE, "hiprtc compile failed:\n" + cr.log);
+    if (code_bytes) *code_bytes = cr.code.size();
+    return RFK_OK;
+}
+
+int rfk_kernel_free(rfk_kernel* k) {
+    if (k && k->mod) (void)hipModuleUnload(k->mod);
+    delete k;
+    return RFK_OK;
+}
+
+int rfk_kernel_size_reals(const rfk_kernel* k, size_t* out) {
+    if (!k || !out) return fail(RFK_ERR_INVALID, "null argument");
+    *out = size_t(k->L.size_reals);
+    return RFK_OK;
+}
+
+int rfk_kernel_threads_per_block(const rfk_kernel* k, uint32_t* out) {
+    if (!k || !out) return fail(RFK_ERR_INVALID, "null argument");
+    *out = k->opt.tpb;
+    return RFK_OK;
+}
+
+double rfk_kernel_compile_ms(const rfk_kernel* k) { return k ? k->compile_ms : 0.0; }
+
+// ------------------------------------------------------------------ render
+
+int rfk_fix_rotation(double* samples, size_t sample_size, size_t flame_size, const size_t* affine_indices,
+                     size_t n_indices, int count) {
+    if (!samples || (!affine_indices && n_indices) || count <= 0 || flame_size > sample_size)
+        return fail(RFK_ERR_INVALID, "bad argument");
+    rfk::fix_rotation(samples, sample_size, flame_size, affine_indices, n_indices, count);
+    return RFK_OK;
+}
+
+int rfk_warmup(rfk_kernel* k, void* stream, const double* samples, size_t n_reals, uint32_t width, uint32_t height,
+               uint32_t seed, uint32_t fuse, uint32_t walkers, int temporal_multiplier, rfk_state** out) {
+    if (!k || !samples || !out) return fail(RFK_ERR_INVALID, "null argument");
+    if (width == 0 || height == 0 || width > 32768 || height > 32768) return fail(RFK_ERR_INVALID, "bad image size");
+    if (temporal_multiplier < 1 || temporal_multiplier > 64) return fail(RFK_ERR_INVALID, "bad temporal multiplier");
+    const size_t sample_size = size_t(k->L.size_reals) + 768;
+    if (n_reals % sample_size != 0 || n_reals / sample_size < 4)
+        return fail(RFK_ERR_INVALID, "samples must hold >= 4 packed samples of " + std::to_string(sample_size) + " reals");
+    if (int rc = check_device()) return rc;
+    const uint32_t nsamp = uint32_t(n_reals / sample_size);
+    const uint32_t nseg = nsamp - 3;
+
+    auto s = std::make_unique<rfk_state>();
+    s->k = k;
+    s->width = width;
+    s->height = height;
+    s->tpb = k->opt.tpb;
+    if (walkers == 0) walkers = 256u * 1024u;
+    uint64_t base_blocks = (uint64_t(walkers) + s->tpb - 1) / s->tpb;
+    uint64_t blocks = base_blocks * uint64_t(temporal_multiplier);
+    if (blocks * s->tpb > (1ull << 31)) return fail(RFK_ERR_INVALID, "too many walkers");
+    s->blocks = uint32_t(blocks);
+    s->walkers = uint32_t(blocks * s->tpb);
+    s->size_reals = uint32_t(k->L.size_reals);
+    s->precision = k->opt.precision;
+    const uint32_t align = s->precision ? 8u : 16u;  // 64-byte rows
+    s->param_stride = (s->size_reals + align - 1) / align * align;
+    s->seed = seed;
+    if (stream) {
+        s->stream = static_cast<hipStream_t>(stream);
+    } else {
+        RFK_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        s->own_stream = true;
+    }
+    int dev = 0;
+    RFK_HIP(hipGetDevice(&dev));
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+        s->wallclock_khz = double(khz);
+
+    rfk_state* S = s.get();
+    auto cleanup_on_error = [&](int rc) {
+        free_state_buffers(S);
+        S->own_stream = false;
+        return rc;
+    };
+    const size_t elem = s->precision ? sizeof(double) : sizeof(float);
+    const size_t npix = size_t(width) * height;
+#define RFK_HIPC(expr)                                                                                    \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) return cleanup_on_error(fail(RFK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+    RFK_HIPC(hipMalloc(&s->params, size_t(s->blocks) * s->param_stride * elem));
+    RFK_HIPC(dalloc(&s->palettes, size_t(s->blocks) * 256));
+    RFK_HIPC(dalloc(&s->wx, s->walkers));
+    RFK_HIPC(dalloc(&s->wy, s->walkers));
+    RFK_HIPC(dalloc(&s->wc, s->walkers));
+    if (k->L.chaos_base >= 0) RFK_HIPC(dalloc(&s->wlast, s->walkers));
+    RFK_HIPC(dalloc(&s->block_pass, s->blocks));
+    RFK_HIPC(dalloc(&s->hist, npix * 4));
+    RFK_HIPC(dalloc(&s->counters, 8));
+    RFK_HIPC(dalloc(&s->dstats, 98));
+    RFK_HIPC(hipHostMalloc(reinterpret_cast<void**>(&s->counters_host), 128 * sizeof(unsigned long long),
+                           hipHostMallocDefault));
+    RFK_HIPC(hipHostMalloc(reinterpret_cast<void**>(&s->stop_host), 64, hipHostMallocMapped | hipHostMallocCoherent));
+    *s->stop_host = 0;
+    RFK_HIPC(hipHostGetDevicePointer(reinterpret_cast<void**>(&s->stop_dev), s->stop_host, 0));
+    RFK_HIPC(hipEventCreate(&s->ev0));
+    RFK_HIPC(hipEventCreate(&s->ev1));
+    RFK_HIPC(hipMemsetAsync(s->hist, 0, npix * 16, s->stream));
+    RFK_HIPC(hipMemsetAsync(s->block_pass, 0, size_t(s->blocks) * 4, s->stream));
+    if (s->wlast) RFK_HIPC(hipMemsetAsync(s->wlast, 0xFF, size_t(s->walkers) * 4, s->stream));
+
+    // host fix_rotation, upload, Catmull-Rom, per-sample evaluation
+    std::vector<double> host(samples, samples + n_reals);
+    rfk::fix_rotation(host.data(), sample_size, size_t(k->L.size_reals), k->L.affine_indices.data(),
+                      k->L.affine_indices.size(), int(nsamp));
+    std::vector<int8_t> kind(size_t(k->L.size_reals), 0);
+    for (size_t a : k->L.affine_indices)
+        for (int c = 0; c < 4; c++) kind[a + size_t(c)] = int8_t(1 + c);
+    double *d_samples = nullptr, *d_segments = nullptr;
+    int8_t* d_kind = nullptr;
+    RFK_HIPC(hipMalloc(&d_samples, n_reals * sizeof(double)));
+    RFK_HIPC(hipMalloc(&d_segments, size_t(nseg) * sample_size * 4 * sizeof(double)));
+    RFK_HIPC(hipMalloc(&d_kind, kind.size() + 16));
+    RFK_HIPC(hipMemcpyAsync(d_samples, host.data(), n_reals * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    RFK_HIPC(hipMemcpyAsync(d_kind, kind.data(), kind.size(), hipMemcpyHostToDevice, s->stream));
+    RFK_HIPC(rfk::sk::catmull(d_samples, uint32_t(sample_size), nseg, d_segments, s->stream));
+    RFK_HIPC(rfk::sk::eval_samples(d_segments, uint32_t(sample_size), s->size_reals, nseg, s->blocks, d_kind,
+                                   s->precision, s->params, s->param_stride, s->palettes, s->stream));
+
+    // per-sample precalc + jitter (generated), then walker seeding and fuse passes
+    rfk_iter_args a = base_args(S);
+    {
+        void* argv[] = {&a};
+        uint32_t tb = 64, g = (s->blocks + tb - 1) / tb;
+        RFK_HIPC(hipModuleLaunchKernel(k->f_precalc, g, 1, 1, tb, 1, 1, 0, s->stream, argv, nullptr));
+    }
+    if (int rc = reset_counters(S)) return cleanup_on_error(rc);
+    a.flags = RFK_FLAG_INIT;
+    a.npasses = fuse;
+    if (int rc = launch_iterate(S, a)) return cleanup_on_error(rc);
+    RFK_HIPC(hipStreamSynchronize(s->stream));
+    (void)hipFree(d_samples);
+    (void)hipFree(d_segments);
+    (void)hipFree(d_kind);
+    s->passes_done = fuse;
+#undef RFK_HIPC
+    *out = s.release();
+    return RFK_OK;
+}
+
+int rfk_state_free(rfk_state* s) {
+    if (!s) return RFK_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_state_buffers(s);
+    delete s;
+    return RFK_OK;
+}
+
+int rfk_bin_async(rfk_kernel* k, rfk_state* s, const rfk_bailout* bo) {
+    if (!k || !s || !bo) return fail(RFK_ERR_INVALID, "null argument");
+    if (s->k != k) return fail(RFK_ERR_INVALID, "state belongs to another kernel");
+    if (s->pending) return fail(RFK_ERR_INVALID, "a bin call is already pending on this state");
+    const bool has_time = bo->millis > 0.0, has_q = bo->quality > 0.0;
+    if (bo->iters == 0 && !has_time && !has_q) return fail(RFK_ERR_INVALID, "no bailout set");
+    rfk_iter_args a = base_args(s);
+    uint64_t passes = bo->iters ? (bo->iters + s->walkers - 1) / s->walkers : 0;
+    if (passes == 0) passes = 0x7FFFFFFFull;  // bounded by the time / quality bailouts
+    if (passes > 0x7FFFFFFFull) return fail(RFK_ERR_INVALID, "iteration budget too large for one call");
+    a.npasses = uint32_t(passes);
+    a.flags = RFK_FLAG_DRAW;
+    if (has_time || has_q) a.flags |= RFK_FLAG_BAILOUT;
+    a.time_limit_ticks = has_time ? uint64_t(bo->millis * s->wallclock_khz) : 0;
+    a.quality_target = has_q ? uint64_t(bo->quality * double(s->width) * s->height * 255.0 / s->blocks) : 0;
+    *s->stop_host = 0;
+    if (int rc = reset_counters(s)) return rc;
+    RFK_HIP(hipEventRecord(s->ev0, s->stream));
+    if (int rc = launch_iterate(s, a)) return rc;
+    RFK_HIP(hipEventRecord(s->ev1, s->stream));
+    // density statistics after every bin, as the reference does (flame_compiler.cpp:928-937)
+    RFK_HIP(hipMemsetAsync(s->dstats, 0, 98 * sizeof(unsigned long long), s->stream));
+    RFK_HIP(rfk::sk::density_histogram(s->hist, size_t(s->width) * s->height, s->dstats, s->dstats + 97, s->stream));
+    RFK_HIP(hipMemcpyAsync(s->counters_host, s->counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           s->stream));
+    RFK_HIP(hipMemcpyAsync(s->counters_host + 8, s->dstats, 98 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           s->stream));
+    s->pending = true;
+    s->pending_passes = passes;
+    return RFK_OK;
+}
+
+int rfk_bin_wait(rfk_state* s, rfk_bin_result* out) {
+    if (!s || !out) return fail(RFK_ERR_INVALID, "null argument");
+    if (!s->pending) return fail(RFK_ERR_INVALID, "no bin call pending");
+    RFK_HIP(hipStreamSynchronize(s->stream));
+    s->pending = false;
+    const unsigned long long* c = s->counters_host;
+    const double bins = double(s->width) * s->height;
+    std::memset(out, 0, sizeof *out);
+    out->hit_sum = c[RFK_CNT_HITS];
+    out->quality = double(c[RFK_CNT_HITS]) / (bins * 255.0);
+    out->total_passes = c[RFK_CNT_PASSES];
+    out->total_draws = c[RFK_CNT_DRAWS];
+    out->bad_values = c[RFK_CNT_BAD];
+    out->total_bins = uint64_t(bins);
+    out->passes_per_thread = double(c[RFK_CNT_PASSES]) / s->walkers;
+    out->elapsed_ms = c[RFK_CNT_STOP] > c[RFK_CNT_START] ? double(c[RFK_CNT_STOP] - c[RFK_CNT_START]) / s->wallclock_khz : 0.0;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, s->ev0, s->ev1) == hipSuccess) out->kernel_ms = ms;
+    out->max_density = double(c[8 + 97]);
+    s->quality += out->quality;
+    s->passes_done += uint64_t(out->passes_per_thread + 0.5);
+    return RFK_OK;
+}
+
+int rfk_bin(rfk_kernel* k, rfk_state* s, const rfk_bailout* bo, rfk_bin_result* out) {
+    if (int rc = rfk_bin_async(k, s, bo)) return rc;
+    rfk_bin_result tmp;
+    int rc = rfk_bin_wait(s, out ? out : &tmp);
+    return rc;
+}
+
+int rfk_abort(rfk_state* s) {
+    if (!s || !s->stop_host) return fail(RFK_ERR_INVALID, "null state");
+    __atomic_store_n(s->stop_host, 1u, __ATOMIC_SEQ_CST);
+    return RFK_OK;
+}
+
+int rfk_bin_points(rfk_state* s, const float* xyco, size_t n) {
+    if (!s || (!xyco && n)) return fail(RFK_ERR_INVALID, "null argument");
+    if (s->precision != 0) return fail(RFK_ERR_INVALID, "bin_points needs an f32 state");
+    RFK_HIP(rfk::sk::bin_points(static_cast<const float*>(s->params), s->palettes, xyco, n, s->hist, s->width,
+                                s->height, s->stream));
+    RFK_HIP(hipStreamSynchronize(s->stream));
+    return RFK_OK;
+}
+
+int rfk_trace(rfk_kernel* k, rfk_state* s, uint32_t passes, float* trace, uint32_t* xform_ids) {
+    if (!k || !s || !trace) return fail(RFK_ERR_INVALID, "null argument");
+    if (s->k != k) return fail(RFK_ERR_INVALID, "state belongs to another kernel");
+    rfk_iter_args a = base_args(s);
+    a.npasses = passes;
+    a.flags = RFK_FLAG_TRACE;
+    a.trace = trace;
+    a.trace_xid = xform_ids;
+    if (int rc = reset_counters(s)) return rc;
+    if (int rc = launch_iterate(s, a)) return rc;
+    RFK_HIP(hipStreamSynchronize(s->stream));
+    s->passes_done += passes;
+    return RFK_OK;
+}
+
+int rfk_state_get_info(const rfk_state* s, rfk_state_info* out) {
+    if (!s || !out) return fail(RFK_ERR_INVALID, "null argument");
+    out->width = s->width;
+    out->height = s->height;
+    out->walkers = s->walkers;
+    out->threads_per_block = s->tpb;
+    out->blocks = s->blocks;
+    out->size_reals = s->size_reals;
+    out->param_stride = s->param_stride;
+    out->seed = s->seed;
+    out->passes_done = s->passes_done;
+    out->histogram = s->hist;
+    out->params = static_cast<float*>(s->params);
+    out->palettes = s->palettes;
+    out->walker_x = s->wx;
+    out->walker_y = s->wy;
+    out->walker_c = s->wc;
+    out->stream = s->stream;
+    out->quality = s->quality;
+    return RFK_OK;
+}
+
+int rfk_state_clear_histogram(rfk_state* s) {
+    if (!s) return fail(RFK_ERR_INVALID, "null state");
+    RFK_HIP(hipMemsetAsync(s->hist, 0, size_t(s->width) * s->height * 16, s->stream));
+    RFK_HIP(hipStreamSynchronize(s->stream));
+    s->quality = 0.0;
+    return RFK_OK;
+}
+
+// ------------------------------------------------------------ image kernels
+
+int rfk_density_histogram(rfk_state* s, uint64_t* bins97, double* max_density) {
+    if (!s) return fail(RFK_ERR_INVALID, "null state");
+    RFK_HIP(hipMemsetAsync(s->dstats, 0, 98 * sizeof(unsigned long long), s->stream));
+    RFK_HIP(rfk::sk::density_histogram(s->hist, size_t(s->width) * s->height, s->dstats, s->dstats + 97, s->stream));
+    RFK_HIP(hipMemcpyAsync(s->counters_host + 8, s->dstats, 98 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           s->stream));
+    RFK_HIP(hipStreamSynchronize(s->stream));
+    if (bins97) std::memcpy(bins97, s->counters_host + 8, 97 * sizeof(uint64_t));
+    if (max_density) *max_density = double(s->counters_host[8 + 97]);
+    return RFK_OK;
+}
+
+int rfk_density_estimate(rfk_state* s, float max_radius, float min_radius, float curve, float* out_rgba) {
+    if (!s || !out_rgba) return fail(RFK_ERR_INVALID, "null argument");
+    if (!(max_radius >= 0.0f) || max_radius > 20.0f || !(min_radius >= 0.0f) || min_radius > 20.0f || !(curve >= 0.0f))
+        return fail(RFK_ERR_INVALID, "radii must lie in [0, 20] and curve >= 0");
+    const float hmax = std::max(max_radius, min_radius);
+    const int nlevels = int(hmax * 16.0f + 0.5f) + 1;
+    const int radius = (nlevels - 1 + 15) / 16;
+    // per-level normalisation 1/Z(h), Z(h) = sum over |v| <= h of exp(-2|v|^2/h^2)
+    std::vector<float> inv_z(static_cast<size_t>(nlevels), 0.0f);
+    for (int l = 0; l < nlevels; l++) {
+        double h = l / 16.0, z = 0.0;
+        for (int dy = -radius; dy <= radius; dy++)
+            for (int dx = -radius; dx <= radius; dx++) {
+                double d2 = double(dx * dx + dy * dy);
+                if (d2 > h * h) continue;
+                z += d2 == 0.0 ? 1.0 : std::exp(-2.0 * d2 / (h * h));
+            }
+        inv_z[size_t(l)] = float(1.0 / z);
+    }
+    const size_t npix = size_t(s->width) * s->height;
+    float* d_iz = nullptr;
+    int* d_lvl = nullptr;
+    RFK_HIP(hipMalloc(&d_iz, inv_z.size() * sizeof(float)));
+    hipError_t e = hipMalloc(&d_lvl, npix * sizeof(int));
+    if (e != hipSuccess) {
+        (void)hipFree(d_iz);
+        return fail(RFK_ERR_HIP, hipGetErrorString(e));
+    }
+    e = hipMemcpyAsync(d_iz, inv_z.data(), inv_z.size() * sizeof(float), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = rfk::sk::density_estimate(s->hist, s->width, s->height, max_radius, min_radius, curve, d_iz, nlevels,
+                                      radius, d_lvl, out_rgba, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(d_iz);
+    (void)hipFree(d_lvl);
+    if (e != hipSuccess) return fail(RFK_ERR_HIP, hipGetErrorString(e));
+    return RFK_OK;
+}
+
+static int tonemap_common(const void* in, bool in_float, size_t npix, void* out, int fmt, const rfk_tonemap_args* a,
+                          double state_quality, hipStream_t stream) {
+    if (!in || !out || !a) return fail(RFK_ERR_INVALID, "null argument");
+    if (fmt < 0 || fmt > 4) return fail(RFK_ERR_INVALID, "bad pixel format");
+    double q = a->quality > 0 ? a->quality : state_quality;
+    if (!(q > 0)) return fail(RFK_ERR_INVALID, "quality must be positive (render first)");
+    // scale constant as tonemapper.cpp:52: 10^(-log10(q) - 0.5) in float
+    float k = std::pow(10.0f, -std::log10(float(q)) - 0.5f);
+    RFK_HIP(rfk::sk::tonemap(in, in_float, npix, out, fmt, k, float(a->brightness), float(a->gamma), float(a->vibrancy),
+                             stream));
+    RFK_HIP(hipStreamSynchronize(stream));
+    return RFK_OK;
+}
+
+int rfk_tonemap(rfk_state* s, void* out, int pixel_format, const rfk_tonemap_args* a) {
+    if (!s) return fail(RFK_ERR_INVALID, "null state");
+    return tonemap_common(s->hist, false, size_t(s->width) * s->height, out, pixel_format, a, s->quality, s->stream);
+}
+
+int rfk_tonemap_image(const float* in_rgba, uint32_t width, uint32_t height, void* out, int pixel_format,
+                      const rfk_tonemap_args* a, void* stream) {
+    return tonemap_common(in_rgba, true, size_t(width) * height, out, pixel_format, a, 0.0,
+                          static_cast<hipStream_t>(stream));
+}
+
+int rfk_catmull_coefficients(const double* samples, uint32_t sample_size, uint32_t nseg, double* segments,
+                             void* stream) {
+    if (!samples || !segments) return fail(RFK_ERR_INVALID, "null argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    RFK_HIP(rfk::sk::catmull(samples, sample_size, nseg, segments, st));
+    RFK_HIP(hipStreamSynchronize(st));
+    return RFK_OK;
+}
+
+int rfk_atomic_peak(uint64_t footprint_bytes, uint64_t increments, double* increments_per_s) {
+    if (!increments_per_s || footprint_bytes < 16) return fail(RFK_ERR_INVALID, "bad argument");
+    if (int rc = check_device()) return rc;
+    const uint64_t nbins = footprint_bytes / 16;
+    uint32_t* buf = nullptr;
+    RFK_HIP(hipMalloc(&buf, nbins * 16));
+    const uint32_t block = 256, grid = 256 * 32;
+    const uint64_t per_launch_threads = uint64_t(block) * grid;
+    uint32_t iters = uint32_t(std::max<uint64_t>(1, increments / 4 / per_launch_threads));
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    hipError_t e = hipMemset(buf, 0, nbins * 16);
+    if (e == hipSuccess) e = rfk::sk::atomic_peak(buf, nbins, grid, block, 1, 1u, nullptr);  // warm
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess) e = rfk::sk::atomic_peak(buf, nbins, grid, block, iters, 7u, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(RFK_ERR_HIP, hipGetErrorString(e));
+    *increments_per_s = double(per_launch_threads) * iters * 4.0 / (ms * 1e-3);
+    return RFK_OK;
+}
+
+// ------------------------------------------------------------ device helpers
+
+int rfk_device_count(int* n) {
+    if (!n) return fail(RFK_ERR_INVALID, "null argument");
+    *n = 0;
+    if (hipGetDeviceCount(n) != hipSuccess) *n = 0;
+    return RFK_OK;
+}
+int rfk_malloc(void** p, size_t bytes) {
+    RFK_HIP(hipMalloc(p, bytes));
+    return RFK_OK;
+}
+int rfk_free(void* p) {
+    RFK_HIP(hipFree(p));
+    return RFK_OK;
+}
+int rfk_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    RFK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return RFK_OK;
+}
+int rfk_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    RFK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return RFK_OK;
+}
+int rfk_memset(void* dst, int value, size_t bytes) {
+    RFK_HIP(hipMemset(dst, value, bytes));
+    return RFK_OK;
+}
+int rfk_synchronize(void) {
+    RFK_HIP(hipDeviceSynchronize());
+    return RFK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,374 @@
+// Kernels compiled ahead of time for gfx950. All are streaming kernels over
+// HBM-resident images except the per-sample evaluation.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "device/flame_device.h"
+#include "static_kernels.hpp"
+
+namespace rfk::sk {
+
+namespace {
+
+constexpr unsigned kBlock = 256;
+
+// --------------------------------------------------------------- Catmull-Rom
+// Centripetal-style knot spacing pow(hypot(dp, 1), alpha) with tension 0 and
+// alpha 0.5; coefficients of a t^3 + b t^2 + c t + d over [p1, p2].
+__device__ void catmull_segment(double p0, double p1, double p2, double p3, double tension, double alpha,
+                                double* out) {
+#pragma clang fp contract(off)
+    double t01 = pow(hypot(p1 - p0, 1.0), alpha);
+    double t12 = pow(hypot(p2 - p1, 1.0), alpha);
+    double t23 = pow(hypot(p3 - p2, 1.0), alpha);
+    double m1 = (1.0 - tension) * (p2 - p1 + t12 * ((p1 - p0) / t01 - (p2 - p0) / (t01 + t12)));
+    double m2 = (1.0 - tension) * (p2 - p1 + t12 * ((p3 - p2) / t23 - (p3 - p1) / (t12 + t23)));
+    out[0] = 2.0 * (p1 - p2) + m1 + m2;
+    out[1] = -3.0 * (p1 - p2) - m1 - m1 - m2;
+    out[2] = m1;
+    out[3] = p1;
+}
+
+__global__ void catmull_kernel(const double* __restrict__ samples, uint32_t sample_size, uint32_t nseg,
+                               double* __restrict__ segments) {
+#pragma clang fp contract(off)
+    const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= uint64_t(sample_size) * nseg) return;
+    const uint64_t idx = g;  // = seg * sample_size + param
+    double coef[4];
+    catmull_segment(samples[idx], samples[idx + sample_size], samples[idx + 2ull * sample_size],
+                    samples[idx + 3ull * sample_size], 0.0, 0.5, coef);
+    double4 o = {coef[0], coef[1], coef[2], coef[3]};
+    reinterpret_cast<double4*>(segments)[idx] = o;
+}
+
+// evaluated without FMA contraction so the host restatement reproduces the bits
+__device__ __forceinline__ double seg_eval(const double* seg4, double t) {
+#pragma clang fp contract(off)
+    return seg4[0] * t * t * t + seg4[1] * t * t + seg4[2] * t + seg4[3];
+}
+
+// The reference gives block b of an n-block grid t = (b % seg_size) / (n / nseg)
+// inside segment b / seg_size (refactor.cu:266-270).
+template <typename OutT>
+__global__ void eval_samples_kernel(const double* __restrict__ segments, uint32_t sample_size, uint32_t size_reals,
+                                    uint32_t nseg, uint32_t nsamples, const int8_t* __restrict__ kind,
+                                    OutT* __restrict__ params, uint32_t param_stride, uint32_t* __restrict__ palettes) {
+    const uint32_t per = size_reals + 256u;
+    const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= uint64_t(per) * nsamples) return;
+    const uint32_t s = uint32_t(g / per), i = uint32_t(g % per);
+    const uint32_t seg_size = max(nsamples / nseg, 1u);
+    uint32_t seg = s / seg_size;
+    if (seg >= nseg) seg = nseg - 1;
+    const double t = double(s % seg_size) / double(max(nsamples / nseg, 1u));
+    const double* base = segments + uint64_t(seg) * sample_size * 4ull;
+    if (i < size_reals) {
+        const int k = kind[i];
+        double v;
+        if (k == 0) {
+            v = seg_eval(base + 4ull * i, t);
+        } else {
+            // polar -> cartesian (refactor.cu:279-294): stored as
+            // (angle_x, angle_y, log|x|, log|y|, c, f)
+            const uint32_t a = i - uint32_t(k - 1);
+            const int comp = k - 1;  // 0:a 1:d 2:b 3:e
+            const int col = comp >> 1;
+            const double ang = seg_eval(base + 4ull * (a + col), t);
+            const double mag = exp(seg_eval(base + 4ull * (a + 2 + col), t));
+            v = (comp & 1) ? mag * sin(ang) : mag * cos(ang);
+        }
+        params[uint64_t(s) * param_stride + i] = OutT(v);
+    } else {
+        const uint32_t j = i - size_reals;
+        const double* pb = base + 4ull * (size_reals + 3u * j);
+        const double h = seg_eval(pb, t), sat = seg_eval(pb + 4, t), val = seg_eval(pb + 8, t);
+        // HSV -> RGB of refactor.cu:128-140, evaluated in double, uncontracted
+        auto f = [&](double n) {
+#pragma clang fp contract(off)
+            double k = fmod(n + h / 60.0, 6.0);
+            return val - val * sat * fmax(0.0, fmin(k, fmin(4.0 - k, 1.0)));
+        };
+        auto to8 = [](double c) -> uint32_t {
+            double v = c * 255.0;
+            if (!(v > 0.0)) return 0u;
+            if (v >= 255.0) return 255u;
+            return uint32_t(v);
+        };
+        palettes[uint64_t(s) * 256u + j] = to8(f(5.0)) | (to8(f(3.0)) << 8) | (to8(f(1.0)) << 16) | (255u << 24);
+    }
+}
+
+// ------------------------------------------------------------------ points
+__global__ void bin_points_kernel(const float* __restrict__ screen6, const uint32_t* __restrict__ palette,
+                                  const float4* __restrict__ pts, size_t n, uint32_t* __restrict__ hist,
+                                  uint32_t width, uint32_t height) {
+    __shared__ uint32_t s_pal[256];
+    for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) s_pal[i] = palette[i];
+    float S[6];
+    for (int k = 0; k < 6; k++) S[k] = screen6[k];
+    __syncthreads();
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const float4 p = pts[g];
+    rfk_draw_t d = rfk_make_draw(S, s_pal, width, height, p.x, p.y, p.z, p.w);
+    if (d.hit) rfk_commit_draw(hist, d);
+}
+
+// ------------------------------------------------------- density histogram
+__global__ void density_histogram_kernel(const uint4* __restrict__ hist, size_t npix,
+                                         unsigned long long* __restrict__ bins97,
+                                         unsigned long long* __restrict__ max_density) {
+    constexpr unsigned kBins = 4 * 24 + 1;
+    __shared__ unsigned long long s_bins[kBins];
+    __shared__ unsigned long long s_max;
+    for (unsigned i = threadIdx.x; i < kBins; i += blockDim.x) s_bins[i] = 0;
+    if (threadIdx.x == 0) s_max = 0;
+    __syncthreads();
+    const float log_div = 4.0f / logf(2.0f);
+    for (size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x; g < npix; g += size_t(gridDim.x) * blockDim.x) {
+        const float value = float(hist[g].w) * (1.0f / 255.0f);
+        if (value >= 1.0f) {
+            unsigned long long b = (unsigned long long)(logf(value) * log_div) + 1ull;
+            if (b >= kBins) b = kBins - 1;
+            atomicAdd(&s_bins[b], 1ull);
+            atomicMax(&s_max, (unsigned long long)value);
+        } else {
+            atomicAdd(&s_bins[0], 1ull);
+        }
+    }
+    __syncthreads();
+    for (unsigned i = threadIdx.x; i < kBins; i += blockDim.x)
+        if (s_bins[i]) atomicAdd(&bins97[i], s_bins[i]);
+    if (threadIdx.x == 0) atomicMax(max_density, s_max);
+}
+
+// ------------------------------------------------------ density estimation
+// Level of a source bin: h = max(min_r, max_r * max(n,1)^-curve), quantised
+// to 1/16 pixel; -1 for an empty bin.
+__global__ void de_levels_kernel(const uint4* __restrict__ hist, size_t npix, float max_radius, float min_radius,
+                                 float curve, int max_level, int* __restrict__ levels) {
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= npix) return;
+    const uint32_t w = hist[g].w;
+    if (w == 0u) {
+        levels[g] = -1;
+        return;
+    }
+    const float n = float(w) * (1.0f / 255.0f);
+    float h = max_radius * powf(fmaxf(n, 1.0f), -curve);
+    h = fmaxf(h, min_radius);
+    int lvl = int(h * 16.0f + 0.5f);
+    levels[g] = min(lvl, max_level);
+}
+
+template <int TILE>
+__global__ void de_gather_kernel(const uint4* __restrict__ hist, const int* __restrict__ levels, uint32_t width,
+                                 uint32_t height, int radius, const float* __restrict__ inv_z, int nlevels,
+                                 float4* __restrict__ out) {
+    extern __shared__ unsigned char smem[];
+    const int span = TILE + 2 * radius;
+    float4* s_col = reinterpret_cast<float4*>(smem);
+    int* s_lvl = reinterpret_cast<int*>(s_col + span * span);
+    float* s_iz = reinterpret_cast<float*>(s_lvl + span * span);
+    const int x0 = int(blockIdx.x) * TILE - radius, y0 = int(blockIdx.y) * TILE - radius;
+    const int tid = int(threadIdx.y) * TILE + int(threadIdx.x);
+    for (int i = tid; i < nlevels; i += TILE * TILE) s_iz[i] = inv_z[i];
+    for (int i = tid; i < span * span; i += TILE * TILE) {
+        const int xx = x0 + i % span, yy = y0 + i / span;
+        int lv = -1;
+        float4 c = {0.f, 0.f, 0.f, 0.f};
+        if (xx >= 0 && yy >= 0 && xx < int(width) && yy < int(height)) {
+            const size_t p = size_t(yy) * width + size_t(xx);
+            lv = levels[p];
+            if (lv >= 0) {
+                const uint4 h = hist[p];
+                const float k = 1.0f / 255.0f;
+                c = {float(h.x) * k, float(h.y) * k, float(h.z) * k, float(h.w) * k};
+            }
+        }
+        s_lvl[i] = lv;
+        s_col[i] = c;
+    }
+    __syncthreads();
+    const int ox = int(blockIdx.x) * TILE + int(threadIdx.x), oy = int(blockIdx.y) * TILE + int(threadIdx.y);
+    if (ox >= int(width) || oy >= int(height)) return;
+    float4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int dy = -radius; dy <= radius; dy++) {
+        const int row = (int(threadIdx.y) + radius + dy) * span + int(threadIdx.x) + radius;
+        for (int dx = -radius; dx <= radius; dx++) {
+            const int lv = s_lvl[row + dx];
+            if (lv < 0) continue;
+            const float h = float(lv) * (1.0f / 16.0f);
+            const float d2 = float(dx * dx + dy * dy);
+            if (d2 > h * h) continue;
+            const float wgt = (d2 == 0.0f ? 1.0f : __expf(-2.0f * d2 / (h * h))) * s_iz[lv];
+            const float4 c = s_col[row + dx];
+            acc.x = fmaf(c.x, wgt, acc.x);
+            acc.y = fmaf(c.y, wgt, acc.y);
+            acc.z = fmaf(c.z, wgt, acc.z);
+            acc.w = fmaf(c.w, wgt, acc.w);
+        }
+    }
+    out[size_t(oy) * width + size_t(ox)] = acc;
+}
+
+// ------------------------------------------------------------------ tonemap
+struct half3_t {
+    __half x, y, z;
+};
+struct half4_t {
+    __half x, y, z, w;
+};
+
+__device__ __forceinline__ float sat(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return a + t * (b - a); }
+
+template <bool InFloat, int Fmt>
+__global__ void tonemap_kernel(const void* __restrict__ in, size_t npix, void* __restrict__ out, float scale_constant,
+                               float brightness, float gamma, float vibrancy) {
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= npix) return;
+    float4 col;
+    if (InFloat) {
+        col = reinterpret_cast<const float4*>(in)[g];
+    } else {
+        const uint4 h = reinterpret_cast<const uint4*>(in)[g];
+        const float k = 1.0f / 255.0f;
+        col = {float(h.x) * k, float(h.y) * k, float(h.z) * k, float(h.w) * k};
+    }
+    float4 o = {0.f, 0.f, 0.f, 0.f};
+    if (col.w != 0.0f) {
+        const float input_density = col.w;
+        const float factor = 0.5f * brightness * logf(1.0f + col.w * scale_constant) * 0.434294481903251827651128918916f / col.w;
+        col.x *= factor;
+        col.y *= factor;
+        col.z *= factor;
+        col.w *= factor;
+        const double inv_gamma = 1.0 / double(gamma);
+        const double z = pow(double(col.w), inv_gamma);
+        const double gamma_factor = z / double(col.w);
+        const float ig = float(inv_gamma);
+        o.x = sat(lerpf(powf(col.x, ig), float(double(col.x) * gamma_factor), vibrancy));
+        o.y = sat(lerpf(powf(col.y, ig), float(double(col.y) * gamma_factor), vibrancy));
+        o.z = sat(lerpf(powf(col.z, ig), float(double(col.z) * gamma_factor), vibrancy));
+        o.w = sat(logf(input_density) * (4.0f / logf(2.0f)) / 96.0f);
+    }
+    if (Fmt == 0) {
+        reinterpret_cast<float4*>(out)[g] = o;
+    } else if (Fmt == 1) {
+        reinterpret_cast<half4_t*>(out)[g] = {__float2half(o.x), __float2half(o.y), __float2half(o.z), __float2half(o.w)};
+    } else if (Fmt == 2) {
+        float* f = reinterpret_cast<float*>(out) + 3 * g;
+        f[0] = o.x;
+        f[1] = o.y;
+        f[2] = o.z;
+    } else if (Fmt == 3) {
+        reinterpret_cast<half3_t*>(out)[g] = {__float2half(o.x), __float2half(o.y), __float2half(o.z)};
+    } else {
+        auto q = [](float v) { return uint32_t(fminf(fmaxf(v * 255.0f + 0.5f, 0.0f), 255.0f)); };
+        reinterpret_cast<uint32_t*>(out)[g] = q(o.x) | (q(o.y) << 8) | (q(o.z) << 16) | (q(o.w) << 24);
+    }
+}
+
+// --------------------------------------------------------------- atomics
+__global__ void atomic_peak_kernel(uint32_t* __restrict__ buf, uint64_t nbins, uint32_t iters, uint32_t seed) {
+    uint32_t h = (blockIdx.x * blockDim.x + threadIdx.x) * 0x9E3779B1u ^ seed;
+    for (uint32_t i = 0; i < iters; i++) {
+        h ^= h >> 16;
+        h *= 0x7FEB352Du;
+        h ^= h >> 15;
+        h *= 0x846CA68Bu;
+        h ^= h >> 16;
+        const uint64_t bin = (uint64_t(h) * nbins) >> 32;
+        uint32_t* p = buf + 4ull * bin;
+        atomicAdd(p + 0, 1u);
+        atomicAdd(p + 1, 1u);
+        atomicAdd(p + 2, 1u);
+        atomicAdd(p + 3, 1u);
+        h += i;
+    }
+}
+
+unsigned nblocks(size_t n, unsigned b = kBlock) { return unsigned((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t catmull(const double* samples, uint32_t sample_size, uint32_t nseg, double* segments, hipStream_t s) {
+    size_t n = size_t(sample_size) * nseg;
+    if (!n) return hipSuccess;
+    catmull_kernel<<<nblocks(n), kBlock, 0, s>>>(samples, sample_size, nseg, segments);
+    return hipGetLastError();
+}
+
+hipError_t eval_samples(const double* segments, uint32_t sample_size, uint32_t size_reals, uint32_t nseg,
+                        uint32_t nsamples, const int8_t* kind, int precision, void* params, uint32_t param_stride,
+                        uint32_t* palettes, hipStream_t s) {
+    size_t n = size_t(size_reals + 256u) * nsamples;
+    if (precision == 1)
+        eval_samples_kernel<double><<<nblocks(n), kBlock, 0, s>>>(segments, sample_size, size_reals, nseg, nsamples,
+                                                                   kind, (double*)params, param_stride, palettes);
+    else
+        eval_samples_kernel<float><<<nblocks(n), kBlock, 0, s>>>(segments, sample_size, size_reals, nseg, nsamples,
+                                                                  kind, (float*)params, param_stride, palettes);
+    return hipGetLastError();
+}
+
+hipError_t bin_points(const float* screen6, const uint32_t* palette, const float* pts, size_t n, uint32_t* hist,
+                      uint32_t width, uint32_t height, hipStream_t s) {
+    if (!n) return hipSuccess;
+    bin_points_kernel<<<nblocks(n), kBlock, 0, s>>>(screen6, palette, reinterpret_cast<const float4*>(pts), n, hist,
+                                                     width, height);
+    return hipGetLastError();
+}
+
+hipError_t density_histogram(const uint32_t* hist, size_t npix, unsigned long long* bins97,
+                             unsigned long long* max_density, hipStream_t s) {
+    unsigned grid = std::min(nblocks(npix), 2048u);
+    if (grid == 0) grid = 1;
+    density_histogram_kernel<<<grid, kBlock, 0, s>>>(reinterpret_cast<const uint4*>(hist), npix, bins97, max_density);
+    return hipGetLastError();
+}
+
+hipError_t density_estimate(const uint32_t* hist, uint32_t width, uint32_t height, float max_radius, float min_radius,
+                            float curve, const float* inv_z, int nlevels, int radius, int* level_scratch,
+                            float* out_rgba, hipStream_t s) {
+    size_t npix = size_t(width) * height;
+    de_levels_kernel<<<nblocks(npix), kBlock, 0, s>>>(reinterpret_cast<const uint4*>(hist), npix, max_radius,
+                                                      min_radius, curve, nlevels - 1, level_scratch);
+    constexpr int T = 16;
+    const int span = T + 2 * radius;
+    size_t shmem = size_t(span) * span * (sizeof(float4) + sizeof(int)) + size_t(nlevels) * sizeof(float);
+    dim3 grid((width + T - 1) / T, (height + T - 1) / T);
+    de_gather_kernel<T><<<grid, dim3(T, T), shmem, s>>>(reinterpret_cast<const uint4*>(hist), level_scratch, width,
+                                                        height, radius, inv_z, nlevels,
+                                                        reinterpret_cast<float4*>(out_rgba));
+    return hipGetLastError();
+}
+
+hipError_t tonemap(const void* in, bool in_float, size_t npix, void* out, int format, float scale_constant,
+                   float brightness, float gamma, float vibrancy, hipStream_t s) {
+    if (!npix) return hipSuccess;
+    unsigned g = nblocks(npix);
+#define RFK_TM(F)                                                                                              \
+    if (in_float) tonemap_kernel<true, F><<<g, kBlock, 0, s>>>(in, npix, out, scale_constant, brightness, gamma, vibrancy); \
+    else tonemap_kernel<false, F><<<g, kBlock, 0, s>>>(in, npix, out, scale_constant, brightness, gamma, vibrancy);
+    switch (format) {
+        case 0: RFK_TM(0) break;
+        case 1: RFK_TM(1) break;
+        case 2: RFK_TM(2) break;
+        case 3: RFK_TM(3) break;
+        case 4: RFK_TM(4) break;
+        default: return hipErrorInvalidValue;
+    }
+#undef RFK_TM
+    return hipGetLastError();
+}
+
+hipError_t atomic_peak(uint32_t* buf, uint64_t nbins, uint32_t grid, uint32_t block, uint32_t iters, uint32_t seed,
+                       hipStream_t s) {
+    atomic_peak_kernel<<<grid, block, 0, s>>>(buf, nbins, iters, seed);
+    return hipGetLastError();
+}
+
+}  // namespace rfk::sk

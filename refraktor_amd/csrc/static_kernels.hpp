@@ -1,0 +1,44 @@
+// Host launchers of the kernels compiled ahead of time into the library.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace rfk::sk {
+
+// Catmull-Rom segment coefficients (assets/kernels/catmull.cu:11-42).
+hipError_t catmull(const double* samples, uint32_t sample_size, uint32_t nseg, double* segments, hipStream_t s);
+
+// Evaluate the cubic segments for every sample (block) at its own t, convert
+// polar affines back to cartesian and build the RGBA8 palette
+// (refactor.cu:105-151, 256-294). kind[i]: 0 plain, 1..4 component of an
+// affine whose first real sits at i-(kind-1).
+hipError_t eval_samples(const double* segments, uint32_t sample_size, uint32_t size_reals, uint32_t nseg,
+                        uint32_t nsamples, const int8_t* kind, int precision, void* params, uint32_t param_stride,
+                        uint32_t* palettes, hipStream_t s);
+
+// Bin n plane-space points through a screen transform / palette.
+hipError_t bin_points(const float* screen6, const uint32_t* palette, const float* pts, size_t n, uint32_t* hist,
+                      uint32_t width, uint32_t height, hipStream_t s);
+
+// calculate_histogram<4> (density_histo.cu:1-41) over the u32 histogram.
+hipError_t density_histogram(const uint32_t* hist, size_t npix, unsigned long long* bins97,
+                             unsigned long long* max_density, hipStream_t s);
+
+// Density estimation: levels + gather. inv_z has `nlevels` entries.
+hipError_t density_estimate(const uint32_t* hist, uint32_t width, uint32_t height, float max_radius, float min_radius,
+                            float curve, const float* inv_z, int nlevels, int radius, int* level_scratch,
+                            float* out_rgba, hipStream_t s);
+
+// Tonemap (tonemap.cu:50-105). Input: u32 RGBA fixed point (in_float=false)
+// or float4 draw units (in_float=true).
+hipError_t tonemap(const void* in, bool in_float, size_t npix, void* out, int format, float scale_constant,
+                   float brightness, float gamma, float vibrancy, hipStream_t s);
+
+// Random RGBA atomic increments for the binning roofline.
+hipError_t atomic_peak(uint32_t* buf, uint64_t nbins, uint32_t grid, uint32_t block, uint32_t iters, uint32_t seed,
+                       hipStream_t s);
+
+}  // namespace rfk::sk

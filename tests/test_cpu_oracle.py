@@ -1,0 +1,199 @@
+"""Pins the oracle itself: published vectors, closed forms and invariants."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from refraktor_amd import flame as fl
+from refraktor_amd import render
+
+# Random123 known-answer vectors for Philox4x32-10 (ctr, key, output)
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,out", PHILOX_KAT)
+def test_philox_kat(ctr, key, out):
+    assert tuple(oracle.c_philox(ctr, key)) == out
+    assert tuple(oracle.philox(np.array([ctr]), *key)[0]) == out
+
+
+def test_philox_numpy_matches_c():
+    rng = np.random.default_rng(1)
+    ctr = rng.integers(0, 2**32, size=(64, 4), dtype=np.uint64).astype(np.uint32)
+    key = (123456789, 987654321)
+    v = oracle.philox(ctr, *key)
+    for i in range(64):
+        assert np.array_equal(v[i], oracle.c_philox(ctr[i], key))
+
+
+def _identity_flame(weights=(1.0,), variation="linear"):
+    xfs = [fl.xform(w, 0.0, 0.0, [fl.vlink([1, 0, 0, 1, 0, 0], {variation: 1.0})]) for w in weights]
+    pal = [[0.0, 0.0, 1.0]] * 256
+    return fl.make_flame(xfs, pal)
+
+
+def _prepare(f, blocks=4, w=64, h=64, seed=1):
+    return oracle.prepare(f, render.CONFIG_DIR, w, h, blocks, seed)
+
+
+def test_permutation_is_a_bijection():
+    """identity maps: one pass only permutes points within each block"""
+    f = _identity_flame()
+    L, P, pal = _prepare(f)
+    n = 4 * 256
+    x = np.arange(n, dtype=np.float64)
+    y = -x
+    c = np.zeros(n)
+    tr, _ = oracle.trace(L, P, P.shape[1], 4, 256, 1, 0, 3, x, y, c)
+    # trace holds each thread's output before the permutation; the outputs of
+    # pass 2 must be a permutation of the block's inputs
+    for b in range(4):
+        sl = slice(b * 256, (b + 1) * 256)
+        # polar interpolation leaves ~1e-16 in the off-diagonal affine terms
+        np.testing.assert_allclose(np.sort(tr[2, sl, 0]), np.sort(x[sl]), atol=1e-9)
+    assert not np.array_equal(tr[2, :, 0], x), "points should move between threads"
+
+
+def test_selection_frequencies_follow_weights():
+    w = (0.2, 0.3, 0.5)
+    f = _identity_flame(w)
+    L, P, pal = _prepare(f, blocks=64)
+    n = 64 * 256
+    z = np.zeros(n)
+    _, xid = oracle.trace(L, P, P.shape[1], 64, 256, 5, 0, 200, z, z, z)
+    counts = np.bincount(xid.ravel(), minlength=3) / xid.size
+    np.testing.assert_allclose(counts, w, atol=0.01)
+
+
+def test_zero_weight_xform_never_chosen():
+    f = _identity_flame((0.5, 0.0, 0.5, 0.0))
+    L, P, pal = _prepare(f, blocks=16)
+    z = np.zeros(16 * 256)
+    _, xid = oracle.trace(L, P, P.shape[1], 16, 256, 5, 0, 100, z, z, z)
+    assert set(np.unique(xid)) <= {0, 2}
+
+
+def test_last_xform_fallback():
+    """when the last weight is zero, a ratio past the cumulative sum picks the last non-zero"""
+    f = _identity_flame((1.0, 0.0))
+    L, P, pal = _prepare(f, blocks=4)
+    z = np.zeros(4 * 256)
+    _, xid = oracle.trace(L, P, P.shape[1], 4, 256, 5, 0, 50, z, z, z)
+    assert set(np.unique(xid)) == {0}
+
+
+def test_hammersley_seeding_closed_form():
+    f = _identity_flame()
+    w = h = 64
+    L, P, pal = _prepare(f, blocks=4, w=w, h=h)
+    x, y, c = oracle.seed_walkers(P, P.shape[1], 4, 256, w, h, 3)
+    n = 1024
+    i = np.arange(n)
+    rev = np.array([int(f"{v:010b}"[::-1], 2) for v in i])
+    hx, hy = i / n * 2 - 1, rev / n * 2 - 1
+    px, py = (hx + 1) / 2 * w, (hy + 1) / 2 * h
+    ps = oracle.plane_space(f, w, h)
+    np.testing.assert_allclose(x, ps[0] * px + ps[2] * py + ps[4], rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(y, ps[1] * px + ps[3] * py + ps[5], rtol=1e-6, atol=1e-6)
+    assert np.all((c >= 0) & (c <= 1))
+
+
+def test_plane_and_screen_are_inverse():
+    f = fl.flame_1080(2)
+    f["rotate"] = 33.0
+    f["center_x"], f["center_y"] = 0.3, -0.2
+    s, p = oracle.screen_space(f, 1920, 1080), oracle.plane_space(f, 1920, 1080)
+    pts = np.array([[0, 0], [1920, 1080], [100, 700]], dtype=np.float64)
+    for px, py in pts:
+        qx = p[0] * px + p[2] * py + p[4]
+        qy = p[1] * px + p[3] * py + p[5]
+        sx = s[0] * qx + s[2] * qy + s[4]
+        sy = s[1] * qx + s[3] * qy + s[5]
+        assert abs(sx - px) < 1e-9 and abs(sy - py) < 1e-9
+
+
+def test_bin_points_closed_form():
+    """hand-computed increments: palette lerp, rounding, bounds and the -0 column"""
+    S = np.array([1, 0, 0, 1, 0, 0], dtype=np.float32)  # plane == pixel coords
+    pal = np.arange(256, dtype=np.uint32) * 0x010101      # grey ramp, entry i = (i, i, i)
+    pts = np.array([[3.5, 2.25, 0.0, 1.0],   # pixel (3,2): palette[0], W += 255
+                    [3.9, 2.0, 1.0, 0.5],    # pixel (3,2): 255*0.5 -> 128 per channel, W += 128
+                    [-0.5, 0.0, 0.5, 1.0],   # trunc(-0.5) = -0 -> column 0; lerp(127,128,0.5)=127.5 -> 128
+                    [10.0, 0.0, 0.0, 1.0],   # x = width: outside
+                    [np.nan, 0, 0, 1],       # non-finite: no draw
+                    [0.0, 0.0, 0.0, 0.0]],   # zero opacity: no draw
+                   dtype=np.float32)
+    hst = oracle.bin_points(S, pal, pts, 10, 5)
+    assert tuple(hst[2, 3]) == (128, 128, 128, 383)
+    assert tuple(hst[0, 0]) == (128, 128, 128, 255)
+    assert int(hst.astype(np.int64).sum()) == 128 * 3 + 383 + 128 * 3 + 255
+
+
+def test_catmull_invariants():
+    ss = 5
+    # constant samples -> constant segment
+    s = np.tile(np.array([1.5, -2.0, 0.0, 3.0, 7.0]), 4)
+    seg = oracle.catmull(s, ss)
+    assert np.allclose(seg[0, :, :3], 0) and np.allclose(seg[0, :, 3], s[:ss])
+    # equally spaced collinear samples -> linear in t
+    s = np.concatenate([np.full(ss, v) for v in (0.0, 1.0, 2.0, 3.0)])
+    seg = oracle.catmull(s, ss)[0]
+    for t in (0.0, 0.25, 0.5, 1.0):
+        val = seg[:, 0] * t ** 3 + seg[:, 1] * t ** 2 + seg[:, 2] * t + seg[:, 3]
+        np.testing.assert_allclose(val, 1.0 + t, atol=1e-12)
+    # interpolating: p1 at t = 0 and p2 at t = 1
+    rng = np.random.default_rng(2)
+    s = rng.normal(size=4 * ss)
+    seg = oracle.catmull(s, ss)[0]
+    np.testing.assert_allclose(seg[:, 3], s[ss:2 * ss])
+    np.testing.assert_allclose(seg.sum(axis=1), s[2 * ss:3 * ss], atol=1e-12)
+
+
+def test_tonemap_special_cases():
+    hst = np.zeros((1, 3, 4), dtype=np.uint32)
+    hst[0, 1] = (255 * 40, 255 * 20, 255 * 10, 255 * 50)
+    hst[0, 2] = (255 * 3, 255 * 3, 255 * 3, 255 * 3)
+    q, g, b = 2.0, 2.2, 4.0
+    out0 = oracle.tonemap(hst, q, g, b, 0.0)
+    out1 = oracle.tonemap(hst, q, g, b, 1.0)
+    assert np.all(out0[0, 0] == 0) and np.all(out1[0, 0] == 0)
+    k = 10 ** (-math.log10(q) - 0.5)
+    w = 50.0
+    fac = 0.5 * b * math.log10(1 + w * k) / w
+    c = np.array([40.0, 20.0, 10.0]) * fac
+    np.testing.assert_allclose(out0[0, 1, :3], np.clip(c ** (1 / g), 0, 1))          # vibrancy 0: c^(1/gamma)
+    np.testing.assert_allclose(out1[0, 1, :3], np.clip(c * (w * fac) ** (1 / g - 1), 0, 1))  # vibrancy 1
+    assert abs(out1[0, 1, 3] - math.log2(50.0) / 24.0) < 1e-12
+
+
+def test_density_estimate_radius_zero_is_identity():
+    rng = np.random.default_rng(4)
+    hst = rng.integers(0, 2000, size=(12, 15, 4)).astype(np.uint32)
+    hst[rng.random((12, 15)) < 0.3] = 0
+    out = oracle.density_estimate(hst, 0.0, 0.0, 0.4)
+    np.testing.assert_allclose(out, hst / 255.0, atol=1e-12)
+
+
+def test_density_estimate_conserves_energy():
+    hst = np.zeros((40, 40, 4), dtype=np.uint32)
+    hst[20, 20] = (255, 510, 765, 255)       # density 1 -> widest kernel
+    hst[10, 25] = (2550, 0, 0, 255 * 1000)   # dense bin -> narrow kernel
+    out = oracle.density_estimate(hst, 6.0, 0.0, 0.4)
+    np.testing.assert_allclose(out.sum(axis=(0, 1)), hst.sum(axis=(0, 1)) / 255.0, rtol=1e-12)
+    # the spread of a density-1 bin is symmetric and peaks at its centre
+    blob = out[14:27, 14:27, 3]
+    assert np.allclose(blob, blob[::-1, ::-1]) and blob.argmax() == 6 * 13 + 6
+
+
+def test_oracle_render_counts():
+    f = fl.flame_small(2)
+    L, P, pal = _prepare(f, blocks=8, w=80, h=45)
+    hst, draws = oracle.render(L, P, P.shape[1], pal, 8, 256, 80, 45, 2, 5, 32)
+    assert draws <= 8 * 256 * 32
+    assert int(hst[..., 3].astype(np.int64).sum()) == draws * 255  # opacity 1 everywhere
