@@ -229,6 +229,45 @@ int rfk_catmull_coefficients(const double* samples, uint32_t sample_size, uint32
 int rfk_fix_rotation(double* samples /* host */, size_t sample_size, size_t flame_size, const size_t* affine_indices,
                      size_t n_indices, int count);
 
+/* ------------------------------------------------------------ denoiser --
+ * rfkt::denoiser (src/librefrakt/include/librefrakt/interface/denoiser.h:21-39,
+ * factory.h:52): the reference wraps OIDN / OptiX binaries; this one is a
+ * 4-level U-Net (3x3 convs, 32/64/96/128 channels, BF16 on the matrix cores,
+ * FP32 accumulation) that predicts an RGB residual over the tonemapped
+ * low-iteration frame, trained on the GPU by the same library.
+ * Shapes: N images of H x W, H and W multiples of 8; features are BF16 NHWC
+ * with 8 channels, base / target / output are FP32 RGB (N*H*W*3).        */
+typedef struct rfk_denoiser rfk_denoiser;
+int rfk_denoiser_create(uint64_t seed, rfk_denoiser** out);
+int rfk_denoiser_free(rfk_denoiser* d);
+int rfk_denoiser_param_count(const rfk_denoiser* d, size_t* weights, size_t* biases);
+int rfk_denoiser_save(const rfk_denoiser* d, const char* path);
+int rfk_denoiser_load(rfk_denoiser* d, const char* path);
+int rfk_denoiser_get_params(const rfk_denoiser* d, float* w /* host */, float* b /* host */);
+int rfk_denoiser_set_params(rfk_denoiser* d, const float* w /* host */, const float* b /* host */);
+int rfk_denoiser_get_grads(const rfk_denoiser* d, float* gw /* host */, float* gb /* host */);
+/* features + base image from a fixed-point histogram and its float4 tonemap */
+int rfk_denoiser_features(const uint32_t* hist, const float* tonemapped_rgba, size_t npix, void* feat, float* base,
+                          void* stream);
+/* out_rgb = clamp(base + U-Net(feat), 0, 1) */
+int rfk_denoiser_forward(rfk_denoiser* d, const void* feat, const float* base, int N, int H, int W, float* out_rgb,
+                         void* stream);
+/* one Adam step on mean squared error vs target; *loss = the batch MSE */
+int rfk_denoiser_train_step(rfk_denoiser* d, const void* feat, const float* base, const float* target, int N, int H,
+                            int W, float lr, float* loss, void* stream);
+/* single layers (gradient checks): y = act(conv3x3(concat(x1, x2)) + bias),
+ * dw = dL/dW for dL/dy = dy, dx = dL/dx. act: 0 none, 1 ReLU.             */
+int rfk_nn_conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N, int H, int W, const void* wgt,
+                           const float* bias, int Cout, int act, void* y, float* yf, void* stream);
+int rfk_nn_conv3x3_wgrad(const void* x1, int c1, const void* x2, int c2, const void* dy, int Cout, int N, int H,
+                         int W, float* dw, void* stream);
+int rfk_nn_conv3x3_dgrad(const void* dy, int Cout, int N, int H, int W, const void* wgt, int Cin, void* dx,
+                         void* stream);
+/* training crops: n crops of ch x cw pixels starting at (image, y, x) triples
+ * (host array) from a device stack of H x W images of pixel_bytes each      */
+int rfk_nn_gather_crops(const void* src, int H, int W, int pixel_bytes, const int* offsets, int n, int ch, int cw,
+                        void* dst, void* stream);
+
 /* Microbenchmark of uniform-random RGBA u32 atomic increments into a buffer of
  * `footprint_bytes`; returns increments per second (the binning roofline).  */
 int rfk_atomic_peak(uint64_t footprint_bytes, uint64_t increments, double* increments_per_s);
@@ -239,6 +278,7 @@ int rfk_malloc(void** p, size_t bytes);
 int rfk_free(void* p);
 int rfk_memcpy_htod(void* dst, const void* src, size_t bytes);
 int rfk_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int rfk_memcpy_dtod(void* dst, const void* src, size_t bytes);
 int rfk_memset(void* dst, int value, size_t bytes);
 int rfk_synchronize(void);
 

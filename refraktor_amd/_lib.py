@@ -104,12 +104,30 @@ _SIGS = [
     ("rfk_catmull_coefficients", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     ("rfk_fix_rotation", C.c_int, [C.POINTER(C.c_double), C.c_size_t, C.c_size_t, C.POINTER(C.c_size_t),
                                    C.c_size_t, C.c_int]),
+    ("rfk_denoiser_create", C.c_int, [C.c_uint64, C.POINTER(_P)]),
+    ("rfk_denoiser_free", C.c_int, [_P]),
+    ("rfk_denoiser_param_count", C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    ("rfk_denoiser_save", C.c_int, [_P, C.c_char_p]),
+    ("rfk_denoiser_load", C.c_int, [_P, C.c_char_p]),
+    ("rfk_denoiser_get_params", C.c_int, [_P, _P, _P]),
+    ("rfk_denoiser_set_params", C.c_int, [_P, _P, _P]),
+    ("rfk_denoiser_get_grads", C.c_int, [_P, _P, _P]),
+    ("rfk_denoiser_features", C.c_int, [_P, _P, C.c_size_t, _P, _P, _P]),
+    ("rfk_denoiser_forward", C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
+    ("rfk_denoiser_train_step", C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_float,
+                                          C.POINTER(C.c_float), _P]),
+    ("rfk_nn_conv3x3_forward", C.c_int, [_P, C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int,
+                                         C.c_int, _P, _P, _P]),
+    ("rfk_nn_conv3x3_wgrad", C.c_int, [_P, C.c_int, _P, C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P]),
+    ("rfk_nn_conv3x3_dgrad", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, C.c_int, _P, _P]),
+    ("rfk_nn_gather_crops", C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
     ("rfk_atomic_peak", C.c_int, [C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]),
     ("rfk_device_count", C.c_int, [C.POINTER(C.c_int)]),
     ("rfk_malloc", C.c_int, [C.POINTER(_P), C.c_size_t]),
     ("rfk_free", C.c_int, [_P]),
     ("rfk_memcpy_htod", C.c_int, [_P, _P, C.c_size_t]),
     ("rfk_memcpy_dtoh", C.c_int, [_P, _P, C.c_size_t]),
+    ("rfk_memcpy_dtod", C.c_int, [_P, _P, C.c_size_t]),
     ("rfk_memset", C.c_int, [_P, C.c_int, C.c_size_t]),
     ("rfk_synchronize", C.c_int, []),
 ]

@@ -77,6 +77,15 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+}  // namespace
+
+namespace rfk {
+// shared by every translation unit of the library (rfk_last_error reports it)
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace rfk
+
+namespace {
+
 #define RFK_HIP(expr)                                                                          \
     do {                                                                                       \
         hipError_t e_ = (expr);                                                                \
@@ -769,6 +778,10 @@ int rfk_memcpy_htod(void* dst, const void* src, size_t bytes) {
 }
 int rfk_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
     RFK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return RFK_OK;
+}
+int rfk_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+    RFK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return RFK_OK;
 }
 int rfk_memset(void* dst, int value, size_t bytes) {

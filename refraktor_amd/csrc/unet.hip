@@ -1,0 +1,545 @@
+// Denoiser kernels: 3x3 "same" convolutions on BF16 NHWC activations with
+// FP32 accumulation on the gfx950 matrix cores (v_mfma_f32_32x32x16_bf16), and
+// the small streaming kernels a U-Net needs (pooling, upsampling, activation
+// gradients, loss, Adam, feature preparation). The librefrakt denoisers are
+// third-party binaries (OIDN / OptiX); this network has no reference
+// counterpart and is checked against the oracle's float64 convolution.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "unet_kernels.hpp"
+
+namespace rfk::nn {
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int KC = 32;       // input channels staged per step
+constexpr int PAD = 8;       // LDS row padding (bf16) against bank conflicts
+constexpr int TPX = 64;      // output pixels per workgroup (one row segment)
+constexpr int TCO = 64;      // output channels per workgroup
+
+__device__ __forceinline__ float act_fwd(float v, int act, float slope) {
+    if (act == 1) return v > 0.f ? v : 0.f;
+    if (act == 2) return v > 0.f ? v : v * slope;
+    return v;
+}
+
+__device__ __forceinline__ bf16x8 load8(const bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+__device__ __forceinline__ bf16x8 zero8() {
+    bf16x8 z;
+#pragma unroll
+    for (int i = 0; i < 8; i++) z[i] = bf16(0.0f);
+    return z;
+}
+
+// ------------------------------------------------------------- forward
+// Output tile: 64 consecutive pixels of one row x 64 output channels; wave w
+// computes pixels [32*(w&1), +32) x channels [32*(w>>1), +32) as one 32x32
+// MFMA accumulator. Input = virtual concat of x1 (c1 channels) and x2 (c2).
+__global__ __launch_bounds__(256) void conv3x3_fwd_kernel(const bf16* __restrict__ x1, int c1,
+                                                          const bf16* __restrict__ x2, int c2, int N, int H, int W,
+                                                          const bf16* __restrict__ wgt, const float* __restrict__ bias,
+                                                          int Cout, int act, float slope, bf16* __restrict__ y,
+                                                          float* __restrict__ yf) {
+    __shared__ __attribute__((aligned(16))) bf16 s_in[3][TPX + 2][KC + PAD];
+    __shared__ __attribute__((aligned(16))) bf16 s_w[9][TCO][KC + PAD];
+    const int Cin = c1 + c2;
+    const int x0 = int(blockIdx.x) * TPX;
+    const int yy = int(blockIdx.y);
+    const int ntile = (Cout + TCO - 1) / TCO;
+    const int n = int(blockIdx.z) / ntile;
+    const int co0 = (int(blockIdx.z) % ntile) * TCO;
+    const int t = int(threadIdx.x);
+    const int wv = t >> 6, l = t & 63;
+    const int ph = wv & 1, chh = wv >> 1;
+    const int m = l & 31, kh = l >> 5;
+
+    f32x16 acc;
+#pragma unroll
+    for (int i = 0; i < 16; i++) acc[i] = 0.f;
+
+    for (int c0 = 0; c0 < Cin; c0 += KC) {
+        // stage the input halo tile: 3 rows x 66 columns x 32 channels
+        for (int v = t; v < 3 * (TPX + 2) * (KC / 8); v += 256) {
+            const int g = v % (KC / 8), col = (v / (KC / 8)) % (TPX + 2), r = v / ((KC / 8) * (TPX + 2));
+            const int gy = yy + r - 1, gx = x0 + col - 1, c = c0 + g * 8;
+            bf16x8 val = zero8();
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W && c < Cin) {
+                const size_t pix = (size_t(n) * H + gy) * W + gx;
+                val = c < c1 ? load8(x1 + pix * c1 + c) : load8(x2 + pix * c2 + (c - c1));
+            }
+            *reinterpret_cast<bf16x8*>(&s_in[r][col][g * 8]) = val;
+        }
+        // stage weights [tap][co][ci]
+        for (int v = t; v < 9 * TCO * (KC / 8); v += 256) {
+            const int g = v % (KC / 8), co = (v / (KC / 8)) % TCO, tap = v / ((KC / 8) * TCO);
+            const int c = c0 + g * 8, o = co0 + co;
+            bf16x8 val = zero8();
+            if (o < Cout && c < Cin) val = load8(wgt + (size_t(o) * 9 + tap) * Cin + c);
+            *reinterpret_cast<bf16x8*>(&s_w[tap][co][g * 8]) = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap % 3;
+#pragma unroll
+            for (int ks = 0; ks < KC / 16; ks++) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&s_in[dy][ph * 32 + m + dx][ks * 16 + kh * 8]);
+                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&s_w[tap][chh * 32 + m][ks * 16 + kh * 8]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // D[row = pixel][col = channel]: lane holds col m, rows 8i + 4kh + j
+    const int co = co0 + chh * 32 + m;
+    if (co >= Cout) return;
+    const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int px = x0 + ph * 32 + i * 8 + kh * 4 + j;
+            if (px >= W) continue;
+            const float v = act_fwd(acc[i * 4 + j] + bv, act, slope);
+            const size_t o = ((size_t(n) * H + yy) * W + px) * Cout + co;
+            if (y) y[o] = bf16(v);
+            if (yf) yf[o] = v;
+        }
+    }
+}
+
+// --------------------------------------------------------- weight grad
+// dW[co][tap][ci] = sum_p dY[p][co] * X[p + off(tap)][ci], split over pixel
+// groups (blockIdx.x) and written as partial sums; one WG owns a 32x32
+// (co, ci) tile and all 9 taps (wave w: taps w, w+4, w+8).
+constexpr int WG_PX = 64;
+
+__global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(const bf16* __restrict__ x1, int c1,
+                                                            const bf16* __restrict__ x2, int c2,
+                                                            const bf16* __restrict__ dy, int Cout, int N, int H,
+                                                            int W, int tiles_per_group, float* __restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) bf16 s_dy[32][WG_PX + PAD];
+    __shared__ __attribute__((aligned(16))) bf16 s_x[3][3][32][WG_PX + PAD];
+    const int Cin = c1 + c2;
+    const int co0 = int(blockIdx.y) * 32, ci0 = int(blockIdx.z) * 32;
+    const int t = int(threadIdx.x), wv = t >> 6, l = t & 63, m = l & 31, kh = l >> 5;
+    const int tiles_x = (W + WG_PX - 1) / WG_PX;
+    const long total_tiles = long(N) * H * tiles_x;
+
+    f32x16 acc[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int i = 0; i < 16; i++) acc[q][i] = 0.f;
+
+    for (int it = 0; it < tiles_per_group; it++) {
+        const long tile = long(blockIdx.x) * tiles_per_group + it;
+        if (tile >= total_tiles) break;
+        const int tx = int(tile % tiles_x);
+        const int row = int((tile / tiles_x) % H);
+        const int n = int(tile / (long(tiles_x) * H));
+        const int x0 = tx * WG_PX;
+        // dY^T tile: s_dy[co][px]
+        for (int v = t; v < WG_PX * 4; v += 256) {
+            const int g = v & 3, px = v >> 2, co = co0 + g * 8;
+            bf16x8 val = zero8();
+            if (x0 + px < W && co < Cout) val = load8(dy + ((size_t(n) * H + row) * W + x0 + px) * Cout + co);
+#pragma unroll
+            for (int e = 0; e < 8; e++) s_dy[g * 8 + e][px] = val[e];
+        }
+        // X^T tiles shifted by dx: s_x[dy][dx][ci][p] = X[row+dy-1][x0+p+dx-1][ci]
+        for (int v = t; v < 3 * (WG_PX + 2) * 4; v += 256) {
+            const int g = v & 3, col = (v >> 2) % (WG_PX + 2), r = (v >> 2) / (WG_PX + 2);
+            const int gy = row + r - 1, gx = x0 + col - 1, c = ci0 + g * 8;
+            bf16x8 val = zero8();
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W && c < Cin) {
+                const size_t pix = (size_t(n) * H + gy) * W + gx;
+                val = c < c1 ? load8(x1 + pix * c1 + c) : load8(x2 + pix * c2 + (c - c1));
+            }
+#pragma unroll
+            for (int ddx = 0; ddx < 3; ddx++) {
+                const int p = col - ddx;
+                if (p < 0 || p >= WG_PX) continue;
+#pragma unroll
+                for (int e = 0; e < 8; e++) s_x[r][ddx][g * 8 + e][p] = val[e];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int tap = wv + 4 * q;
+            if (tap < 9) {
+                const int ddy = tap / 3, ddx = tap % 3;
+#pragma unroll
+                for (int ks = 0; ks < WG_PX / 16; ks++) {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(&s_dy[m][ks * 16 + kh * 8]);
+                    const bf16x8 b = *reinterpret_cast<const bf16x8*>(&s_x[ddy][ddx][m][ks * 16 + kh * 8]);
+                    acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[q], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // D[row = co][col = ci]
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int tap = wv + 4 * q;
+        if (tap >= 9) continue;
+        const int ci = ci0 + m;
+        if (ci >= Cin) continue;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int co = co0 + i * 8 + kh * 4 + j;
+                if (co >= Cout) continue;
+                partial[((size_t(blockIdx.x) * Cout + co) * 9 + tap) * Cin + ci] = acc[q][i * 4 + j];
+            }
+    }
+}
+
+__global__ void reduce_partials_kernel(const float* __restrict__ partial, int groups, size_t count,
+                                       float* __restrict__ out, float beta) {
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= count) return;
+    float s = 0.f;
+    for (int k = 0; k < groups; k++) s += partial[size_t(k) * count + g];
+    out[g] = beta * out[g] + s;
+}
+
+// dbias[co] = sum over pixels of dY; one block per channel group of 64
+__global__ void bias_grad_kernel(const bf16* __restrict__ dy, size_t npix, int C, float* __restrict__ db, float beta) {
+    __shared__ float s[256];
+    const int c = int(blockIdx.x);
+    float acc = 0.f;
+    for (size_t p = threadIdx.x; p < npix; p += blockDim.x) acc += float(dy[p * C + c]);
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (int(threadIdx.x) < k) s[threadIdx.x] += s[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[c] = beta * db[c] + s[0];
+}
+
+// W'[ci][tap][co] = W[co][8 - tap][ci]: the data gradient is the forward
+// convolution of dY with these weights
+__global__ void flip_weights_kernel(const bf16* __restrict__ w, int Cout, int Cin, bf16* __restrict__ wt) {
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= size_t(Cout) * 9 * Cin) return;
+    const int ci = int(g % Cin), tap = int((g / Cin) % 9), co = int(g / (size_t(Cin) * 9));
+    wt[(size_t(ci) * 9 + (8 - tap)) * Cout + co] = w[g];
+}
+
+// --------------------------------------------------------- elementwise
+__global__ void act_bwd_kernel(const bf16* __restrict__ y, const bf16* __restrict__ dy, size_t n, int act, float slope,
+                               bf16* __restrict__ dx) {
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const float yv = float(y[g]), d = float(dy[g]);
+    float r = d;
+    if (act == 1) r = yv > 0.f ? d : 0.f;
+    else if (act == 2) r = yv > 0.f ? d : d * slope;
+    dx[g] = bf16(r);
+}
+
+__global__ void avgpool2_fwd_kernel(const bf16* __restrict__ x, int N, int H, int W, int C, bf16* __restrict__ y) {
+    const int Ho = H / 2, Wo = W / 2;
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= size_t(N) * Ho * Wo * C) return;
+    const int c = int(g % C), xo = int((g / C) % Wo), yo = int((g / (size_t(C) * Wo)) % Ho),
+              n = int(g / (size_t(C) * Wo * Ho));
+    auto at = [&](int yy, int xx) { return float(x[((size_t(n) * H + yy) * W + xx) * C + c]); };
+    y[g] = bf16(0.25f * (at(2 * yo, 2 * xo) + at(2 * yo, 2 * xo + 1) + at(2 * yo + 1, 2 * xo) + at(2 * yo + 1, 2 * xo + 1)));
+}
+
+__global__ void avgpool2_bwd_kernel(const bf16* __restrict__ dy, int N, int H, int W, int C, bf16* __restrict__ dx) {
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= size_t(N) * H * W * C) return;
+    const int c = int(g % C), xx = int((g / C) % W), yy = int((g / (size_t(C) * W)) % H), n = int(g / (size_t(C) * W * H));
+    const int Ho = H / 2, Wo = W / 2;
+    const int yo = yy / 2, xo = xx / 2;
+    float v = 0.f;
+    if (yo < Ho && xo < Wo) v = 0.25f * float(dy[((size_t(n) * Ho + yo) * Wo + xo) * C + c]);
+    dx[g] = bf16(v);
+}
+
+__global__ void upsample2_fwd_kernel(const bf16* __restrict__ x, int N, int H, int W, int C, bf16* __restrict__ y) {
+    const int Ho = H * 2, Wo = W * 2;
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= size_t(N) * Ho * Wo * C) return;
+    const int c = int(g % C), xo = int((g / C) % Wo), yo = int((g / (size_t(C) * Wo)) % Ho),
+              n = int(g / (size_t(C) * Wo * Ho));
+    y[g] = x[((size_t(n) * H + yo / 2) * W + xo / 2) * C + c];
+}
+
+__global__ void upsample2_bwd_kernel(const bf16* __restrict__ dy, int N, int H, int W, int C, bf16* __restrict__ dx) {
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= size_t(N) * H * W * C) return;
+    const int c = int(g % C), xx = int((g / C) % W), yy = int((g / (size_t(C) * W)) % H), n = int(g / (size_t(C) * W * H));
+    const int Ho = H * 2, Wo = W * 2;
+    auto at = [&](int a, int b) { return float(dy[((size_t(n) * Ho + a) * Wo + b) * C + c]); };
+    dx[g] = bf16(at(2 * yy, 2 * xx) + at(2 * yy, 2 * xx + 1) + at(2 * yy + 1, 2 * xx) + at(2 * yy + 1, 2 * xx + 1));
+}
+
+// split a concatenated gradient [.., c1 + c2] into dx1 = g[..:c1] and
+// dx2 += g[c1:] (the skip connection also receives the encoder's gradient)
+__global__ void split_add_kernel(const bf16* __restrict__ g, size_t npix, int c1, int c2, bf16* __restrict__ dx1,
+                                 bf16* __restrict__ dx2) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int C = c1 + c2;
+    if (i >= npix * C) return;
+    const size_t p = i / C;
+    const int c = int(i % C);
+    if (c < c1) {
+        if (dx1) dx1[p * c1 + c] = g[i];
+    } else if (dx2) {
+        const size_t o = p * c2 + (c - c1);
+        dx2[o] = bf16(float(dx2[o]) + float(g[i]));
+    }
+}
+
+__global__ void add_bf16_kernel(const bf16* __restrict__ a, bf16* __restrict__ b, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = bf16(float(a[i]) + float(b[i]));
+}
+
+// out (N,H,W,3) = base RGB + residual (first 3 channels of r, Cr stride);
+// loss = mean squared error vs target; gradient of the loss w.r.t. r
+__global__ void residual_out_kernel(const float* __restrict__ base, int Cb, const float* __restrict__ r, int Cr,
+                                    size_t npix, float* __restrict__ out) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npix * 3) return;
+    const size_t p = i / 3;
+    const int c = int(i % 3);
+    out[i] = base[p * Cb + c] + r[p * Cr + c];
+}
+
+__global__ void mse_grad_kernel(const float* __restrict__ out, const float* __restrict__ target, size_t npix, int Cr,
+                                float scale, bf16* __restrict__ dr, float* __restrict__ loss_acc) {
+    __shared__ float s[256];
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    float e2 = 0.f;
+    if (i < npix * size_t(Cr)) {
+        const size_t p = i / Cr;
+        const int c = int(i % Cr);
+        float g = 0.f;
+        if (c < 3) {
+            const float d = out[p * 3 + c] - target[p * 3 + c];
+            e2 = d * d;
+            g = 2.f * d * scale;
+        }
+        dr[i] = bf16(g);
+    }
+    s[threadIdx.x] = e2;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (int(threadIdx.x) < k) s[threadIdx.x] += s[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss_acc, s[0]);
+}
+
+// Adam on FP32 master weights, refreshing the BF16 copy used by the kernels
+__global__ void adam_kernel(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                            const float* __restrict__ g, size_t n, float lr, float b1, float b2, float eps, float bc1,
+                            float bc2, float wd, bf16* __restrict__ wb) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i] + wd * w[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = bf16(wi);
+}
+
+__global__ void f32_to_bf16_kernel(const float* __restrict__ a, bf16* __restrict__ b, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = bf16(a[i]);
+}
+
+__global__ void bf16_to_f32_kernel(const bf16* __restrict__ a, float* __restrict__ b, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = float(a[i]);
+}
+
+// Denoiser input features from a fixed-point histogram and its tonemapped
+// image: [R, G, B, alpha] of the tonemapped frame, log2(1 + density) / 16,
+// log2(1 + hits) / 16 and zero padding up to Cpad channels; also the FP32
+// base image the network adds its residual to.
+__global__ void features_kernel(const uint4* __restrict__ hist, const float4* __restrict__ tm, size_t npix, int Cpad,
+                                bf16* __restrict__ feat, float* __restrict__ base) {
+    const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const uint4 h = hist[p];
+    const float4 c = tm[p];
+    const float dens = float(h.w) * (1.0f / 255.0f);
+    float f[8] = {c.x, c.y, c.z, c.w, log2f(1.0f + dens) * (1.0f / 16.0f), log2f(1.0f + float(h.w > 0)) , 0.f, 0.f};
+    for (int k = 0; k < Cpad; k++) feat[p * Cpad + k] = bf16(k < 6 ? f[k] : 0.f);
+    base[p * 3 + 0] = c.x;
+    base[p * 3 + 1] = c.y;
+    base[p * 3 + 2] = c.z;
+}
+
+__global__ void clamp_rgb_kernel(const float* __restrict__ in, size_t n, float* __restrict__ out) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = fminf(fmaxf(in[i], 0.f), 1.f);
+}
+
+// copy n crops of ch x cw pixels (C channels of elem_bytes each) out of a
+// stack of H x W images; crop k starts at (img[k], y[k], x[k]) = offs[3k..]
+__global__ void gather_crops_kernel(const unsigned char* __restrict__ src, int H, int W, int row_bytes,
+                                    const int* __restrict__ offs, int ch, int cw, unsigned char* __restrict__ dst) {
+    const int k = int(blockIdx.z), r = int(blockIdx.y);
+    const int img = offs[3 * k], y0 = offs[3 * k + 1], x0 = offs[3 * k + 2];
+    const size_t srow = (size_t(img) * H + y0 + r) * W + x0;
+    const size_t drow = (size_t(k) * ch + r) * cw;
+    for (int b = int(blockIdx.x) * blockDim.x + threadIdx.x; b < cw * row_bytes; b += gridDim.x * blockDim.x)
+        dst[drow * row_bytes + b] = src[srow * row_bytes + b];
+}
+
+unsigned nb(size_t n, unsigned b = 256) { return unsigned((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N, int H, int W, const void* wgt,
+                           const float* bias, int Cout, int act, float slope, void* y, float* yf, hipStream_t s) {
+    if ((c1 % 8) || (c2 % 8) || c1 <= 0 || c2 < 0 || Cout <= 0) return hipErrorInvalidValue;
+    dim3 grid((W + TPX - 1) / TPX, H, N * ((Cout + TCO - 1) / TCO));
+    conv3x3_fwd_kernel<<<grid, 256, 0, s>>>((const bf16*)x1, c1, (const bf16*)x2, c2, N, H, W, (const bf16*)wgt, bias,
+                                            Cout, act, slope, (bf16*)y, yf);
+    return hipGetLastError();
+}
+
+size_t conv3x3_wgrad_workspace(int c1, int c2, int Cout, int N, int H, int W, int* groups, int* tiles_per_group) {
+    const long tiles = long(N) * H * ((W + WG_PX - 1) / WG_PX);
+    int g = int(std::min<long>(tiles, 256));
+    int tpg = int((tiles + g - 1) / g);
+    g = int((tiles + tpg - 1) / tpg);
+    if (groups) *groups = g;
+    if (tiles_per_group) *tiles_per_group = tpg;
+    return size_t(g) * Cout * 9 * (c1 + c2) * sizeof(float);
+}
+
+hipError_t conv3x3_wgrad(const void* x1, int c1, const void* x2, int c2, const void* dy, int Cout, int N, int H, int W,
+                         float* dw, float beta, float* workspace, hipStream_t s) {
+    if ((c1 % 8) || (c2 % 8) || (Cout % 8)) return hipErrorInvalidValue;
+    int groups = 0, tpg = 0;
+    conv3x3_wgrad_workspace(c1, c2, Cout, N, H, W, &groups, &tpg);
+    const int Cin = c1 + c2;
+    dim3 grid(groups, (Cout + 31) / 32, (Cin + 31) / 32);
+    conv3x3_wgrad_kernel<<<grid, 256, 0, s>>>((const bf16*)x1, c1, (const bf16*)x2, c2, (const bf16*)dy, Cout, N, H, W,
+                                              tpg, workspace);
+    const size_t count = size_t(Cout) * 9 * Cin;
+    reduce_partials_kernel<<<nb(count), 256, 0, s>>>(workspace, groups, count, dw, beta);
+    return hipGetLastError();
+}
+
+hipError_t bias_grad(const void* dy, size_t npix, int C, float* db, float beta, hipStream_t s) {
+    bias_grad_kernel<<<C, 256, 0, s>>>((const bf16*)dy, npix, C, db, beta);
+    return hipGetLastError();
+}
+
+hipError_t flip_weights(const void* w, int Cout, int Cin, void* wt, hipStream_t s) {
+    size_t n = size_t(Cout) * 9 * Cin;
+    flip_weights_kernel<<<nb(n), 256, 0, s>>>((const bf16*)w, Cout, Cin, (bf16*)wt);
+    return hipGetLastError();
+}
+
+hipError_t act_backward(const void* y, const void* dy, size_t n, int act, float slope, void* dx, hipStream_t s) {
+    act_bwd_kernel<<<nb(n), 256, 0, s>>>((const bf16*)y, (const bf16*)dy, n, act, slope, (bf16*)dx);
+    return hipGetLastError();
+}
+
+hipError_t avgpool2_forward(const void* x, int N, int H, int W, int C, void* y, hipStream_t s) {
+    size_t n = size_t(N) * (H / 2) * (W / 2) * C;
+    avgpool2_fwd_kernel<<<nb(n), 256, 0, s>>>((const bf16*)x, N, H, W, C, (bf16*)y);
+    return hipGetLastError();
+}
+
+hipError_t avgpool2_backward(const void* dy, int N, int H, int W, int C, void* dx, hipStream_t s) {
+    size_t n = size_t(N) * H * W * C;
+    avgpool2_bwd_kernel<<<nb(n), 256, 0, s>>>((const bf16*)dy, N, H, W, C, (bf16*)dx);
+    return hipGetLastError();
+}
+
+hipError_t upsample2_forward(const void* x, int N, int H, int W, int C, void* y, hipStream_t s) {
+    size_t n = size_t(N) * H * 2 * W * 2 * C;
+    upsample2_fwd_kernel<<<nb(n), 256, 0, s>>>((const bf16*)x, N, H, W, C, (bf16*)y);
+    return hipGetLastError();
+}
+
+hipError_t upsample2_backward(const void* dy, int N, int H, int W, int C, void* dx, hipStream_t s) {
+    size_t n = size_t(N) * H * W * C;
+    upsample2_bwd_kernel<<<nb(n), 256, 0, s>>>((const bf16*)dy, N, H, W, C, (bf16*)dx);
+    return hipGetLastError();
+}
+
+hipError_t split_add(const void* g, size_t npix, int c1, int c2, void* dx1, void* dx2, hipStream_t s) {
+    size_t n = npix * size_t(c1 + c2);
+    split_add_kernel<<<nb(n), 256, 0, s>>>((const bf16*)g, npix, c1, c2, (bf16*)dx1, (bf16*)dx2);
+    return hipGetLastError();
+}
+
+hipError_t add_bf16(const void* a, void* b, size_t n, hipStream_t s) {
+    add_bf16_kernel<<<nb(n), 256, 0, s>>>((const bf16*)a, (bf16*)b, n);
+    return hipGetLastError();
+}
+
+hipError_t residual_out(const float* base, int Cb, const float* r, int Cr, size_t npix, float* out, hipStream_t s) {
+    residual_out_kernel<<<nb(npix * 3), 256, 0, s>>>(base, Cb, r, Cr, npix, out);
+    return hipGetLastError();
+}
+
+hipError_t mse_grad(const float* out, const float* target, size_t npix, int Cr, float scale, void* dr, float* loss_acc,
+                    hipStream_t s) {
+    mse_grad_kernel<<<nb(npix * Cr), 256, 0, s>>>(out, target, npix, Cr, scale, (bf16*)dr, loss_acc);
+    return hipGetLastError();
+}
+
+hipError_t adam(float* w, float* m, float* v, const float* g, size_t n, float lr, float b1, float b2, float eps,
+                int step, float wd, void* wb, hipStream_t s) {
+    float bc1 = 1.f - powf(b1, float(step)), bc2 = 1.f - powf(b2, float(step));
+    adam_kernel<<<nb(n), 256, 0, s>>>(w, m, v, g, n, lr, b1, b2, eps, bc1, bc2, wd, (bf16*)wb);
+    return hipGetLastError();
+}
+
+hipError_t f32_to_bf16(const float* a, void* b, size_t n, hipStream_t s) {
+    f32_to_bf16_kernel<<<nb(n), 256, 0, s>>>(a, (bf16*)b, n);
+    return hipGetLastError();
+}
+
+hipError_t bf16_to_f32(const void* a, float* b, size_t n, hipStream_t s) {
+    bf16_to_f32_kernel<<<nb(n), 256, 0, s>>>((const bf16*)a, b, n);
+    return hipGetLastError();
+}
+
+hipError_t features(const uint32_t* hist, const float* tm_rgba, size_t npix, int Cpad, void* feat, float* base,
+                    hipStream_t s) {
+    features_kernel<<<nb(npix), 256, 0, s>>>((const uint4*)hist, (const float4*)tm_rgba, npix, Cpad, (bf16*)feat, base);
+    return hipGetLastError();
+}
+
+hipError_t gather_crops(const void* src, int H, int W, int pixel_bytes, const int* offs, int n, int ch, int cw,
+                        void* dst, hipStream_t s) {
+    dim3 grid(std::max(1, std::min(64, (cw * pixel_bytes + 255) / 256)), ch, n);
+    gather_crops_kernel<<<grid, 256, 0, s>>>((const unsigned char*)src, H, W, pixel_bytes, offs, ch, cw,
+                                             (unsigned char*)dst);
+    return hipGetLastError();
+}
+
+hipError_t clamp_rgb(const float* in, size_t n, float* out, hipStream_t s) {
+    clamp_rgb_kernel<<<nb(n), 256, 0, s>>>(in, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace rfk::nn

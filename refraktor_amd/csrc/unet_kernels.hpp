@@ -1,0 +1,45 @@
+// Host launchers of the denoiser kernels (unet.hip). Activations are NHWC
+// BF16, weights [Cout][9][Cin] BF16, accumulation FP32.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace rfk::nn {
+
+// y = act(conv3x3(concat(x1, x2)) + bias); c1, c2 multiples of 8 (c2 may be 0).
+// act: 0 none, 1 ReLU, 2 leaky ReLU(slope). y (BF16) and/or yf (FP32) may be null.
+hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N, int H, int W, const void* wgt,
+                           const float* bias, int Cout, int act, float slope, void* y, float* yf, hipStream_t s);
+// bytes of FP32 workspace conv3x3_wgrad needs
+size_t conv3x3_wgrad_workspace(int c1, int c2, int Cout, int N, int H, int W, int* groups, int* tiles_per_group);
+// dw = beta * dw + dL/dW (FP32, [Cout][9][Cin])
+hipError_t conv3x3_wgrad(const void* x1, int c1, const void* x2, int c2, const void* dy, int Cout, int N, int H, int W,
+                         float* dw, float beta, float* workspace, hipStream_t s);
+hipError_t bias_grad(const void* dy, size_t npix, int C, float* db, float beta, hipStream_t s);
+// wt[ci][tap][co] = w[co][8 - tap][ci] (data gradient = forward conv of dY with wt)
+hipError_t flip_weights(const void* w, int Cout, int Cin, void* wt, hipStream_t s);
+hipError_t act_backward(const void* y, const void* dy, size_t n, int act, float slope, void* dx, hipStream_t s);
+hipError_t avgpool2_forward(const void* x, int N, int H, int W, int C, void* y, hipStream_t s);
+hipError_t avgpool2_backward(const void* dy, int N, int H, int W, int C, void* dx, hipStream_t s);
+hipError_t upsample2_forward(const void* x, int N, int H, int W, int C, void* y, hipStream_t s);
+hipError_t upsample2_backward(const void* dy, int N, int H, int W, int C, void* dx, hipStream_t s);
+hipError_t split_add(const void* g, size_t npix, int c1, int c2, void* dx1, void* dx2, hipStream_t s);
+hipError_t add_bf16(const void* a, void* b, size_t n, hipStream_t s);
+hipError_t residual_out(const float* base, int Cb, const float* r, int Cr, size_t npix, float* out, hipStream_t s);
+hipError_t mse_grad(const float* out, const float* target, size_t npix, int Cr, float scale, void* dr, float* loss_acc,
+                    hipStream_t s);
+hipError_t adam(float* w, float* m, float* v, const float* g, size_t n, float lr, float b1, float b2, float eps,
+                int step, float wd, void* wb, hipStream_t s);
+hipError_t f32_to_bf16(const float* a, void* b, size_t n, hipStream_t s);
+hipError_t bf16_to_f32(const void* a, float* b, size_t n, hipStream_t s);
+hipError_t features(const uint32_t* hist, const float* tm_rgba, size_t npix, int Cpad, void* feat, float* base,
+                    hipStream_t s);
+hipError_t clamp_rgb(const float* in, size_t n, float* out, hipStream_t s);
+// crops (img, y, x) from a stack of H x W images with pixel_bytes per pixel
+hipError_t gather_crops(const void* src, int H, int W, int pixel_bytes, const int* offs, int n, int ch, int cw,
+                        void* dst, hipStream_t s);
+
+}  // namespace rfk::nn

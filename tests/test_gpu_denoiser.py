@@ -1,0 +1,111 @@
+"""Denoiser kernels against the float64 oracle, and training on the GPU."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import nn_oracle as nn
+from refraktor_amd import _lib
+from refraktor_amd import denoise
+from refraktor_amd import render
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a: np.ndarray) -> render.DeviceBuffer:
+    b = render.DeviceBuffer(a.nbytes)
+    b.upload(a)
+    return b
+
+
+def _bf16_dev(a: np.ndarray) -> render.DeviceBuffer:
+    return _dev(nn.bf16_bits(a.astype(np.float32)))
+
+
+CASES = [  # (N, H, W, c1, c2, Cout)
+    (1, 5, 70, 8, 0, 32),
+    (2, 9, 33, 32, 0, 64),
+    (1, 6, 130, 64, 32, 96),
+    (1, 4, 64, 128, 96, 96),
+    (1, 8, 20, 32, 0, 8),
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_conv_forward_matches_oracle(gpu, case):
+    N, H, W, c1, c2, Co = case
+    rng = np.random.default_rng(sum(case))
+    x1 = nn.to_bf16(rng.normal(size=(N, H, W, c1)).astype(np.float32))
+    x2 = nn.to_bf16(rng.normal(size=(N, H, W, c2)).astype(np.float32)) if c2 else None
+    w = nn.to_bf16((rng.normal(size=(Co, 9, c1 + c2)) * 0.1).astype(np.float32))
+    b = rng.normal(size=Co).astype(np.float32)
+    yf = render.DeviceBuffer(N * H * W * Co * 4)
+    d1, d2 = _bf16_dev(x1), (_bf16_dev(x2) if c2 else None)
+    dw, db_ = _bf16_dev(w), _dev(b)
+    _lib.check(_lib.lib().rfk_nn_conv3x3_forward(d1.ptr, c1, d2.ptr if d2 else None, c2, N, H, W, dw.ptr, db_.ptr,
+                                                Co, 1, None, yf.ptr, None))
+    got = yf.download(np.float32, (N, H, W, Co))
+    x = np.concatenate([x1, x2], axis=-1) if c2 else x1
+    want = np.maximum(nn.conv3x3(x, w, b), 0)
+    np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("case", CASES[:4])
+def test_conv_gradients_match_oracle(gpu, case):
+    N, H, W, c1, c2, Co = case
+    rng = np.random.default_rng(7 + sum(case))
+    x = nn.to_bf16(rng.normal(size=(N, H, W, c1 + c2)).astype(np.float32))
+    w = nn.to_bf16((rng.normal(size=(Co, 9, c1 + c2)) * 0.1).astype(np.float32))
+    gy = nn.to_bf16(rng.normal(size=(N, H, W, Co)).astype(np.float32))
+    dx_, dgy, dw_ = _bf16_dev(x), _bf16_dev(gy), _bf16_dev(w)
+    gw = render.DeviceBuffer(Co * 9 * (c1 + c2) * 4)
+    _lib.check(_lib.lib().rfk_nn_conv3x3_wgrad(dx_.ptr, c1 + c2, None, 0, dgy.ptr, Co, N, H, W, gw.ptr, None))
+    np.testing.assert_allclose(gw.download(np.float32, (Co, 9, c1 + c2)), nn.conv3x3_wgrad(x, gy),
+                               rtol=1e-3, atol=1e-3)
+    gx = render.DeviceBuffer(N * H * W * (c1 + c2) * 2)
+    _lib.check(_lib.lib().rfk_nn_conv3x3_dgrad(dgy.ptr, Co, N, H, W, dw_.ptr, c1 + c2, gx.ptr, None))
+    got = nn.from_bf16_bits(gx.download(np.uint16, (N, H, W, c1 + c2)))
+    want = nn.conv3x3_dgrad(gy, w)
+    np.testing.assert_allclose(got, want, rtol=1e-2, atol=2e-2 * np.abs(want).max())
+
+
+def test_unet_gradients_match_oracle(gpu):
+    """every parameter gradient of one training step vs the float64 network"""
+    rng = np.random.default_rng(11)
+    N, H, W = 2, 16, 24
+    den = denoise.Denoiser(seed=3)
+    w, b = den.params()
+    b = (rng.normal(size=b.shape) * 0.01).astype(np.float32)
+    den.set_params(w, b)
+    feat = nn.to_bf16(rng.normal(size=(N, H, W, nn.FEAT)).astype(np.float32))
+    base = rng.uniform(size=(N, H, W, 3)).astype(np.float32)
+    tgt = rng.uniform(size=(N, H, W, 3)).astype(np.float32)
+    df, dbase, dt = _bf16_dev(feat), _dev(base), _dev(tgt)
+    loss = den.train_step(df, dbase, dt, N, H, W, 0.0)  # lr 0: gradients only
+    gw, gb = den.grads()
+    oloss, ogw, ogb, _ = nn.unet_loss_and_grads(feat.astype(np.float64), base, tgt, nn.to_bf16(w).astype(np.float64),
+                                                b.astype(np.float64))
+    assert abs(loss - oloss) <= 1e-3 * oloss + 1e-7
+    # BF16 activations and gradients: compare as vectors
+    for g, o in ((gw, ogw), (gb, ogb)):
+        rel = np.linalg.norm(g - o) / np.linalg.norm(o)
+        cos = float(np.dot(g, o) / (np.linalg.norm(g) * np.linalg.norm(o)))
+        assert rel < 0.05 and cos > 0.998, (rel, cos)
+
+
+def test_training_reduces_loss(gpu):
+    """a few hundred Adam steps on a fixed smooth target"""
+    rng = np.random.default_rng(2)
+    N, H, W = 4, 32, 32
+    den = denoise.Denoiser(seed=1)
+    yy, xx = np.mgrid[0:H, 0:W] / 32.0
+    clean = np.stack([np.sin(3 * xx + 1) * 0.5 + 0.5, np.cos(2 * yy) * 0.5 + 0.5, xx * yy], -1)
+    clean = np.broadcast_to(clean, (N, H, W, 3)).astype(np.float32)
+    noisy = np.clip(clean + rng.normal(size=clean.shape) * 0.1, 0, 1).astype(np.float32)
+    feat = np.zeros((N, H, W, nn.FEAT), np.float32)
+    feat[..., :3] = noisy
+    df, dbase, dt = _bf16_dev(feat), _dev(noisy), _dev(np.ascontiguousarray(clean))
+    first = den.train_step(df, dbase, dt, N, H, W, 1e-3)
+    for _ in range(300):
+        last = den.train_step(df, dbase, dt, N, H, W, 1e-3)
+    assert last < 0.25 * first, (first, last)
