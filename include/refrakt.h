@@ -124,7 +124,52 @@ double rfk_kernel_compile_ms(const rfk_kernel* k);
  * fixed-point RGBA u32 histogram of width x height.                         */
 int rfk_warmup(rfk_kernel* k, void* stream, const double* samples, size_t n_reals, uint32_t width, uint32_t height,
                uint32_t seed, uint32_t fuse, uint32_t walkers, int temporal_multiplier, rfk_state** out);
+/* rfk_warmup with an explicit stream id: the Philox key of every draw of this
+ * render is (seed, stream_id); stream f of a batch reproduces frame f.       */
+int rfk_warmup_ex(rfk_kernel* k, void* stream, const double* samples, size_t n_reals, uint32_t width, uint32_t height,
+                  uint32_t seed, uint32_t stream_id, uint32_t fuse, uint32_t walkers, int temporal_multiplier,
+                  rfk_state** out);
 int rfk_state_free(rfk_state* s);
+
+/* ----------------------------------------------------------- autotune --
+ * No reference counterpart (librefrakt fixes its launch shape per build).
+ * Compiles the flame once per candidate shape, renders probe_iters after a
+ * quarter-size untimed bin, and records iterations/s of the device time in
+ * each candidate; *best is the index of the fastest. Candidates that fail to
+ * compile or launch keep their error code in `status`.                     */
+typedef struct rfk_tune_candidate {
+    uint32_t threads_per_block, atomic_width, bin_tile_log2, walkers; /* in  */
+    double iters_per_s, compile_ms;                                   /* out */
+    int status;                                                       /* out */
+} rfk_tune_candidate;
+int rfk_autotune(const rfk_db* db, const rfk_flame* f, int precision, const double* samples, size_t n_reals,
+                 uint32_t width, uint32_t height, uint64_t probe_iters, rfk_tune_candidate* cands, size_t n_cands,
+                 size_t* best);
+
+/* ---------------------------------------------------------- animation --
+ * No single reference counterpart: librefrakt renders an animation one frame
+ * at a time (src/refrakt-gui/src/gui/modals/render_modal.cpp:262-364,
+ * warmup + bin per frame). A batch renders `frames` frames of the keyframe
+ * samples (>= 4 packed samples, Catmull-Rom segments between the middle ones)
+ * in ONE persistent launch: every frame has its own histogram and iteration
+ * budget, work items (frame, block) are pulled longest-frame-first from a
+ * device queue. rfk_batch_run can capture the whole sequence in a hipGraph
+ * on its first call and replay it afterwards.                               */
+typedef struct rfk_batch rfk_batch;
+typedef struct rfk_batch_frame_info {
+    uint32_t* histogram;      /* device, height*width*4 u32                   */
+    double quality;           /* mean density (draw units)                    */
+    uint64_t total_draws, total_passes, bad_values;
+    uint32_t passes_per_walker, walkers;
+} rfk_batch_frame_info;
+int rfk_batch_create(rfk_kernel* k, void* stream, const double* samples, size_t n_reals, uint32_t frames,
+                     const uint64_t* iters_per_frame /* host */, uint32_t width, uint32_t height,
+                     uint32_t walkers_per_frame, uint32_t seed, uint32_t fuse, rfk_batch** out);
+int rfk_batch_run(rfk_batch* b, int use_graph, double* kernel_ms);
+int rfk_batch_frame(const rfk_batch* b, uint32_t frame, rfk_batch_frame_info* out);
+int rfk_batch_info(const rfk_batch* b, uint32_t* frames, uint32_t* width, uint32_t* height, uint32_t* grid,
+                   uint32_t* items);
+int rfk_batch_free(rfk_batch* b);
 
 /* rfkt::flame_kernel::bailout_args (flame_compiler.h:92-96). iters is an exact
  * iteration budget (rounded up to whole passes of all walkers; 0 = no budget),

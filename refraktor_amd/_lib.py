@@ -51,6 +51,18 @@ class StateInfo(C.Structure):
                 ("stream", C.c_void_p), ("quality", C.c_double)]
 
 
+class BatchFrameInfo(C.Structure):
+    _fields_ = [("histogram", C.c_void_p), ("quality", C.c_double), ("total_draws", C.c_uint64),
+                ("total_passes", C.c_uint64), ("bad_values", C.c_uint64), ("passes_per_walker", C.c_uint32),
+                ("walkers", C.c_uint32)]
+
+
+class TuneCandidate(C.Structure):
+    _fields_ = [("threads_per_block", C.c_uint32), ("atomic_width", C.c_uint32), ("bin_tile_log2", C.c_uint32),
+                ("walkers", C.c_uint32), ("iters_per_s", C.c_double), ("compile_ms", C.c_double),
+                ("status", C.c_int)]
+
+
 class TonemapArgs(C.Structure):
     _fields_ = [("quality", C.c_double), ("gamma", C.c_double), ("brightness", C.c_double),
                 ("vibrancy", C.c_double), ("max_density", C.c_double)]
@@ -88,7 +100,18 @@ _SIGS = [
     ("rfk_kernel_compile_ms", C.c_double, [_P]),
     ("rfk_warmup", C.c_int, [_P, _P, C.POINTER(C.c_double), C.c_size_t, C.c_uint32, C.c_uint32, C.c_uint32,
                              C.c_uint32, C.c_uint32, C.c_int, C.POINTER(_P)]),
+    ("rfk_warmup_ex", C.c_int, [_P, _P, C.POINTER(C.c_double), C.c_size_t, C.c_uint32, C.c_uint32, C.c_uint32,
+                                C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(_P)]),
     ("rfk_state_free", C.c_int, [_P]),
+    ("rfk_autotune", C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_double), C.c_size_t, C.c_uint32, C.c_uint32,
+                               C.c_uint64, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("rfk_batch_create", C.c_int, [_P, _P, C.POINTER(C.c_double), C.c_size_t, C.c_uint32, C.POINTER(C.c_uint64),
+                                   C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
+    ("rfk_batch_run", C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
+    ("rfk_batch_frame", C.c_int, [_P, C.c_uint32, C.c_void_p]),
+    ("rfk_batch_info", C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("rfk_batch_free", C.c_int, [_P]),
     ("rfk_bin", C.c_int, [_P, _P, C.POINTER(Bailout), C.POINTER(BinResult)]),
     ("rfk_bin_async", C.c_int, [_P, _P, C.POINTER(Bailout)]),
     ("rfk_bin_wait", C.c_int, [_P, C.POINTER(BinResult)]),

@@ -5,6 +5,7 @@
 // status codes instead of exit(), and a pinned host stop flag.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -37,6 +38,7 @@ struct rfk_kernel {
     hipModule_t mod = nullptr;
     hipFunction_t f_iterate = nullptr;
     hipFunction_t f_precalc = nullptr;
+    hipFunction_t f_batch = nullptr;
     double compile_ms = 0.0;
     bool cache_hit = false;
     int device = 0;
@@ -66,6 +68,28 @@ struct rfk_state {
     bool pending = false;
     uint64_t pending_passes = 0;
     double wallclock_khz = 100000.0;
+    uint32_t stream_id = 0;
+};
+
+struct rfk_batch {
+    rfk_kernel* k = nullptr;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    uint32_t frames = 0, width = 0, height = 0, tpb = 0, blocks_per_frame = 0, total_blocks = 0;
+    uint32_t param_stride = 0, seed = 0, fuse = 0, grid = 0;
+    void* params = nullptr;
+    uint32_t* palettes = nullptr;
+    uint32_t* hist = nullptr;
+    uint32_t *d_item_frame = nullptr, *d_item_block = nullptr, *d_frame_row0 = nullptr, *d_frame_blocks = nullptr,
+             *d_frame_passes = nullptr, *d_queue = nullptr, *d_row_stream = nullptr, *d_row_local = nullptr;
+    unsigned long long* d_frame_counters = nullptr;
+    unsigned long long* h_frame_counters = nullptr;
+    std::vector<uint32_t> frame_passes;
+    uint32_t nitems = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
 };
 
 namespace {
@@ -131,7 +155,7 @@ rfk_iter_args base_args(rfk_state* s) {
     a.height = s->height;
     a.param_stride = s->param_stride;
     a.seed = s->seed;
-    a.stream = 0;
+    a.stream = s->stream_id;
     a.total_walkers = s->walkers;
     a.check_every = 16;
     a.nsamples = s->blocks;
@@ -288,6 +312,13 @@ static int make_options(const rfk_exec_config* cfg, int precision, rfk::compile_
         return fail(RFK_ERR_INVALID, "threads_per_block must be 64, 128, 256, 512 or 1024");
     if (o.atomic_width != 32 && o.atomic_width != 64) return fail(RFK_ERR_INVALID, "atomic_width must be 32 or 64");
     if (precision != RFK_PRECISION_F32 && precision != RFK_PRECISION_F64) return fail(RFK_ERR_INVALID, "bad precision");
+    o.bin_tile_log2 = cfg ? cfg->bin_tile_log2 : 0;
+    if (o.bin_tile_log2 != 0 && (o.bin_tile_log2 < 6 || o.bin_tile_log2 > 12))
+        return fail(RFK_ERR_INVALID, "bin_tile_log2 must be 0 or 6..12");
+    // static LDS of one workgroup: palette, two point buffers, bin cache
+    const size_t lds = 1024 + 2 * 3 * size_t(o.tpb) * (precision ? 8 : 4) +
+                       (o.bin_tile_log2 ? (size_t(20) << o.bin_tile_log2) : 0) + 64;
+    if (lds > 160u * 1024u) return fail(RFK_ERR_INVALID, "launch shape needs more than 160 KiB of LDS per workgroup");
     return RFK_OK;
 }
 
@@ -339,6 +370,7 @@ int rfk_compile(const rfk_db* db, const rfk_flame* f, int precision, const rfk_e
     RFK_HIP(hipModuleLoadData(&k->mod, cr.code.data()));
     RFK_HIP(hipModuleGetFunction(&k->f_iterate, k->mod, "rfk_iterate"));
     RFK_HIP(hipModuleGetFunction(&k->f_precalc, k->mod, "rfk_precalc"));
+    RFK_HIP(hipModuleGetFunction(&k->f_batch, k->mod, "rfk_iterate_batch"));
     *out = k.release();
     return RFK_OK;
 }
@@ -399,6 +431,12 @@ int rfk_fix_rotation(double* samples, size_t sample_size, size_t flame_size, con
 
 int rfk_warmup(rfk_kernel* k, void* stream, const double* samples, size_t n_reals, uint32_t width, uint32_t height,
                uint32_t seed, uint32_t fuse, uint32_t walkers, int temporal_multiplier, rfk_state** out) {
+    return rfk_warmup_ex(k, stream, samples, n_reals, width, height, seed, 0u, fuse, walkers, temporal_multiplier, out);
+}
+
+int rfk_warmup_ex(rfk_kernel* k, void* stream, const double* samples, size_t n_reals, uint32_t width, uint32_t height,
+                  uint32_t seed, uint32_t stream_id, uint32_t fuse, uint32_t walkers, int temporal_multiplier,
+                  rfk_state** out) {
     if (!k || !samples || !out) return fail(RFK_ERR_INVALID, "null argument");
     if (width == 0 || height == 0 || width > 32768 || height > 32768) return fail(RFK_ERR_INVALID, "bad image size");
     if (temporal_multiplier < 1 || temporal_multiplier > 64) return fail(RFK_ERR_INVALID, "bad temporal multiplier");
@@ -425,6 +463,7 @@ int rfk_warmup(rfk_kernel* k, void* stream, const double* samples, size_t n_real
     const uint32_t align = s->precision ? 8u : 16u;  // 64-byte rows
     s->param_stride = (s->size_reals + align - 1) / align * align;
     s->seed = seed;
+    s->stream_id = stream_id;
     if (stream) {
         s->stream = static_cast<hipStream_t>(stream);
     } else {
@@ -638,6 +677,313 @@ int rfk_state_clear_histogram(rfk_state* s) {
     RFK_HIP(hipMemsetAsync(s->hist, 0, size_t(s->width) * s->height * 16, s->stream));
     RFK_HIP(hipStreamSynchronize(s->stream));
     s->quality = 0.0;
+    return RFK_OK;
+}
+
+// ---------------------------------------------------------------- autotune
+
+int rfk_autotune(const rfk_db* db, const rfk_flame* f, int precision, const double* samples, size_t n_reals,
+                 uint32_t width, uint32_t height, uint64_t probe_iters, rfk_tune_candidate* cands, size_t n_cands,
+                 size_t* best) {
+    if (!db || !f || !samples || !cands || !n_cands || !best) return fail(RFK_ERR_INVALID, "null argument");
+    if (probe_iters == 0) return fail(RFK_ERR_INVALID, "probe_iters must be > 0");
+    double best_rate = -1.0;
+    size_t best_i = n_cands;
+    for (size_t i = 0; i < n_cands; i++) {
+        rfk_tune_candidate& c = cands[i];
+        c.iters_per_s = 0.0;
+        c.compile_ms = 0.0;
+        rfk_exec_config cfg{c.threads_per_block, c.atomic_width, c.bin_tile_log2, 0u};
+        rfk_kernel* k = nullptr;
+        c.status = rfk_compile(db, f, precision, &cfg, &k, nullptr, 0);
+        if (c.status != RFK_OK) continue;
+        c.compile_ms = k->compile_ms;
+        rfk_state* s = nullptr;
+        c.status = rfk_warmup(k, nullptr, samples, n_reals, width, height, 1u, 20u, c.walkers, 1, &s);
+        if (c.status == RFK_OK) {
+            rfk_bailout bo{std::max<uint64_t>(probe_iters / 4, 1), 0.0, 0.0};
+            rfk_bin_result r{};
+            c.status = rfk_bin(k, s, &bo, &r);  // untimed: clocks and caches settle
+            if (c.status == RFK_OK) {
+                bo.iters = probe_iters;
+                c.status = rfk_bin(k, s, &bo, &r);
+            }
+            if (c.status == RFK_OK && r.kernel_ms > 0.0) c.iters_per_s = double(r.total_passes) / (r.kernel_ms * 1e-3);
+            rfk_state_free(s);
+        }
+        rfk_kernel_free(k);
+        if (c.status == RFK_OK && c.iters_per_s > best_rate) {
+            best_rate = c.iters_per_s;
+            best_i = i;
+        }
+    }
+    if (best_i == n_cands) return fail(RFK_ERR_INVALID, "no candidate configuration ran");
+    *best = best_i;
+    return RFK_OK;
+}
+
+// ------------------------------------------------------------------- batch
+
+static void free_batch(rfk_batch* b) {
+    auto f = [](void* p) { if (p) (void)hipFree(p); };
+    f(b->params); f(b->palettes); f(b->hist); f(b->d_item_frame); f(b->d_item_block); f(b->d_frame_row0);
+    f(b->d_frame_blocks); f(b->d_frame_passes); f(b->d_queue); f(b->d_frame_counters); f(b->d_row_stream);
+    f(b->d_row_local);
+    if (b->h_frame_counters) (void)hipHostFree(b->h_frame_counters);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+int rfk_batch_create(rfk_kernel* k, void* stream, const double* samples, size_t n_reals, uint32_t frames,
+                     const uint64_t* iters, uint32_t width, uint32_t height, uint32_t walkers_per_frame, uint32_t seed,
+                     uint32_t fuse, rfk_batch** out) {
+    if (!k || !samples || !iters || !out || frames == 0) return fail(RFK_ERR_INVALID, "bad argument");
+    if (k->L.chaos_base >= 0) return fail(RFK_ERR_INVALID, "batches need a kernel without xaos");
+    if (width == 0 || height == 0 || width > 32768 || height > 32768) return fail(RFK_ERR_INVALID, "bad image size");
+    const size_t sample_size = size_t(k->L.size_reals) + 768;
+    if (n_reals % sample_size != 0 || n_reals / sample_size < 4)
+        return fail(RFK_ERR_INVALID, "samples must hold >= 4 packed keyframe samples");
+    if (int rc = check_device()) return rc;
+    auto b = std::make_unique<rfk_batch>();
+    rfk_batch* B = b.get();
+    B->k = k;
+    B->frames = frames;
+    B->width = width;
+    B->height = height;
+    B->tpb = k->opt.tpb;
+    B->blocks_per_frame = std::max<uint32_t>(1, (walkers_per_frame + B->tpb - 1) / B->tpb);
+    const uint64_t total = uint64_t(B->blocks_per_frame) * frames;
+    if (total > (1u << 24)) return fail(RFK_ERR_INVALID, "too many blocks in the batch");
+    B->total_blocks = uint32_t(total);
+    const uint32_t align = k->opt.precision ? 8u : 16u;
+    B->param_stride = (uint32_t(k->L.size_reals) + align - 1) / align * align;
+    B->seed = seed;
+    B->fuse = fuse;
+    const uint64_t walkers_f = uint64_t(B->blocks_per_frame) * B->tpb;
+    B->frame_passes.resize(frames);
+    for (uint32_t f = 0; f < frames; f++) {
+        uint64_t p = (iters[f] + walkers_f - 1) / walkers_f;
+        if (p == 0 || p > 0x7FFFFFF0ull) return fail(RFK_ERR_INVALID, "bad frame iteration budget");
+        B->frame_passes[f] = uint32_t(p);
+    }
+    auto cleanup = [&](int rc) {
+        free_batch(B);
+        B->own_stream = false;
+        return rc;
+    };
+#define RFK_HIPB(expr)                                                                                    \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) return cleanup(fail(RFK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+    if (stream) {
+        B->stream = static_cast<hipStream_t>(stream);
+    } else {
+        RFK_HIPB(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+        B->own_stream = true;
+    }
+    const size_t elem = k->opt.precision ? sizeof(double) : sizeof(float);
+    const size_t fpix = size_t(width) * height;
+    RFK_HIPB(hipMalloc(&B->params, size_t(B->total_blocks) * B->param_stride * elem));
+    RFK_HIPB(dalloc(&B->palettes, size_t(B->total_blocks) * 256));
+    RFK_HIPB(dalloc(&B->hist, fpix * 4 * frames));
+    RFK_HIPB(dalloc(&B->d_frame_row0, frames));
+    RFK_HIPB(dalloc(&B->d_frame_blocks, frames));
+    RFK_HIPB(dalloc(&B->d_frame_passes, frames));
+    RFK_HIPB(dalloc(&B->d_frame_counters, size_t(frames) * 4));
+    RFK_HIPB(dalloc(&B->d_queue, 4));
+    RFK_HIPB(dalloc(&B->d_item_frame, B->total_blocks));
+    RFK_HIPB(dalloc(&B->d_item_block, B->total_blocks));
+    RFK_HIPB(dalloc(&B->d_row_stream, B->total_blocks));
+    RFK_HIPB(dalloc(&B->d_row_local, B->total_blocks));
+    RFK_HIPB(hipHostMalloc(reinterpret_cast<void**>(&B->h_frame_counters), size_t(frames) * 4 * 8, hipHostMallocDefault));
+    RFK_HIPB(hipEventCreate(&B->ev0));
+    RFK_HIPB(hipEventCreate(&B->ev1));
+
+    // rows: frame f owns rows [f * bpf, (f+1) * bpf); block b of frame f sits at
+    // global time (f + b / bpf) * nseg / frames (one frame interval of motion blur)
+    const uint32_t nsamp = uint32_t(n_reals / sample_size), nseg = nsamp - 3, bpf = B->blocks_per_frame;
+    std::vector<uint32_t> row0(frames), fblocks(frames, bpf), row_stream(B->total_blocks), row_local(B->total_blocks);
+    std::vector<double> times(B->total_blocks);
+    for (uint32_t f = 0; f < frames; f++) {
+        row0[f] = f * bpf;
+        for (uint32_t bl = 0; bl < bpf; bl++) {
+            times[f * bpf + bl] = (double(f) + double(bl) / bpf) * nseg / frames;
+            row_stream[f * bpf + bl] = f;
+            row_local[f * bpf + bl] = bl;
+        }
+    }
+    // work items, longest frames first so the persistent grid drains evenly
+    std::vector<uint32_t> order(frames);
+    for (uint32_t f = 0; f < frames; f++) order[f] = f;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return B->frame_passes[x] > B->frame_passes[y]; });
+    std::vector<uint32_t> item_frame, item_block;
+    for (uint32_t f : order)
+        for (uint32_t bl = 0; bl < bpf; bl++) {
+            item_frame.push_back(f);
+            item_block.push_back(bl);
+        }
+    B->nitems = uint32_t(item_frame.size());
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, B->stream);
+    };
+    RFK_HIPB(up(B->d_frame_row0, row0.data(), frames * 4));
+    RFK_HIPB(up(B->d_frame_blocks, fblocks.data(), frames * 4));
+    RFK_HIPB(up(B->d_frame_passes, B->frame_passes.data(), frames * 4));
+    RFK_HIPB(up(B->d_item_frame, item_frame.data(), item_frame.size() * 4));
+    RFK_HIPB(up(B->d_item_block, item_block.data(), item_block.size() * 4));
+    RFK_HIPB(up(B->d_row_stream, row_stream.data(), row_stream.size() * 4));
+    RFK_HIPB(up(B->d_row_local, row_local.data(), row_local.size() * 4));
+
+    // keyframes -> segments -> per-row parameters and palettes -> precalc
+    std::vector<double> host(samples, samples + n_reals);
+    rfk::fix_rotation(host.data(), sample_size, size_t(k->L.size_reals), k->L.affine_indices.data(),
+                      k->L.affine_indices.size(), int(nsamp));
+    std::vector<int8_t> kind(size_t(k->L.size_reals), 0);
+    for (size_t ai : k->L.affine_indices)
+        for (int c = 0; c < 4; c++) kind[ai + size_t(c)] = int8_t(1 + c);
+    double *d_samples = nullptr, *d_segments = nullptr, *d_times = nullptr;
+    int8_t* d_kind = nullptr;
+    RFK_HIPB(hipMalloc(&d_samples, n_reals * sizeof(double)));
+    RFK_HIPB(hipMalloc(&d_segments, size_t(nseg) * sample_size * 4 * sizeof(double)));
+    RFK_HIPB(hipMalloc(&d_times, times.size() * sizeof(double)));
+    RFK_HIPB(hipMalloc(&d_kind, kind.size() + 16));
+    RFK_HIPB(up(d_samples, host.data(), n_reals * sizeof(double)));
+    RFK_HIPB(up(d_times, times.data(), times.size() * sizeof(double)));
+    RFK_HIPB(up(d_kind, kind.data(), kind.size()));
+    RFK_HIPB(rfk::sk::catmull(d_samples, uint32_t(sample_size), nseg, d_segments, B->stream));
+    RFK_HIPB(rfk::sk::eval_samples(d_segments, uint32_t(sample_size), uint32_t(k->L.size_reals), nseg, B->total_blocks,
+                                   d_kind, k->opt.precision, B->params, B->param_stride, B->palettes, B->stream,
+                                   d_times));
+    rfk_iter_args a;
+    std::memset(&a, 0, sizeof a);
+    a.params = static_cast<const float*>(B->params);
+    a.param_stride = B->param_stride;
+    a.seed = seed;
+    a.nsamples = B->total_blocks;
+    a.row_stream = B->d_row_stream;
+    a.row_local = B->d_row_local;
+    {
+        void* argv[] = {&a};
+        uint32_t tb = 64, g = (B->total_blocks + tb - 1) / tb;
+        RFK_HIPB(hipModuleLaunchKernel(k->f_precalc, g, 1, 1, tb, 1, 1, 0, B->stream, argv, nullptr));
+    }
+    RFK_HIPB(hipStreamSynchronize(B->stream));
+    (void)hipFree(d_samples);
+    (void)hipFree(d_segments);
+    (void)hipFree(d_times);
+    (void)hipFree(d_kind);
+    // persistent grid: enough groups to fill every CU at the kernel's occupancy
+    int dev = 0, cus = 0, per_cu = 0;
+    RFK_HIPB(hipGetDevice(&dev));
+    RFK_HIPB(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k->f_batch, int(B->tpb), 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    B->grid = std::min<uint32_t>(B->nitems, uint32_t(cus) * uint32_t(per_cu));
+#undef RFK_HIPB
+    *out = b.release();
+    return RFK_OK;
+}
+
+int rfk_batch_free(rfk_batch* b) {
+    if (!b) return RFK_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    free_batch(b);
+    delete b;
+    return RFK_OK;
+}
+
+static int batch_enqueue(rfk_batch* B) {
+    rfk_kernel* k = B->k;
+    const size_t fpix = size_t(B->width) * B->height;
+    // clears are kernels, not memsets, so the sequence can live in a graph
+    RFK_HIP(rfk::sk::fill_u32(B->hist, 0u, fpix * 4 * B->frames, B->stream));
+    RFK_HIP(rfk::sk::fill_u32(B->d_queue, 0u, 1, B->stream));
+    RFK_HIP(rfk::sk::fill_u32(reinterpret_cast<uint32_t*>(B->d_frame_counters), 0u, size_t(B->frames) * 8, B->stream));
+    rfk_iter_args a;
+    std::memset(&a, 0, sizeof a);
+    a.params = static_cast<const float*>(B->params);
+    a.palettes = B->palettes;
+    a.hist = B->hist;
+    a.width = B->width;
+    a.height = B->height;
+    a.param_stride = B->param_stride;
+    a.seed = B->seed;
+    a.check_every = 16;
+    rfk_batch_args ba;
+    std::memset(&ba, 0, sizeof ba);
+    ba.queue = B->d_queue;
+    ba.item_frame = B->d_item_frame;
+    ba.item_block = B->d_item_block;
+    ba.frame_row0 = B->d_frame_row0;
+    ba.frame_blocks = B->d_frame_blocks;
+    ba.frame_passes = B->d_frame_passes;
+    ba.frame_counters = reinterpret_cast<u64*>(B->d_frame_counters);
+    ba.nitems = B->nitems;
+    ba.fuse = B->fuse;
+    ba.frame_pixels = uint32_t(fpix);
+    void* argv[] = {&a, &ba};
+    RFK_HIP(hipModuleLaunchKernel(k->f_batch, B->grid, 1, 1, B->tpb, 1, 1, 0, B->stream, argv, nullptr));
+    return RFK_OK;
+}
+
+int rfk_batch_run(rfk_batch* B, int use_graph, double* kernel_ms) {
+    if (!B) return fail(RFK_ERR_INVALID, "null batch");
+    if (use_graph && !B->exec) {
+        RFK_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+        int rc = batch_enqueue(B);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(B->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(RFK_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        B->graph = g;
+        RFK_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+    }
+    RFK_HIP(hipEventRecord(B->ev0, B->stream));
+    if (use_graph) {
+        RFK_HIP(hipGraphLaunch(B->exec, B->stream));
+    } else if (int rc = batch_enqueue(B)) {
+        return rc;
+    }
+    RFK_HIP(hipEventRecord(B->ev1, B->stream));
+    RFK_HIP(hipMemcpyAsync(B->h_frame_counters, B->d_frame_counters, size_t(B->frames) * 32, hipMemcpyDeviceToHost,
+                           B->stream));
+    RFK_HIP(hipStreamSynchronize(B->stream));
+    RFK_HIP(hipEventElapsedTime(&B->last_ms, B->ev0, B->ev1));
+    if (kernel_ms) *kernel_ms = B->last_ms;
+    return RFK_OK;
+}
+
+int rfk_batch_frame(const rfk_batch* B, uint32_t frame, rfk_batch_frame_info* out) {
+    if (!B || !out || frame >= B->frames) return fail(RFK_ERR_INVALID, "bad argument");
+    const unsigned long long* c = B->h_frame_counters + size_t(frame) * 4;
+    const double bins = double(B->width) * B->height;
+    out->histogram = B->hist + size_t(frame) * B->width * B->height * 4;
+    out->quality = double(c[0]) / (bins * 255.0);
+    out->total_draws = c[1];
+    out->total_passes = c[2];
+    out->bad_values = c[3];
+    out->passes_per_walker = B->frame_passes[frame];
+    out->walkers = B->blocks_per_frame * B->tpb;
+    return RFK_OK;
+}
+
+int rfk_batch_info(const rfk_batch* B, uint32_t* frames, uint32_t* width, uint32_t* height, uint32_t* grid,
+                   uint32_t* items) {
+    if (!B) return fail(RFK_ERR_INVALID, "null batch");
+    if (frames) *frames = B->frames;
+    if (width) *width = B->width;
+    if (height) *height = B->height;
+    if (grid) *grid = B->grid;
+    if (items) *items = B->nitems;
     return RFK_OK;
 }
 

@@ -200,6 +200,7 @@ std::string make_source(const variation_db& db, const flame& f, const flame_layo
     os << "#define RFK_USE_CHAOS " << (L.chaos_base >= 0 ? 1 : 0) << "\n";
     os << "#define RFK_SIZE_REALS " << L.size_reals << "\n";
     os << "#define RFK_ATOMIC64 " << (o.atomic_width == 64 ? 1 : 0) << "\n";
+    os << "#define RFK_BIN_TILE_LOG2 " << o.bin_tile_log2 << "\n";
     os << device_runtime_source() << "\n";
     os << "// ---------------------------------------------------------------- flame\n";
     os << make_flame_functions(db, f, L) << "\n";

@@ -15,6 +15,7 @@ struct compile_options {
     int precision = 0;        // 0 f32, 1 f64
     uint32_t tpb = 256;       // walkers per block (power of two, 64..1024)
     uint32_t atomic_width = 32;
+    uint32_t bin_tile_log2 = 0;  // 0: no LDS bin cache, else 2^n cached bins per block
 };
 
 // Generated source: configuration macros, the device runtime, the per-flame

@@ -311,6 +311,8 @@ struct rfk_iter_args {
     float* trace;              // [npasses][walkers][4] or null
     u32* trace_xid;            // [npasses][walkers/64] or null
     u32* wlast;                // [walkers] last xform per walker (xaos) or null
+    const u32* row_stream;     // precalc: stream id of each params row (null: a.stream)
+    const u32* row_local;      // precalc: jitter counter of each row (null: the row index)
     u64 time_limit_ticks;      // 0 = none
     u64 quality_target;        // per block, in 255*opacity units; 0 = none
     u32 width, height;
@@ -321,6 +323,21 @@ struct rfk_iter_args {
     u32 flags;                 // RFK_FLAG_*
     u32 check_every;           // passes between bailout checks (power of two)
     u32 nsamples;              // rows of params (precalc kernel)
+};
+
+// Persistent batch launch: work items are (frame, block) pairs
+struct rfk_batch_args {
+    u32* queue;                 // work-item counter (zeroed before the launch)
+    const u32* item_frame;      // [nitems]
+    const u32* item_block;      // [nitems] block inside its frame
+    const u32* frame_row0;      // [frames] first params/palette row of the frame
+    const u32* frame_blocks;    // [frames]
+    const u32* frame_passes;    // [frames] binned passes per walker
+    u64* frame_counters;        // [frames][4] hits, draws, passes, bad
+    u32 nitems;
+    u32 fuse;                   // un-binned passes after seeding
+    u32 frame_pixels;           // width * height
+    u32 pad;
 };
 
 #define RFK_FLAG_DRAW 1u

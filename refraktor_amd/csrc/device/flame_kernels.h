@@ -4,18 +4,94 @@
 // rfk_precalc_sample.
 //
 // Scheme (replaces refactor.cu:176-210, 247-343, 388-475):
-//  * one block = one temporal sample = RFK_TPB walkers, parameters read
-//    through scalar loads from the block's row of `params`;
+//  * one workgroup = one temporal sample = RFK_TPB walkers, parameters read
+//    through scalar loads from the sample's row of `params`;
 //  * every wave applies one xform per pass, drawn from the block's Philox
 //    stream (xaos flames pick per walker instead);
 //  * after each pass the block's points are permuted through LDS with a
 //    counter-keyed bijection so a walker's next xform is decorrelated from its
 //    wave neighbours (the reference uses host-built shuffle tables);
 //  * every in-bounds draw is four integer atomics into the RGBA u32 bin.
+// rfk_iterate runs one launch of `npasses` for every sample; rfk_iterate_batch
+// is a persistent kernel that pulls (frame, block) work items from a queue so
+// frames with uneven budgets share one launch.
 #pragma once
 
 #ifndef RFK_ATOMIC64
 #define RFK_ATOMIC64 0
+#endif
+#ifndef RFK_BIN_TILE_LOG2
+#define RFK_BIN_TILE_LOG2 0
+#endif
+
+// LDS bin cache: 2^RFK_BIN_TILE_LOG2 direct-mapped slots per block. The first
+// bin to claim a free slot owns it until the next flush; its draws accumulate
+// with LDS atomics, draws of other bins mapping there go straight to global
+// memory. Flushes every RFK_TILE_FLUSH passes and at the end of the block
+// keep a slot's u32 sums below 2^32 (RFK_TPB * RFK_TILE_FLUSH * 255).
+// Integer sums make the histogram identical with or without the cache.
+#define RFK_TILE (1u << RFK_BIN_TILE_LOG2)
+#define RFK_TILE_EMPTY 0xFFFFFFFFu
+#define RFK_TILE_FLUSH 64u
+
+struct rfk_shared {
+    u32 pal[256];
+    Real pts[2][3][RFK_TPB];
+    u64 hits, draws, bad, start;
+    u32 stop, item;
+#if RFK_BIN_TILE_LOG2
+    u32 tkey[RFK_TILE];
+    u32 tacc[4][RFK_TILE];
+#endif
+};
+
+#if RFK_BIN_TILE_LOG2
+RFK_INLINE u32 rfk_tile_slot(u32 pixel) { return (pixel * 0x9E3779B1u) >> (32 - RFK_BIN_TILE_LOG2); }
+
+RFK_INLINE void rfk_tile_clear(rfk_shared& sm) {
+    for (u32 s = threadIdx.x; s < RFK_TILE; s += RFK_TPB) {
+        sm.tkey[s] = RFK_TILE_EMPTY;
+        sm.tacc[0][s] = 0u;
+        sm.tacc[1][s] = 0u;
+        sm.tacc[2][s] = 0u;
+        sm.tacc[3][s] = 0u;
+    }
+}
+
+// every thread of the block; LDS must be quiescent (callers sync before)
+RFK_INLINE void rfk_tile_flush(rfk_shared& sm, u32* __restrict__ hist) {
+    for (u32 s = threadIdx.x; s < RFK_TILE; s += RFK_TPB) {
+        const u32 key = sm.tkey[s];
+        if (key != RFK_TILE_EMPTY) {
+            rfk_draw_t d;
+            d.pixel = key;
+            d.r = sm.tacc[0][s];
+            d.g = sm.tacc[1][s];
+            d.b = sm.tacc[2][s];
+            d.w = sm.tacc[3][s];
+            d.hit = true;
+            rfk_commit_draw(hist, d);
+            sm.tkey[s] = RFK_TILE_EMPTY;
+            sm.tacc[0][s] = 0u;
+            sm.tacc[1][s] = 0u;
+            sm.tacc[2][s] = 0u;
+            sm.tacc[3][s] = 0u;
+        }
+    }
+}
+
+RFK_INLINE void rfk_tile_draw(rfk_shared& sm, u32* __restrict__ hist, const rfk_draw_t& d) {
+    const u32 s = rfk_tile_slot(d.pixel);
+    const u32 prev = atomicCAS(&sm.tkey[s], RFK_TILE_EMPTY, d.pixel);
+    if (prev == RFK_TILE_EMPTY || prev == d.pixel) {
+        atomicAdd(&sm.tacc[0][s], d.r);
+        atomicAdd(&sm.tacc[1][s], d.g);
+        atomicAdd(&sm.tacc[2][s], d.b);
+        atomicAdd(&sm.tacc[3][s], d.w);
+    } else {
+        rfk_commit_draw(hist, d);
+    }
+}
 #endif
 
 RFK_INLINE u32 rfk_wave_pick(u32 pass, u32 block, u32 wave, u32 k0, u32 k1, u32x4& cache, bool refresh) {
@@ -24,75 +100,69 @@ RFK_INLINE u32 rfk_wave_pick(u32 pass, u32 block, u32 wave, u32 k0, u32 k1, u32x
     return j == 0 ? cache.x : j == 1 ? cache.y : j == 2 ? cache.z : cache.w;
 }
 
-extern "C" __global__ void rfk_precalc(rfk_iter_args a) {
-    const u32 s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.nsamples) return;
-    Real* P = (Real*)a.params + (size_t)s * a.param_stride;
-    rng_t rs;
-    rs.init(a.seed, a.stream, 0u, RFK_TAG_JITTER);
-    rs.begin(s);
-    rfk_precalc_sample(P, rs);
-}
-
-extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate(rfk_iter_args a) {
-    __shared__ u32 s_pal[256];
-    __shared__ Real s_pts[2][3][RFK_TPB];
-    __shared__ u64 s_hits, s_draws, s_bad;
-    __shared__ u32 s_stop;
-    __shared__ u64 s_start;
-
+// One block of walkers for `nfuse` un-binned passes followed by `npasses`
+// passes (binned when flags has RFK_FLAG_DRAW), starting at pass `pass0`.
+//   P, pal:   the sample's parameters and palette (global memory)
+//   block:    block id inside its stream (RNG counters), key1: stream id
+//   hist:     the frame's histogram
+//   init:     seed walkers (Hammersley over `total` walkers) instead of loading
+//   wx..wl:   walker state arrays (null: the block keeps its walkers only
+//             for this call)
+// Returns the passes done (bailouts may stop early); counters land in sm.
+RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, const Real* __restrict__ P, const u32* pal,
+                         u32 block, u32 key1, u32* hist, bool init, u32 total, u32 pass0, u32 nfuse, u32 npasses,
+                         u32 flags, float* wx, float* wy, float* wc, u32* wl) {
     const u32 t = threadIdx.x;
-    const u32 block = blockIdx.x;
     const u32 gid = block * RFK_TPB + t;
     const u32 wave = __builtin_amdgcn_readfirstlane(t >> 6);
-    const Real* __restrict__ P = (const Real*)a.params + (size_t)block * a.param_stride;
-    const bool draw = (a.flags & RFK_FLAG_DRAW) != 0u;
-    const bool trace = (a.flags & RFK_FLAG_TRACE) != 0u;
-    const bool bailout = (a.flags & RFK_FLAG_BAILOUT) != 0u;
+    const bool draw = (flags & RFK_FLAG_DRAW) != 0u;
+    const bool trace = (flags & RFK_FLAG_TRACE) != 0u;
+    const bool bailout = (flags & RFK_FLAG_BAILOUT) != 0u;
 
-    for (u32 i = t; i < 256u; i += RFK_TPB) s_pal[i] = a.palettes[(size_t)block * 256u + i];
+    for (u32 i = t; i < 256u; i += RFK_TPB) sm.pal[i] = pal[i];
     if (t == 0) {
-        s_hits = 0;
-        s_draws = 0;
-        s_bad = 0;
-        s_stop = 0;
-        s_start = wall_clock64();
+        sm.hits = 0;
+        sm.draws = 0;
+        sm.bad = 0;
+        sm.stop = 0;
+        sm.start = wall_clock64();
     }
+#if RFK_BIN_TILE_LOG2
+    if (draw) rfk_tile_clear(sm);
+#endif
     float S[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) S[k] = float(P[k]);
 
-    u32 pass0;
     Real x, y, c;
     u32 last = 255u;
-    if (a.flags & RFK_FLAG_INIT) {
+    if (init) {
         // Hammersley start positions in pixel space mapped through plane_space
         // (refactor.cu:301-308), colour from the walker's init stream.
-        pass0 = 0u;
-        v2 h = rfk_hammersley(gid, a.total_walkers);
+        v2 h = rfk_hammersley(gid, total);
         v2 pix = {(h.x + Real(1)) / Real(2) * Real(a.width), (h.y + Real(1)) / Real(2) * Real(a.height)};
         v2 pl = fl::affine(P + 6, pix);
         x = pl.x;
         y = pl.y;
-        c = rfk_u01(philox4x32_10({0u, gid, 0u, RFK_TAG_INIT}, a.seed, a.stream).x);
+        c = rfk_u01(philox4x32_10({0u, gid, 0u, RFK_TAG_INIT}, a.seed, key1).x);
     } else {
-        pass0 = a.block_pass[block];
-        x = Real(a.wx[gid]);
-        y = Real(a.wy[gid]);
-        c = Real(a.wc[gid]);
-    }
+        x = Real(wx[gid]);
+        y = Real(wy[gid]);
+        c = Real(wc[gid]);
 #if RFK_USE_CHAOS
-    if (!(a.flags & RFK_FLAG_INIT) && a.wlast) last = a.wlast[gid];
+        if (wl) last = wl[gid];
 #endif
+    }
     __syncthreads();
 
     rng_t rs;
-    rs.init(a.seed, a.stream, gid, RFK_TAG_LANE);
+    rs.init(a.seed, key1, gid, RFK_TAG_LANE);
     u64 hits = 0, draws = 0, bad = 0;
     u32 done = 0, buf = 0;
     u32x4 pick_cache = {0u, 0u, 0u, 0u}, perm_cache = {0u, 0u, 0u, 0u};
 
-    for (u32 i = 0; i < a.npasses; i++) {
+    const u32 ntotal = nfuse + npasses;
+    for (u32 i = 0; i < ntotal; i++) {
         const u32 pass = pass0 + i;
         rs.begin(pass);
         v3 pt = {x, y, c};
@@ -115,7 +185,8 @@ extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate(rfk_iter_args 
         pt = res;
         last = xid;
 #else
-        xid = rfk_select_xform(P, rfk_wave_pick(pass, block, wave, a.seed, a.stream, pick_cache, i == 0 || (pass & 3u) == 0u));
+        xid = rfk_select_xform(P, rfk_u01(rfk_wave_pick(pass, block, wave, a.seed, key1, pick_cache,
+                                                        i == 0 || (pass & 3u) == 0u)));
         xid = __builtin_amdgcn_readfirstlane(xid);
         op = rfk_dispatch(xid, P, pt, rs);
 #endif
@@ -126,19 +197,30 @@ extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate(rfk_iter_args 
             op = Real(0);
             bad++;
         }
-        if (draw) {
+        if (draw && i >= nfuse) {
             v3 d = pt;
 #if RFK_HAS_FINAL
             // the final xform acts on a copy; opacity stays the chosen xform's (refactor.cu:351-354)
             rfk_dispatch(RFK_NUM_XFORMS, P, d, rs);
 #endif
-            rfk_draw_t dr = rfk_make_draw(S, s_pal, a.width, a.height, float(d.x), float(d.y), float(d.z), float(op));
+            rfk_draw_t dr = rfk_make_draw(S, sm.pal, a.width, a.height, float(d.x), float(d.y), float(d.z), float(op));
             if (dr.hit) {
-                rfk_commit_draw(a.hist, dr);
+#if RFK_BIN_TILE_LOG2
+                rfk_tile_draw(sm, hist, dr);
+#else
+                rfk_commit_draw(hist, dr);
+#endif
                 hits += dr.w;
                 draws++;
             }
         }
+#if RFK_BIN_TILE_LOG2
+        if (draw && ((i + 1u) % RFK_TILE_FLUSH) == 0u) {
+            __syncthreads();
+            rfk_tile_flush(sm, hist);
+            __syncthreads();
+        }
+#endif
         if (trace) {
             float* tr = a.trace + ((size_t)i * a.total_walkers + gid) * 4u;
             tr[0] = float(pt.x);
@@ -158,17 +240,18 @@ extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate(rfk_iter_args 
         y = pt.y;
         c = pt.z;
 #else
-        if (i == 0 || (pass & 1u) == 0u) perm_cache = philox4x32_10({pass >> 1, block, 0u, RFK_TAG_PERM}, a.seed, a.stream);
+        if (i == 0 || (pass & 1u) == 0u)
+            perm_cache = philox4x32_10({pass >> 1, block, 0u, RFK_TAG_PERM}, a.seed, key1);
         const u32 wa = (pass & 1u) ? perm_cache.z : perm_cache.x;
         const u32 wb = (pass & 1u) ? perm_cache.w : perm_cache.y;
         const u32 slot = rfk_permute(t, wa, wb, RFK_TPB_BITS);
-        s_pts[buf][0][slot] = pt.x;
-        s_pts[buf][1][slot] = pt.y;
-        s_pts[buf][2][slot] = pt.z;
+        sm.pts[buf][0][slot] = pt.x;
+        sm.pts[buf][1][slot] = pt.y;
+        sm.pts[buf][2][slot] = pt.z;
         __syncthreads();
-        x = s_pts[buf][0][t];
-        y = s_pts[buf][1][t];
-        c = s_pts[buf][2][t];
+        x = sm.pts[buf][0][t];
+        y = sm.pts[buf][1][t];
+        c = sm.pts[buf][2][t];
         buf ^= 1u;
 #endif
         done++;
@@ -176,39 +259,99 @@ extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate(rfk_iter_args 
         if (bailout && ((i + 1u) & (a.check_every - 1u)) == 0u) {
             // bailouts (refactor.cu:443-458): quality target per block, time
             // limit on the device clock, host stop flag in coherent host memory
-            atomicAdd(&s_hits, hits);
+            atomicAdd(&sm.hits, hits);
             hits = 0;
             __syncthreads();
             if (t == 0) {
                 bool stop = a.stop_flag &&
                             __hip_atomic_load((u32*)a.stop_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u;
-                if (a.time_limit_ticks && wall_clock64() - s_start >= a.time_limit_ticks) stop = true;
-                if (a.quality_target && s_hits >= a.quality_target) stop = true;
-                s_stop = stop ? 1u : 0u;
+                if (a.time_limit_ticks && wall_clock64() - sm.start >= a.time_limit_ticks) stop = true;
+                if (a.quality_target && sm.hits >= a.quality_target) stop = true;
+                sm.stop = stop ? 1u : 0u;
             }
             __syncthreads();
-            if (s_stop) break;
+            if (sm.stop) break;
         }
     }
-
-    a.wx[gid] = float(x);
-    a.wy[gid] = float(y);
-    a.wc[gid] = float(c);
-#if RFK_USE_CHAOS
-    if (a.wlast) a.wlast[gid] = last;
+#if RFK_BIN_TILE_LOG2
+    if (draw) {
+        __syncthreads();
+        rfk_tile_flush(sm, hist);
+    }
 #endif
-    atomicAdd(&s_hits, hits);
-    if (draws) atomicAdd(&s_draws, draws);
-    if (bad) atomicAdd(&s_bad, bad);
+
+    if (wx) {
+        wx[gid] = float(x);
+        wy[gid] = float(y);
+        wc[gid] = float(c);
+#if RFK_USE_CHAOS
+        if (wl) wl[gid] = last;
+#endif
+    }
+    atomicAdd(&sm.hits, hits);
+    if (draws) atomicAdd(&sm.draws, draws);
+    if (bad) atomicAdd(&sm.bad, bad);
     __syncthreads();
-    if (t == 0) {
+    return done;
+}
+
+extern "C" __global__ void rfk_precalc(rfk_iter_args a) {
+    const u32 s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.nsamples) return;
+    Real* P = (Real*)a.params + (size_t)s * a.param_stride;
+    rng_t rs;
+    rs.init(a.seed, a.row_stream ? a.row_stream[s] : a.stream, 0u, RFK_TAG_JITTER);
+    rs.begin(a.row_local ? a.row_local[s] : s);
+    rfk_precalc_sample(P, rs);
+}
+
+extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate(rfk_iter_args a) {
+    __shared__ rfk_shared sm;
+    const u32 block = blockIdx.x;
+    const Real* __restrict__ P = (const Real*)a.params + (size_t)block * a.param_stride;
+    const bool init = (a.flags & RFK_FLAG_INIT) != 0u;
+    const u32 pass0 = init ? 0u : a.block_pass[block];
+    const u32 done = rfk_block(a, sm, P, a.palettes + (size_t)block * 256u, block, a.stream, a.hist, init,
+                               a.total_walkers, pass0, 0u, a.npasses, a.flags, a.wx, a.wy, a.wc, a.wlast);
+    if (threadIdx.x == 0) {
         a.block_pass[block] = pass0 + done;
-        atomicAdd(&a.counters[RFK_CNT_HITS], s_hits);
+        atomicAdd(&a.counters[RFK_CNT_HITS], sm.hits);
         atomicAdd(&a.counters[RFK_CNT_PASSES], (u64)done * RFK_TPB);
-        atomicMin(&a.counters[RFK_CNT_START], s_start);
+        atomicMin(&a.counters[RFK_CNT_START], sm.start);
         atomicMax(&a.counters[RFK_CNT_STOP], wall_clock64());
-        if (s_stop) atomicAdd(&a.counters[RFK_CNT_STOPPED], 1ull);
-        if (s_draws) atomicAdd(&a.counters[RFK_CNT_DRAWS], s_draws);
-        if (s_bad) atomicAdd(&a.counters[RFK_CNT_BAD], s_bad);
+        if (sm.stop) atomicAdd(&a.counters[RFK_CNT_STOPPED], 1ull);
+        if (sm.draws) atomicAdd(&a.counters[RFK_CNT_DRAWS], sm.draws);
+        if (sm.bad) atomicAdd(&a.counters[RFK_CNT_BAD], sm.bad);
+    }
+}
+
+// Persistent animation kernel: work item i (popped from a.queue in the
+// host-chosen order, longest first) is block b of frame f. Each item seeds its
+// walkers, runs the frame's fuse passes without drawing and then the frame's
+// binned passes into the frame's histogram; streams are keyed by (seed, f).
+extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate_batch(rfk_iter_args a, rfk_batch_args b) {
+    __shared__ rfk_shared sm;
+    for (;;) {
+        if (threadIdx.x == 0) sm.item = atomicAdd(b.queue, 1u);
+        __syncthreads();
+        const u32 item = sm.item;
+        __syncthreads();
+        if (item >= b.nitems) break;  // uniform: every wave of the group leaves together
+        const u32 f = b.item_frame[item], blk = b.item_block[item];
+        const u32 row = b.frame_row0[f] + blk;
+        const Real* __restrict__ P = (const Real*)a.params + (size_t)row * a.param_stride;
+        const u32* pal = a.palettes + (size_t)row * 256u;
+        u32* hist = a.hist + (size_t)f * b.frame_pixels * 4u;
+        const u32 total = b.frame_blocks[f] * RFK_TPB;
+        const u32 done = rfk_block(a, sm, P, pal, blk, f, hist, true, total, 0u, b.fuse, b.frame_passes[f],
+                                   RFK_FLAG_DRAW, nullptr, nullptr, nullptr, nullptr) - b.fuse;
+        if (threadIdx.x == 0) {
+            u64* fc = b.frame_counters + (size_t)f * 4u;
+            atomicAdd(&fc[0], sm.hits);
+            atomicAdd(&fc[1], sm.draws);
+            atomicAdd(&fc[2], (u64)done * RFK_TPB);
+            atomicAdd(&fc[3], sm.bad);
+        }
+        __syncthreads();
     }
 }

@@ -50,18 +50,30 @@ __device__ __forceinline__ double seg_eval(const double* seg4, double t) {
 
 // The reference gives block b of an n-block grid t = (b % seg_size) / (n / nseg)
 // inside segment b / seg_size (refactor.cu:266-270).
+// With `times` given, sample s sits at global time times[s] in [0, nseg):
+// segment floor(times[s]), local t = the fraction (animation batches).
 template <typename OutT>
 __global__ void eval_samples_kernel(const double* __restrict__ segments, uint32_t sample_size, uint32_t size_reals,
                                     uint32_t nseg, uint32_t nsamples, const int8_t* __restrict__ kind,
-                                    OutT* __restrict__ params, uint32_t param_stride, uint32_t* __restrict__ palettes) {
+                                    OutT* __restrict__ params, uint32_t param_stride, uint32_t* __restrict__ palettes,
+                                    const double* __restrict__ times) {
     const uint32_t per = size_reals + 256u;
     const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= uint64_t(per) * nsamples) return;
     const uint32_t s = uint32_t(g / per), i = uint32_t(g % per);
-    const uint32_t seg_size = max(nsamples / nseg, 1u);
-    uint32_t seg = s / seg_size;
-    if (seg >= nseg) seg = nseg - 1;
-    const double t = double(s % seg_size) / double(max(nsamples / nseg, 1u));
+    uint32_t seg;
+    double t;
+    if (times) {
+        const double gt = fmin(fmax(times[s], 0.0), double(nseg));
+        seg = uint32_t(gt);
+        if (seg >= nseg) seg = nseg - 1;
+        t = gt - double(seg);
+    } else {
+        const uint32_t seg_size = max(nsamples / nseg, 1u);
+        seg = s / seg_size;
+        if (seg >= nseg) seg = nseg - 1;
+        t = double(s % seg_size) / double(max(nsamples / nseg, 1u));
+    }
     const double* base = segments + uint64_t(seg) * sample_size * 4ull;
     if (i < size_reals) {
         const int k = kind[i];
@@ -303,14 +315,25 @@ hipError_t catmull(const double* samples, uint32_t sample_size, uint32_t nseg, d
 
 hipError_t eval_samples(const double* segments, uint32_t sample_size, uint32_t size_reals, uint32_t nseg,
                         uint32_t nsamples, const int8_t* kind, int precision, void* params, uint32_t param_stride,
-                        uint32_t* palettes, hipStream_t s) {
+                        uint32_t* palettes, hipStream_t s, const double* times) {
     size_t n = size_t(size_reals + 256u) * nsamples;
     if (precision == 1)
         eval_samples_kernel<double><<<nblocks(n), kBlock, 0, s>>>(segments, sample_size, size_reals, nseg, nsamples,
-                                                                   kind, (double*)params, param_stride, palettes);
+                                                                   kind, (double*)params, param_stride, palettes, times);
     else
         eval_samples_kernel<float><<<nblocks(n), kBlock, 0, s>>>(segments, sample_size, size_reals, nseg, nsamples,
-                                                                  kind, (float*)params, param_stride, palettes);
+                                                                  kind, (float*)params, param_stride, palettes, times);
+    return hipGetLastError();
+}
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+
+hipError_t fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    unsigned g = std::min(nblocks(n), 8192u);
+    fill_u32_kernel<<<g, kBlock, 0, s>>>(p, v, n);
     return hipGetLastError();
 }
 

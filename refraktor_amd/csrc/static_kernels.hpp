@@ -17,7 +17,10 @@ hipError_t catmull(const double* samples, uint32_t sample_size, uint32_t nseg, d
 // affine whose first real sits at i-(kind-1).
 hipError_t eval_samples(const double* segments, uint32_t sample_size, uint32_t size_reals, uint32_t nseg,
                         uint32_t nsamples, const int8_t* kind, int precision, void* params, uint32_t param_stride,
-                        uint32_t* palettes, hipStream_t s);
+                        uint32_t* palettes, hipStream_t s, const double* times = nullptr);
+
+// graph-safe clear / fill (a kernel instead of hipMemsetAsync)
+hipError_t fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 
 // Bin n plane-space points through a screen transform / palette.
 hipError_t bin_points(const float* screen6, const uint32_t* palette, const float* pts, size_t n, uint32_t* hist,

@@ -92,14 +92,23 @@ class VariationDB:
     def flame(self, flame: dict | str) -> "Flame":
         return Flame(self, flame)
 
+    def align(self, flames: list) -> list["Flame"]:
+        """structurally aligned copies of keyframes (one kernel serves them all)"""
+        fs = [f if isinstance(f, Flame) else self.flame(f) for f in flames]
+        arr = (C.c_void_p * len(fs))(*[f.handle for f in fs])
+        out = (C.c_void_p * len(fs))()
+        check(self._lib.rfk_flames_align(self.handle, arr, len(fs), out))
+        return [Flame(self, None, handle=out[i]) for i in range(len(fs))]
+
     def flame_from_flam3(self, xml: str) -> "Flame":
         h = C.c_void_p()
         check(self._lib.rfk_flame_from_flam3(self.handle, xml.encode(), C.byref(h)))
         return Flame(self, None, handle=h.value)
 
-    def compile(self, flame, precision: int = 0, threads_per_block: int = 256, atomic_width: int = 32) -> "Kernel":
+    def compile(self, flame, precision: int = 0, threads_per_block: int = 256, atomic_width: int = 32,
+                bin_tile_log2: int = 0) -> "Kernel":
         f = flame if isinstance(flame, Flame) else self.flame(flame)
-        return Kernel(self, f, precision, threads_per_block, atomic_width)
+        return Kernel(self, f, precision, threads_per_block, atomic_width, bin_tile_log2)
 
     def __del__(self):
         try:
@@ -156,9 +165,10 @@ class Flame:
                                         C.byref(need)))
         return buf.value.decode()
 
-    def compile_check(self, precision: int = 0, threads_per_block: int = 256, atomic_width: int = 32) -> int:
+    def compile_check(self, precision: int = 0, threads_per_block: int = 256, atomic_width: int = 32,
+                      bin_tile_log2: int = 0) -> int:
         """hiprtc-compile the generated source without a device; returns code-object bytes"""
-        cfg = _lib.ExecConfig(threads_per_block, atomic_width, 0, 0)
+        cfg = _lib.ExecConfig(threads_per_block, atomic_width, bin_tile_log2, 0)
         n = C.c_size_t()
         log = C.create_string_buffer(1 << 16)
         rc = self._lib.rfk_compile_check(self.db.handle, self.handle, precision, C.byref(cfg), C.byref(n), log,
@@ -176,10 +186,11 @@ class Flame:
 
 
 class Kernel:
-    def __init__(self, db: VariationDB, flame: Flame, precision: int, tpb: int, atomic_width: int):
+    def __init__(self, db: VariationDB, flame: Flame, precision: int, tpb: int, atomic_width: int,
+                 bin_tile_log2: int = 0):
         self.db, self.flame, self._lib = db, flame, db._lib
         self.precision = precision
-        cfg = _lib.ExecConfig(tpb, atomic_width, 0, 0)
+        cfg = _lib.ExecConfig(tpb, atomic_width, bin_tile_log2, 0)
         h = C.c_void_p()
         log = C.create_string_buffer(1 << 16)
         rc = self._lib.rfk_compile(db.handle, flame.handle, precision, C.byref(cfg), C.byref(h), log, len(log))
@@ -194,12 +205,25 @@ class Kernel:
         return self.flame.pack_samples(width, height, start, offset, count)
 
     def warmup(self, samples: np.ndarray, width: int, height: int, seed: int, fuse: int = 20,
-               walkers: int = 1 << 20, temporal_multiplier: int = 1) -> "State":
+               walkers: int = 1 << 20, temporal_multiplier: int = 1, stream_id: int = 0) -> "State":
         samples = np.ascontiguousarray(samples, dtype=np.float64)
         h = C.c_void_p()
-        check(self._lib.rfk_warmup(self.handle, None, samples.ctypes.data_as(C.POINTER(C.c_double)), samples.size,
-                                   width, height, seed, fuse, walkers, temporal_multiplier, C.byref(h)))
+        check(self._lib.rfk_warmup_ex(self.handle, None, samples.ctypes.data_as(C.POINTER(C.c_double)), samples.size,
+                                      width, height, seed, stream_id, fuse, walkers, temporal_multiplier,
+                                      C.byref(h)))
         return State(self, h.value)
+
+    def batch(self, samples: np.ndarray, frames: int, iters, width: int, height: int, walkers: int = 1 << 18,
+              seed: int = 1, fuse: int = 20) -> "Batch":
+        """`frames` frames over the Catmull-Rom segments of `samples` (>= 4
+        packed samples) in one persistent launch; iters: per-frame budgets"""
+        samples = np.ascontiguousarray(samples, dtype=np.float64)
+        it = np.ascontiguousarray(np.broadcast_to(np.asarray(iters, dtype=np.uint64), (frames,)))
+        h = C.c_void_p()
+        check(self._lib.rfk_batch_create(self.handle, None, samples.ctypes.data_as(C.POINTER(C.c_double)),
+                                         samples.size, frames, it.ctypes.data_as(C.POINTER(C.c_uint64)), width,
+                                         height, walkers, seed, fuse, C.byref(h)))
+        return Batch(self, h.value)
 
     def __del__(self):
         try:
@@ -298,6 +322,36 @@ class State:
         try:
             if self.handle:
                 self._lib.rfk_state_free(self.handle)
+        except Exception:
+            pass
+
+
+class Batch:
+    """frames of an animation rendered by one persistent launch (optionally a hipGraph replay)"""
+
+    def __init__(self, kernel: Kernel, handle: int):
+        self.kernel, self._lib, self.handle = kernel, kernel._lib, handle
+        n = [C.c_uint32() for _ in range(5)]
+        check(self._lib.rfk_batch_info(self.handle, *[C.byref(v) for v in n]))
+        self.frames, self.width, self.height, self.grid, self.items = (v.value for v in n)
+
+    def run(self, graph: bool = True) -> float:
+        ms = C.c_double()
+        check(self._lib.rfk_batch_run(self.handle, 1 if graph else 0, C.byref(ms)))
+        return ms.value
+
+    def frame(self, f: int) -> dict:
+        inf = _lib.BatchFrameInfo()
+        check(self._lib.rfk_batch_frame(self.handle, f, C.byref(inf)))
+        return {name: getattr(inf, name) for name, _ in inf._fields_}
+
+    def histogram(self, f: int) -> np.ndarray:
+        return download(self.frame(f)["histogram"], np.uint32, (self.height, self.width, 4))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self._lib.rfk_batch_free(self.handle)
         except Exception:
             pass
 
