@@ -94,10 +94,10 @@ int rfk_flames_align(const rfk_db* db, rfk_flame* const* in, int n, rfk_flame** 
 #define RFK_PRECISION_F32 0
 #define RFK_PRECISION_F64 1
 typedef struct rfk_exec_config {
-    uint32_t threads_per_block;  /* walkers per block: 64, 256 or 1024 (0 = 256) */
-    uint32_t atomic_width;       /* 32: four u32 atomics per draw; 64: two u64 (0 = 32) */
-    uint32_t bin_tile_log2;      /* reserved for LDS bin tiles (0 = off)       */
-    uint32_t reserved;
+    uint32_t threads_per_block;  /* walkers per block: 64..1024, power of 2 (0 = 256) */
+    uint32_t atomic_width;       /* 32-bit bins: four u32 atomics per draw (32) or two u64 (64); 0 = 32 */
+    uint32_t bin_tile_log2;      /* LDS bin cache of 2^n slots per block, 6..12 (0 = off) */
+    uint32_t hist_bits;          /* histogram bin width: 32 (u32 RGBA) or 64 (u64 RGBA); 0 = 32 */
 } rfk_exec_config;
 int rfk_compile(const rfk_db* db, const rfk_flame* f, int precision, const rfk_exec_config* cfg,
                 rfk_kernel** out, char* log, size_t log_cap);
@@ -157,10 +157,11 @@ int rfk_autotune(const rfk_db* db, const rfk_flame* f, int precision, const doub
  * on its first call and replay it afterwards.                               */
 typedef struct rfk_batch rfk_batch;
 typedef struct rfk_batch_frame_info {
-    uint32_t* histogram;      /* device, height*width*4 u32                   */
+    uint32_t* histogram;      /* device, height*width*4 bins of hist_bits     */
     double quality;           /* mean density (draw units)                    */
     uint64_t total_draws, total_passes, bad_values;
     uint32_t passes_per_walker, walkers;
+    uint32_t hist_bits;       /* 32 or 64                                     */
 } rfk_batch_frame_info;
 int rfk_batch_create(rfk_kernel* k, void* stream, const double* samples, size_t n_reals, uint32_t frames,
                      const uint64_t* iters_per_frame /* host */, uint32_t width, uint32_t height,
@@ -221,7 +222,7 @@ typedef struct rfk_state_info {
     uint32_t size_reals, param_stride;
     uint32_t seed;
     uint64_t passes_done;     /* passes per walker completed (block 0)       */
-    uint32_t* histogram;      /* device, height*width*4 u32 (R,G,B,W)        */
+    uint32_t* histogram;      /* device, height*width*4 bins (R,G,B,W) of hist_bits */
     float* params;            /* device, blocks*param_stride floats          */
     uint32_t* palettes;       /* device, blocks*256 packed RGBA8             */
     float* walker_x;          /* device, walkers                             */
@@ -229,6 +230,7 @@ typedef struct rfk_state_info {
     float* walker_c;
     void* stream;             /* hipStream_t                                 */
     double quality;           /* accumulated mean density over all bin calls */
+    uint32_t hist_bits;       /* 32 or 64                                    */
 } rfk_state_info;
 int rfk_state_get_info(const rfk_state* s, rfk_state_info* out);
 /* Zero the histogram and counters (walkers keep their positions). */
@@ -291,9 +293,10 @@ int rfk_denoiser_load(rfk_denoiser* d, const char* path);
 int rfk_denoiser_get_params(const rfk_denoiser* d, float* w /* host */, float* b /* host */);
 int rfk_denoiser_set_params(rfk_denoiser* d, const float* w /* host */, const float* b /* host */);
 int rfk_denoiser_get_grads(const rfk_denoiser* d, float* gw /* host */, float* gb /* host */);
-/* features + base image from a fixed-point histogram and its float4 tonemap */
-int rfk_denoiser_features(const uint32_t* hist, const float* tonemapped_rgba, size_t npix, void* feat, float* base,
-                          void* stream);
+/* features + base image from a fixed-point histogram (hist_bits 32 or 64)
+ * and its float4 tonemap */
+int rfk_denoiser_features(const uint32_t* hist, int hist_bits, const float* tonemapped_rgba, size_t npix, void* feat,
+                          float* base, void* stream);
 /* out_rgb = clamp(base + U-Net(feat), 0, 1) */
 int rfk_denoiser_forward(rfk_denoiser* d, const void* feat, const float* base, int N, int H, int W, float* out_rgb,
                          void* stream);

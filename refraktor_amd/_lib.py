@@ -25,7 +25,7 @@ PIXEL_BYTES = {0: 16, 1: 8, 2: 12, 3: 6, 4: 4}
 
 class ExecConfig(C.Structure):
     _fields_ = [("threads_per_block", C.c_uint32), ("atomic_width", C.c_uint32),
-                ("bin_tile_log2", C.c_uint32), ("reserved", C.c_uint32)]
+                ("bin_tile_log2", C.c_uint32), ("hist_bits", C.c_uint32)]
 
 
 class Bailout(C.Structure):
@@ -48,13 +48,13 @@ class StateInfo(C.Structure):
                 ("param_stride", C.c_uint32), ("seed", C.c_uint32), ("passes_done", C.c_uint64),
                 ("histogram", C.c_void_p), ("params", C.c_void_p), ("palettes", C.c_void_p),
                 ("walker_x", C.c_void_p), ("walker_y", C.c_void_p), ("walker_c", C.c_void_p),
-                ("stream", C.c_void_p), ("quality", C.c_double)]
+                ("stream", C.c_void_p), ("quality", C.c_double), ("hist_bits", C.c_uint32)]
 
 
 class BatchFrameInfo(C.Structure):
     _fields_ = [("histogram", C.c_void_p), ("quality", C.c_double), ("total_draws", C.c_uint64),
                 ("total_passes", C.c_uint64), ("bad_values", C.c_uint64), ("passes_per_walker", C.c_uint32),
-                ("walkers", C.c_uint32)]
+                ("walkers", C.c_uint32), ("hist_bits", C.c_uint32)]
 
 
 class TuneCandidate(C.Structure):
@@ -135,7 +135,7 @@ _SIGS = [
     ("rfk_denoiser_get_params", C.c_int, [_P, _P, _P]),
     ("rfk_denoiser_set_params", C.c_int, [_P, _P, _P]),
     ("rfk_denoiser_get_grads", C.c_int, [_P, _P, _P]),
-    ("rfk_denoiser_features", C.c_int, [_P, _P, C.c_size_t, _P, _P, _P]),
+    ("rfk_denoiser_features", C.c_int, [_P, C.c_int, _P, C.c_size_t, _P, _P, _P]),
     ("rfk_denoiser_forward", C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
     ("rfk_denoiser_train_step", C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_float,
                                           C.POINTER(C.c_float), _P]),

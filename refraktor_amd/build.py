@@ -22,7 +22,7 @@ HOST_SOURCES = ["json.cpp", "yaml_lite.cpp", "flang.cpp", "vardb.cpp", "flame.cp
 HIP_SOURCES = ["api.cpp", "kernels_static.hip", "unet.hip", "denoiser.cpp"]
 # hipcc flags: gfx950 only, packed FP32 instructions disabled
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++20",
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++20", "-ffp-contract=fast-honor-pragmas",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++20", "-Wall", "-Wno-unused-function",
@@ -32,7 +32,8 @@ HOST_FLAGS = ["-O2", "-fPIC", "-std=c++20", "-Wall", "-Wno-unused-function",
 def _embed_device_sources() -> None:
     parts = []
     for var, name in (("kFlameDeviceSrc", "flame_device.h"), ("kFlameKernelsSrc", "flame_kernels.h")):
-        text = (CSRC / "device" / name).read_text()
+        # the headers are concatenated into one hiprtc source: drop include guards
+        text = "\n".join(l for l in (CSRC / "device" / name).read_text().splitlines() if l.strip() != "#pragma once")
         # split into chunks below the compilers' string literal limits
         chunks = [text[i:i + 12000] for i in range(0, len(text), 12000)]
         lits = "\n".join(f'R"RFKSRC({c})RFKSRC"' for c in chunks)

@@ -69,6 +69,8 @@ struct rfk_state {
     uint64_t pending_passes = 0;
     double wallclock_khz = 100000.0;
     uint32_t stream_id = 0;
+    bool hist64 = false;
+    size_t hist_words() const { return size_t(width) * height * (hist64 ? 8u : 4u); }
 };
 
 struct rfk_batch {
@@ -90,6 +92,8 @@ struct rfk_batch {
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
+    bool hist64 = false;
+    size_t frame_words() const { return size_t(width) * height * (hist64 ? 8u : 4u); }
 };
 
 namespace {
@@ -313,6 +317,9 @@ static int make_options(const rfk_exec_config* cfg, int precision, rfk::compile_
     if (o.atomic_width != 32 && o.atomic_width != 64) return fail(RFK_ERR_INVALID, "atomic_width must be 32 or 64");
     if (precision != RFK_PRECISION_F32 && precision != RFK_PRECISION_F64) return fail(RFK_ERR_INVALID, "bad precision");
     o.bin_tile_log2 = cfg ? cfg->bin_tile_log2 : 0;
+    const uint32_t hb = (cfg && cfg->hist_bits) ? cfg->hist_bits : 32u;
+    if (hb != 32 && hb != 64) return fail(RFK_ERR_INVALID, "hist_bits must be 32 or 64");
+    o.hist64 = hb == 64;
     if (o.bin_tile_log2 != 0 && (o.bin_tile_log2 < 6 || o.bin_tile_log2 > 12))
         return fail(RFK_ERR_INVALID, "bin_tile_log2 must be 0 or 6..12");
     // static LDS of one workgroup: palette, two point buffers, bin cache
@@ -496,7 +503,8 @@ int rfk_warmup_ex(rfk_kernel* k, void* stream, const double* samples, size_t n_r
     RFK_HIPC(dalloc(&s->wc, s->walkers));
     if (k->L.chaos_base >= 0) RFK_HIPC(dalloc(&s->wlast, s->walkers));
     RFK_HIPC(dalloc(&s->block_pass, s->blocks));
-    RFK_HIPC(dalloc(&s->hist, npix * 4));
+    s->hist64 = k->opt.hist64;
+    RFK_HIPC(dalloc(&s->hist, s->hist_words()));
     RFK_HIPC(dalloc(&s->counters, 8));
     RFK_HIPC(dalloc(&s->dstats, 98));
     RFK_HIPC(hipHostMalloc(reinterpret_cast<void**>(&s->counters_host), 128 * sizeof(unsigned long long),
@@ -506,7 +514,7 @@ int rfk_warmup_ex(rfk_kernel* k, void* stream, const double* samples, size_t n_r
     RFK_HIPC(hipHostGetDevicePointer(reinterpret_cast<void**>(&s->stop_dev), s->stop_host, 0));
     RFK_HIPC(hipEventCreate(&s->ev0));
     RFK_HIPC(hipEventCreate(&s->ev1));
-    RFK_HIPC(hipMemsetAsync(s->hist, 0, npix * 16, s->stream));
+    RFK_HIPC(hipMemsetAsync(s->hist, 0, s->hist_words() * 4, s->stream));
     RFK_HIPC(hipMemsetAsync(s->block_pass, 0, size_t(s->blocks) * 4, s->stream));
     if (s->wlast) RFK_HIPC(hipMemsetAsync(s->wlast, 0xFF, size_t(s->walkers) * 4, s->stream));
 
@@ -579,7 +587,8 @@ int rfk_bin_async(rfk_kernel* k, rfk_state* s, const rfk_bailout* bo) {
     RFK_HIP(hipEventRecord(s->ev1, s->stream));
     // density statistics after every bin, as the reference does (flame_compiler.cpp:928-937)
     RFK_HIP(hipMemsetAsync(s->dstats, 0, 98 * sizeof(unsigned long long), s->stream));
-    RFK_HIP(rfk::sk::density_histogram(s->hist, size_t(s->width) * s->height, s->dstats, s->dstats + 97, s->stream));
+    RFK_HIP(rfk::sk::density_histogram(s->hist, s->hist64, size_t(s->width) * s->height, s->dstats, s->dstats + 97,
+                                       s->stream));
     RFK_HIP(hipMemcpyAsync(s->counters_host, s->counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            s->stream));
     RFK_HIP(hipMemcpyAsync(s->counters_host + 8, s->dstats, 98 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -629,7 +638,7 @@ int rfk_abort(rfk_state* s) {
 int rfk_bin_points(rfk_state* s, const float* xyco, size_t n) {
     if (!s || (!xyco && n)) return fail(RFK_ERR_INVALID, "null argument");
     if (s->precision != 0) return fail(RFK_ERR_INVALID, "bin_points needs an f32 state");
-    RFK_HIP(rfk::sk::bin_points(static_cast<const float*>(s->params), s->palettes, xyco, n, s->hist, s->width,
+    RFK_HIP(rfk::sk::bin_points(static_cast<const float*>(s->params), s->palettes, xyco, n, s->hist, s->hist64, s->width,
                                 s->height, s->stream));
     RFK_HIP(hipStreamSynchronize(s->stream));
     return RFK_OK;
@@ -669,12 +678,13 @@ int rfk_state_get_info(const rfk_state* s, rfk_state_info* out) {
     out->walker_c = s->wc;
     out->stream = s->stream;
     out->quality = s->quality;
+    out->hist_bits = s->hist64 ? 64u : 32u;
     return RFK_OK;
 }
 
 int rfk_state_clear_histogram(rfk_state* s) {
     if (!s) return fail(RFK_ERR_INVALID, "null state");
-    RFK_HIP(hipMemsetAsync(s->hist, 0, size_t(s->width) * s->height * 16, s->stream));
+    RFK_HIP(hipMemsetAsync(s->hist, 0, s->hist_words() * 4, s->stream));
     RFK_HIP(hipStreamSynchronize(s->stream));
     s->quality = 0.0;
     return RFK_OK;
@@ -789,7 +799,8 @@ int rfk_batch_create(rfk_kernel* k, void* stream, const double* samples, size_t 
     const size_t fpix = size_t(width) * height;
     RFK_HIPB(hipMalloc(&B->params, size_t(B->total_blocks) * B->param_stride * elem));
     RFK_HIPB(dalloc(&B->palettes, size_t(B->total_blocks) * 256));
-    RFK_HIPB(dalloc(&B->hist, fpix * 4 * frames));
+    B->hist64 = k->opt.hist64;
+    RFK_HIPB(dalloc(&B->hist, B->frame_words() * frames));
     RFK_HIPB(dalloc(&B->d_frame_row0, frames));
     RFK_HIPB(dalloc(&B->d_frame_blocks, frames));
     RFK_HIPB(dalloc(&B->d_frame_passes, frames));
@@ -902,7 +913,7 @@ static int batch_enqueue(rfk_batch* B) {
     rfk_kernel* k = B->k;
     const size_t fpix = size_t(B->width) * B->height;
     // clears are kernels, not memsets, so the sequence can live in a graph
-    RFK_HIP(rfk::sk::fill_u32(B->hist, 0u, fpix * 4 * B->frames, B->stream));
+    RFK_HIP(rfk::sk::fill_u32(B->hist, 0u, B->frame_words() * B->frames, B->stream));
     RFK_HIP(rfk::sk::fill_u32(B->d_queue, 0u, 1, B->stream));
     RFK_HIP(rfk::sk::fill_u32(reinterpret_cast<uint32_t*>(B->d_frame_counters), 0u, size_t(B->frames) * 8, B->stream));
     rfk_iter_args a;
@@ -966,13 +977,14 @@ int rfk_batch_frame(const rfk_batch* B, uint32_t frame, rfk_batch_frame_info* ou
     if (!B || !out || frame >= B->frames) return fail(RFK_ERR_INVALID, "bad argument");
     const unsigned long long* c = B->h_frame_counters + size_t(frame) * 4;
     const double bins = double(B->width) * B->height;
-    out->histogram = B->hist + size_t(frame) * B->width * B->height * 4;
+    out->histogram = B->hist + B->frame_words() * frame;
     out->quality = double(c[0]) / (bins * 255.0);
     out->total_draws = c[1];
     out->total_passes = c[2];
     out->bad_values = c[3];
     out->passes_per_walker = B->frame_passes[frame];
     out->walkers = B->blocks_per_frame * B->tpb;
+    out->hist_bits = B->hist64 ? 64u : 32u;
     return RFK_OK;
 }
 
@@ -992,7 +1004,8 @@ int rfk_batch_info(const rfk_batch* B, uint32_t* frames, uint32_t* width, uint32
 int rfk_density_histogram(rfk_state* s, uint64_t* bins97, double* max_density) {
     if (!s) return fail(RFK_ERR_INVALID, "null state");
     RFK_HIP(hipMemsetAsync(s->dstats, 0, 98 * sizeof(unsigned long long), s->stream));
-    RFK_HIP(rfk::sk::density_histogram(s->hist, size_t(s->width) * s->height, s->dstats, s->dstats + 97, s->stream));
+    RFK_HIP(rfk::sk::density_histogram(s->hist, s->hist64, size_t(s->width) * s->height, s->dstats, s->dstats + 97,
+                                       s->stream));
     RFK_HIP(hipMemcpyAsync(s->counters_host + 8, s->dstats, 98 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            s->stream));
     RFK_HIP(hipStreamSynchronize(s->stream));
@@ -1031,7 +1044,7 @@ int rfk_density_estimate(rfk_state* s, float max_radius, float min_radius, float
     }
     e = hipMemcpyAsync(d_iz, inv_z.data(), inv_z.size() * sizeof(float), hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess)
-        e = rfk::sk::density_estimate(s->hist, s->width, s->height, max_radius, min_radius, curve, d_iz, nlevels,
+        e = rfk::sk::density_estimate(s->hist, s->hist64, s->width, s->height, max_radius, min_radius, curve, d_iz, nlevels,
                                       radius, d_lvl, out_rgba, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     (void)hipFree(d_iz);
@@ -1040,7 +1053,7 @@ int rfk_density_estimate(rfk_state* s, float max_radius, float min_radius, float
     return RFK_OK;
 }
 
-static int tonemap_common(const void* in, bool in_float, size_t npix, void* out, int fmt, const rfk_tonemap_args* a,
+static int tonemap_common(const void* in, int in_kind, size_t npix, void* out, int fmt, const rfk_tonemap_args* a,
                           double state_quality, hipStream_t stream) {
     if (!in || !out || !a) return fail(RFK_ERR_INVALID, "null argument");
     if (fmt < 0 || fmt > 4) return fail(RFK_ERR_INVALID, "bad pixel format");
@@ -1048,7 +1061,7 @@ static int tonemap_common(const void* in, bool in_float, size_t npix, void* out,
     if (!(q > 0)) return fail(RFK_ERR_INVALID, "quality must be positive (render first)");
     // scale constant as tonemapper.cpp:52: 10^(-log10(q) - 0.5) in float
     float k = std::pow(10.0f, -std::log10(float(q)) - 0.5f);
-    RFK_HIP(rfk::sk::tonemap(in, in_float, npix, out, fmt, k, float(a->brightness), float(a->gamma), float(a->vibrancy),
+    RFK_HIP(rfk::sk::tonemap(in, in_kind, npix, out, fmt, k, float(a->brightness), float(a->gamma), float(a->vibrancy),
                              stream));
     RFK_HIP(hipStreamSynchronize(stream));
     return RFK_OK;
@@ -1056,12 +1069,13 @@ static int tonemap_common(const void* in, bool in_float, size_t npix, void* out,
 
 int rfk_tonemap(rfk_state* s, void* out, int pixel_format, const rfk_tonemap_args* a) {
     if (!s) return fail(RFK_ERR_INVALID, "null state");
-    return tonemap_common(s->hist, false, size_t(s->width) * s->height, out, pixel_format, a, s->quality, s->stream);
+    return tonemap_common(s->hist, s->hist64 ? 2 : 1, size_t(s->width) * s->height, out, pixel_format, a, s->quality,
+                          s->stream);
 }
 
 int rfk_tonemap_image(const float* in_rgba, uint32_t width, uint32_t height, void* out, int pixel_format,
                       const rfk_tonemap_args* a, void* stream) {
-    return tonemap_common(in_rgba, true, size_t(width) * height, out, pixel_format, a, 0.0,
+    return tonemap_common(in_rgba, 0, size_t(width) * height, out, pixel_format, a, 0.0,
                           static_cast<hipStream_t>(stream));
 }
 

@@ -129,6 +129,7 @@ std::string make_flame_functions(const variation_db& db, const flame& f, const f
 
     // cumulative-weight selection (flame.tpl:110-123)
     os << "RFK_INLINE u32 rfk_select_xform(const Real* __restrict__ P, Real ratio) {\n";
+    os << "    RFK_EXACT_FP\n";
     // IEEE-exact products and sums so the choice is reproducible off the GPU
     os << "    ratio = RFK_MUL_RN(ratio, " << P(L.weight_sum) << ");\n";
     os << "    Real rsum = Real(0);\n    u32 last_nonzero = 0;\n";
@@ -147,6 +148,7 @@ std::string make_flame_functions(const variation_db& db, const flame& f, const f
     if (L.chaos_base >= 0) {
         // xaos selection (flame.tpl:127-142): row `last` of the chaos table
         os << "RFK_INLINE u32 rfk_select_xform_chaos(const Real* __restrict__ P, u32 last, Real ratio) {\n";
+        os << "    RFK_EXACT_FP\n";
         os << "    if (last >= " << n << "u) return rfk_select_xform(P, ratio);\n";
         os << "    const Real* row = P + " << L.chaos_base << " + last * " << (n + 1) << "u;\n";
         os << "    ratio = RFK_MUL_RN(ratio, row[0]);\n    Real rsum = Real(0);\n    u32 last_nonzero = 0;\n";
@@ -166,6 +168,7 @@ std::string make_flame_functions(const variation_db& db, const flame& f, const f
     // per-sample precalc (flame.tpl:166-187): weight sum, xaos rows, AA
     // jitter of the screen transform, then each variation's precalc block
     os << "RFK_INLINE void rfk_precalc_sample(Real* __restrict__ P, rng_t& rs) {\n";
+    os << "    RFK_EXACT_FP\n";
     os << "    " << P(L.weight_sum) << " = ";
     for (size_t i = 0; i < n; i++) os << (i ? " + " : "") << P(L.xforms[i].weight);
     os << ";\n";
@@ -201,6 +204,7 @@ std::string make_source(const variation_db& db, const flame& f, const flame_layo
     os << "#define RFK_SIZE_REALS " << L.size_reals << "\n";
     os << "#define RFK_ATOMIC64 " << (o.atomic_width == 64 ? 1 : 0) << "\n";
     os << "#define RFK_BIN_TILE_LOG2 " << o.bin_tile_log2 << "\n";
+    os << "#define RFK_HIST64 " << (o.hist64 ? 1 : 0) << "\n";
     os << device_runtime_source() << "\n";
     os << "// ---------------------------------------------------------------- flame\n";
     os << make_flame_functions(db, f, L) << "\n";
@@ -209,7 +213,10 @@ std::string make_source(const variation_db& db, const flame& f, const flame_layo
 }
 
 std::vector<std::string> default_compile_flags() {
+    // fast math, but NaN-aware and with contraction that the RFK_EXACT_FP
+    // regions (selection, precalc sums, binning) can switch off
     return {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffast-math", "-fno-finite-math-only",
+            "-ffp-contract=fast-honor-pragmas",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
 }
 

@@ -416,10 +416,12 @@ int rfk_denoiser_get_grads(const rfk_denoiser* d, float* gw, float* gb) {
 
 // Build the network input from a fixed-point histogram and its tonemapped
 // float4 image: features (BF16, N*H*W*8) and the FP32 RGB base (N*H*W*3).
-int rfk_denoiser_features(const uint32_t* hist, const float* tonemapped_rgba, size_t npix, void* feat, float* base,
-                          void* stream) {
+int rfk_denoiser_features(const uint32_t* hist, int hist_bits, const float* tonemapped_rgba, size_t npix, void* feat,
+                          float* base, void* stream) {
     if (!hist || !tonemapped_rgba || !feat || !base) return fail(RFK_ERR_INVALID, "null argument");
-    NN_HIP(rfk::nn::features(hist, tonemapped_rgba, npix, kFeat, feat, base, static_cast<hipStream_t>(stream)));
+    if (hist_bits != 32 && hist_bits != 64) return fail(RFK_ERR_INVALID, "hist_bits must be 32 or 64");
+    NN_HIP(rfk::nn::features(hist, hist_bits == 64, tonemapped_rgba, npix, kFeat, feat, base,
+                             static_cast<hipStream_t>(stream)));
     return RFK_OK;
 }
 

@@ -27,7 +27,13 @@ typedef float Real;
 
 #define RFK_INLINE __device__ __forceinline__
 
-// correctly rounded, never contracted or reassociated (xform selection)
+// Start of a block whose arithmetic must come out as written: no FMA
+// contraction, no reassociation (needs -ffp-contract=fast-honor-pragmas
+// when built with -ffast-math). Used where results are compared bit for bit
+// with a host evaluation: xform selection, per-sample sums, binning.
+#define RFK_EXACT_FP _Pragma("clang fp contract(off) reassociate(off)")
+
+// correctly rounded products and sums (xform selection)
 #ifdef RFK_DOUBLE
 #define RFK_MUL_RN(a, b) __dmul_rn((a), (b))
 #define RFK_ADD_RN(a, b) __dadd_rn((a), (b))
@@ -259,6 +265,7 @@ struct rfk_draw_t {
 
 RFK_INLINE rfk_draw_t rfk_make_draw(const float* __restrict__ S, const u32* pal, u32 width, u32 height, float x,
                                     float y, float z, float opacity) {
+    RFK_EXACT_FP
     rfk_draw_t d;
     d.hit = false;
     float sx = truncf(fmaf(S[0], x, fmaf(S[2], y, S[4])));
@@ -281,6 +288,18 @@ RFK_INLINE rfk_draw_t rfk_make_draw(const float* __restrict__ S, const u32* pal,
     return d;
 }
 
+// 64-bit bins: four u64 atomics, no wrap at any budget
+RFK_INLINE void rfk_commit_draw(u64* __restrict__ hist, const rfk_draw_t& d) {
+    u64* h = hist + 4ull * d.pixel;
+    atomicAdd(h + 0, u64(d.r));
+    atomicAdd(h + 1, u64(d.g));
+    atomicAdd(h + 2, u64(d.b));
+    atomicAdd(h + 3, u64(d.w));
+}
+
+// 32-bit bins: four u32 atomics, or two u64 atomics on (R,G) and (B,W) pairs
+// (a channel carries into its neighbour only past 2^32, where u32 bins have
+// wrapped anyway)
 RFK_INLINE void rfk_commit_draw(u32* __restrict__ hist, const rfk_draw_t& d) {
 #if defined(RFK_ATOMIC64) && RFK_ATOMIC64
     u64* h64 = reinterpret_cast<u64*>(hist) + 2ull * d.pixel;
@@ -305,7 +324,7 @@ struct rfk_iter_args {
     float* wy;
     float* wc;
     u32* block_pass;           // [blocks] passes completed by each block
-    u32* hist;                 // [height][width][4]
+    u32* hist;                 // [height][width][4] u32, or u64 when the kernel has RFK_HIST64
     const volatile u32* stop_flag;
     u64* counters;             // [8] hits, passes, draws, bad, min start, max stop, stopped blocks, -
     float* trace;              // [npasses][walkers][4] or null

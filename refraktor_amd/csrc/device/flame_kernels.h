@@ -23,6 +23,17 @@
 #ifndef RFK_BIN_TILE_LOG2
 #define RFK_BIN_TILE_LOG2 0
 #endif
+#ifndef RFK_HIST64
+#define RFK_HIST64 0
+#endif
+
+// bin element: u32 fixed point (wraps past 2^32 / 255 full-opacity draws in
+// one pixel) or u64 for budgets that can put more than that into a pixel
+#if RFK_HIST64
+typedef u64 hist_t;
+#else
+typedef u32 hist_t;
+#endif
 
 // LDS bin cache: 2^RFK_BIN_TILE_LOG2 direct-mapped slots per block. The first
 // bin to claim a free slot owns it until the next flush; its draws accumulate
@@ -59,7 +70,7 @@ RFK_INLINE void rfk_tile_clear(rfk_shared& sm) {
 }
 
 // every thread of the block; LDS must be quiescent (callers sync before)
-RFK_INLINE void rfk_tile_flush(rfk_shared& sm, u32* __restrict__ hist) {
+RFK_INLINE void rfk_tile_flush(rfk_shared& sm, hist_t* __restrict__ hist) {
     for (u32 s = threadIdx.x; s < RFK_TILE; s += RFK_TPB) {
         const u32 key = sm.tkey[s];
         if (key != RFK_TILE_EMPTY) {
@@ -80,7 +91,7 @@ RFK_INLINE void rfk_tile_flush(rfk_shared& sm, u32* __restrict__ hist) {
     }
 }
 
-RFK_INLINE void rfk_tile_draw(rfk_shared& sm, u32* __restrict__ hist, const rfk_draw_t& d) {
+RFK_INLINE void rfk_tile_draw(rfk_shared& sm, hist_t* __restrict__ hist, const rfk_draw_t& d) {
     const u32 s = rfk_tile_slot(d.pixel);
     const u32 prev = atomicCAS(&sm.tkey[s], RFK_TILE_EMPTY, d.pixel);
     if (prev == RFK_TILE_EMPTY || prev == d.pixel) {
@@ -110,7 +121,7 @@ RFK_INLINE u32 rfk_wave_pick(u32 pass, u32 block, u32 wave, u32 k0, u32 k1, u32x
 //             for this call)
 // Returns the passes done (bailouts may stop early); counters land in sm.
 RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, const Real* __restrict__ P, const u32* pal,
-                         u32 block, u32 key1, u32* hist, bool init, u32 total, u32 pass0, u32 nfuse, u32 npasses,
+                         u32 block, u32 key1, hist_t* hist, bool init, u32 total, u32 pass0, u32 nfuse, u32 npasses,
                          u32 flags, float* wx, float* wy, float* wc, u32* wl) {
     const u32 t = threadIdx.x;
     const u32 gid = block * RFK_TPB + t;
@@ -311,7 +322,7 @@ extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate(rfk_iter_args 
     const Real* __restrict__ P = (const Real*)a.params + (size_t)block * a.param_stride;
     const bool init = (a.flags & RFK_FLAG_INIT) != 0u;
     const u32 pass0 = init ? 0u : a.block_pass[block];
-    const u32 done = rfk_block(a, sm, P, a.palettes + (size_t)block * 256u, block, a.stream, a.hist, init,
+    const u32 done = rfk_block(a, sm, P, a.palettes + (size_t)block * 256u, block, a.stream, (hist_t*)a.hist, init,
                                a.total_walkers, pass0, 0u, a.npasses, a.flags, a.wx, a.wy, a.wc, a.wlast);
     if (threadIdx.x == 0) {
         a.block_pass[block] = pass0 + done;
@@ -341,7 +352,7 @@ extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate_batch(rfk_iter
         const u32 row = b.frame_row0[f] + blk;
         const Real* __restrict__ P = (const Real*)a.params + (size_t)row * a.param_stride;
         const u32* pal = a.palettes + (size_t)row * 256u;
-        u32* hist = a.hist + (size_t)f * b.frame_pixels * 4u;
+        hist_t* hist = (hist_t*)a.hist + (size_t)f * b.frame_pixels * 4u;
         const u32 total = b.frame_blocks[f] * RFK_TPB;
         const u32 done = rfk_block(a, sm, P, pal, blk, f, hist, true, total, 0u, b.fuse, b.frame_passes[f],
                                    RFK_FLAG_DRAW, nullptr, nullptr, nullptr, nullptr) - b.fuse;

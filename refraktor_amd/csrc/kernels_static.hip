@@ -112,8 +112,9 @@ __global__ void eval_samples_kernel(const double* __restrict__ segments, uint32_
 }
 
 // ------------------------------------------------------------------ points
+template <class HistT>
 __global__ void bin_points_kernel(const float* __restrict__ screen6, const uint32_t* __restrict__ palette,
-                                  const float4* __restrict__ pts, size_t n, uint32_t* __restrict__ hist,
+                                  const float4* __restrict__ pts, size_t n, HistT* __restrict__ hist,
                                   uint32_t width, uint32_t height) {
     __shared__ uint32_t s_pal[256];
     for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) s_pal[i] = palette[i];
@@ -127,8 +128,24 @@ __global__ void bin_points_kernel(const float* __restrict__ screen6, const uint3
     if (d.hit) rfk_commit_draw(hist, d);
 }
 
+// ------------------------------------------------------------ bin readers
+// Histogram elements: u32 RGBA (uint4) or u64 RGBA (bin64). Draw units are
+// value / 255.
+struct __align__(32) bin64 {
+    unsigned long long x, y, z, w;
+};
+__device__ __forceinline__ float4 bin_units(const uint4& h) {
+    const float k = 1.0f / 255.0f;
+    return {float(h.x) * k, float(h.y) * k, float(h.z) * k, float(h.w) * k};
+}
+__device__ __forceinline__ float4 bin_units(const bin64& h) {
+    const float k = 1.0f / 255.0f;
+    return {float(h.x) * k, float(h.y) * k, float(h.z) * k, float(h.w) * k};
+}
+
 // ------------------------------------------------------- density histogram
-__global__ void density_histogram_kernel(const uint4* __restrict__ hist, size_t npix,
+template <class BinT>
+__global__ void density_histogram_kernel(const BinT* __restrict__ hist, size_t npix,
                                          unsigned long long* __restrict__ bins97,
                                          unsigned long long* __restrict__ max_density) {
     constexpr unsigned kBins = 4 * 24 + 1;
@@ -139,7 +156,7 @@ __global__ void density_histogram_kernel(const uint4* __restrict__ hist, size_t 
     __syncthreads();
     const float log_div = 4.0f / logf(2.0f);
     for (size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x; g < npix; g += size_t(gridDim.x) * blockDim.x) {
-        const float value = float(hist[g].w) * (1.0f / 255.0f);
+        const float value = bin_units(hist[g]).w;
         if (value >= 1.0f) {
             unsigned long long b = (unsigned long long)(logf(value) * log_div) + 1ull;
             if (b >= kBins) b = kBins - 1;
@@ -158,24 +175,24 @@ __global__ void density_histogram_kernel(const uint4* __restrict__ hist, size_t 
 // ------------------------------------------------------ density estimation
 // Level of a source bin: h = max(min_r, max_r * max(n,1)^-curve), quantised
 // to 1/16 pixel; -1 for an empty bin.
-__global__ void de_levels_kernel(const uint4* __restrict__ hist, size_t npix, float max_radius, float min_radius,
+template <class BinT>
+__global__ void de_levels_kernel(const BinT* __restrict__ hist, size_t npix, float max_radius, float min_radius,
                                  float curve, int max_level, int* __restrict__ levels) {
     const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= npix) return;
-    const uint32_t w = hist[g].w;
-    if (w == 0u) {
+    if (hist[g].w == 0u) {
         levels[g] = -1;
         return;
     }
-    const float n = float(w) * (1.0f / 255.0f);
+    const float n = bin_units(hist[g]).w;
     float h = max_radius * powf(fmaxf(n, 1.0f), -curve);
     h = fmaxf(h, min_radius);
     int lvl = int(h * 16.0f + 0.5f);
     levels[g] = min(lvl, max_level);
 }
 
-template <int TILE>
-__global__ void de_gather_kernel(const uint4* __restrict__ hist, const int* __restrict__ levels, uint32_t width,
+template <int TILE, class BinT>
+__global__ void de_gather_kernel(const BinT* __restrict__ hist, const int* __restrict__ levels, uint32_t width,
                                  uint32_t height, int radius, const float* __restrict__ inv_z, int nlevels,
                                  float4* __restrict__ out) {
     extern __shared__ unsigned char smem[];
@@ -193,11 +210,7 @@ __global__ void de_gather_kernel(const uint4* __restrict__ hist, const int* __re
         if (xx >= 0 && yy >= 0 && xx < int(width) && yy < int(height)) {
             const size_t p = size_t(yy) * width + size_t(xx);
             lv = levels[p];
-            if (lv >= 0) {
-                const uint4 h = hist[p];
-                const float k = 1.0f / 255.0f;
-                c = {float(h.x) * k, float(h.y) * k, float(h.z) * k, float(h.w) * k};
-            }
+            if (lv >= 0) c = bin_units(hist[p]);
         }
         s_lvl[i] = lv;
         s_col[i] = c;
@@ -236,19 +249,16 @@ struct half4_t {
 __device__ __forceinline__ float sat(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return a + t * (b - a); }
 
-template <bool InFloat, int Fmt>
+// In: 0 float4 draw units, 1 u32 bins, 2 u64 bins
+template <int In, int Fmt>
 __global__ void tonemap_kernel(const void* __restrict__ in, size_t npix, void* __restrict__ out, float scale_constant,
                                float brightness, float gamma, float vibrancy) {
     const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= npix) return;
     float4 col;
-    if (InFloat) {
-        col = reinterpret_cast<const float4*>(in)[g];
-    } else {
-        const uint4 h = reinterpret_cast<const uint4*>(in)[g];
-        const float k = 1.0f / 255.0f;
-        col = {float(h.x) * k, float(h.y) * k, float(h.z) * k, float(h.w) * k};
-    }
+    if (In == 0) col = reinterpret_cast<const float4*>(in)[g];
+    else if (In == 1) col = bin_units(reinterpret_cast<const uint4*>(in)[g]);
+    else col = bin_units(reinterpret_cast<const bin64*>(in)[g]);
     float4 o = {0.f, 0.f, 0.f, 0.f};
     if (col.w != 0.0f) {
         const float input_density = col.w;
@@ -338,44 +348,65 @@ hipError_t fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
 }
 
 hipError_t bin_points(const float* screen6, const uint32_t* palette, const float* pts, size_t n, uint32_t* hist,
-                      uint32_t width, uint32_t height, hipStream_t s) {
+                      bool hist64, uint32_t width, uint32_t height, hipStream_t s) {
     if (!n) return hipSuccess;
-    bin_points_kernel<<<nblocks(n), kBlock, 0, s>>>(screen6, palette, reinterpret_cast<const float4*>(pts), n, hist,
-                                                     width, height);
+    const float4* p4 = reinterpret_cast<const float4*>(pts);
+    if (hist64)
+        bin_points_kernel<<<nblocks(n), kBlock, 0, s>>>(screen6, palette, p4, n,
+                                                         reinterpret_cast<unsigned long long*>(hist), width, height);
+    else
+        bin_points_kernel<<<nblocks(n), kBlock, 0, s>>>(screen6, palette, p4, n, hist, width, height);
     return hipGetLastError();
 }
 
-hipError_t density_histogram(const uint32_t* hist, size_t npix, unsigned long long* bins97,
+hipError_t density_histogram(const uint32_t* hist, bool hist64, size_t npix, unsigned long long* bins97,
                              unsigned long long* max_density, hipStream_t s) {
     unsigned grid = std::min(nblocks(npix), 2048u);
     if (grid == 0) grid = 1;
-    density_histogram_kernel<<<grid, kBlock, 0, s>>>(reinterpret_cast<const uint4*>(hist), npix, bins97, max_density);
+    if (hist64)
+        density_histogram_kernel<<<grid, kBlock, 0, s>>>(reinterpret_cast<const bin64*>(hist), npix, bins97,
+                                                         max_density);
+    else
+        density_histogram_kernel<<<grid, kBlock, 0, s>>>(reinterpret_cast<const uint4*>(hist), npix, bins97,
+                                                         max_density);
     return hipGetLastError();
 }
 
-hipError_t density_estimate(const uint32_t* hist, uint32_t width, uint32_t height, float max_radius, float min_radius,
-                            float curve, const float* inv_z, int nlevels, int radius, int* level_scratch,
-                            float* out_rgba, hipStream_t s) {
+template <class BinT>
+static hipError_t density_estimate_t(const BinT* hist, uint32_t width, uint32_t height, float max_radius,
+                                     float min_radius, float curve, const float* inv_z, int nlevels, int radius,
+                                     int* level_scratch, float* out_rgba, hipStream_t s) {
     size_t npix = size_t(width) * height;
-    de_levels_kernel<<<nblocks(npix), kBlock, 0, s>>>(reinterpret_cast<const uint4*>(hist), npix, max_radius,
-                                                      min_radius, curve, nlevels - 1, level_scratch);
+    de_levels_kernel<<<nblocks(npix), kBlock, 0, s>>>(hist, npix, max_radius, min_radius, curve, nlevels - 1,
+                                                      level_scratch);
     constexpr int T = 16;
     const int span = T + 2 * radius;
     size_t shmem = size_t(span) * span * (sizeof(float4) + sizeof(int)) + size_t(nlevels) * sizeof(float);
     dim3 grid((width + T - 1) / T, (height + T - 1) / T);
-    de_gather_kernel<T><<<grid, dim3(T, T), shmem, s>>>(reinterpret_cast<const uint4*>(hist), level_scratch, width,
-                                                        height, radius, inv_z, nlevels,
+    de_gather_kernel<T><<<grid, dim3(T, T), shmem, s>>>(hist, level_scratch, width, height, radius, inv_z, nlevels,
                                                         reinterpret_cast<float4*>(out_rgba));
     return hipGetLastError();
 }
 
-hipError_t tonemap(const void* in, bool in_float, size_t npix, void* out, int format, float scale_constant,
+hipError_t density_estimate(const uint32_t* hist, bool hist64, uint32_t width, uint32_t height, float max_radius,
+                            float min_radius, float curve, const float* inv_z, int nlevels, int radius,
+                            int* level_scratch, float* out_rgba, hipStream_t s) {
+    if (hist64)
+        return density_estimate_t(reinterpret_cast<const bin64*>(hist), width, height, max_radius, min_radius, curve,
+                                  inv_z, nlevels, radius, level_scratch, out_rgba, s);
+    return density_estimate_t(reinterpret_cast<const uint4*>(hist), width, height, max_radius, min_radius, curve,
+                              inv_z, nlevels, radius, level_scratch, out_rgba, s);
+}
+
+hipError_t tonemap(const void* in, int in_kind, size_t npix, void* out, int format, float scale_constant,
                    float brightness, float gamma, float vibrancy, hipStream_t s) {
     if (!npix) return hipSuccess;
+    if (in_kind < 0 || in_kind > 2) return hipErrorInvalidValue;
     unsigned g = nblocks(npix);
 #define RFK_TM(F)                                                                                              \
-    if (in_float) tonemap_kernel<true, F><<<g, kBlock, 0, s>>>(in, npix, out, scale_constant, brightness, gamma, vibrancy); \
-    else tonemap_kernel<false, F><<<g, kBlock, 0, s>>>(in, npix, out, scale_constant, brightness, gamma, vibrancy);
+    if (in_kind == 0) tonemap_kernel<0, F><<<g, kBlock, 0, s>>>(in, npix, out, scale_constant, brightness, gamma, vibrancy); \
+    else if (in_kind == 1) tonemap_kernel<1, F><<<g, kBlock, 0, s>>>(in, npix, out, scale_constant, brightness, gamma, vibrancy); \
+    else tonemap_kernel<2, F><<<g, kBlock, 0, s>>>(in, npix, out, scale_constant, brightness, gamma, vibrancy);
     switch (format) {
         case 0: RFK_TM(0) break;
         case 1: RFK_TM(1) break;

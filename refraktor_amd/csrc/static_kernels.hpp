@@ -23,21 +23,22 @@ hipError_t eval_samples(const double* segments, uint32_t sample_size, uint32_t s
 hipError_t fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 
 // Bin n plane-space points through a screen transform / palette.
+// (hist64: the histogram holds u64 RGBA bins, else u32 RGBA)
 hipError_t bin_points(const float* screen6, const uint32_t* palette, const float* pts, size_t n, uint32_t* hist,
-                      uint32_t width, uint32_t height, hipStream_t s);
+                      bool hist64, uint32_t width, uint32_t height, hipStream_t s);
 
-// calculate_histogram<4> (density_histo.cu:1-41) over the u32 histogram.
-hipError_t density_histogram(const uint32_t* hist, size_t npix, unsigned long long* bins97,
+// calculate_histogram<4> (density_histo.cu:1-41) over the fixed-point histogram.
+hipError_t density_histogram(const uint32_t* hist, bool hist64, size_t npix, unsigned long long* bins97,
                              unsigned long long* max_density, hipStream_t s);
 
 // Density estimation: levels + gather. inv_z has `nlevels` entries.
-hipError_t density_estimate(const uint32_t* hist, uint32_t width, uint32_t height, float max_radius, float min_radius,
-                            float curve, const float* inv_z, int nlevels, int radius, int* level_scratch,
-                            float* out_rgba, hipStream_t s);
+hipError_t density_estimate(const uint32_t* hist, bool hist64, uint32_t width, uint32_t height, float max_radius,
+                            float min_radius, float curve, const float* inv_z, int nlevels, int radius,
+                            int* level_scratch, float* out_rgba, hipStream_t s);
 
-// Tonemap (tonemap.cu:50-105). Input: u32 RGBA fixed point (in_float=false)
-// or float4 draw units (in_float=true).
-hipError_t tonemap(const void* in, bool in_float, size_t npix, void* out, int format, float scale_constant,
+// Tonemap (tonemap.cu:50-105). in_kind: 0 float4 draw units, 1 u32 RGBA
+// fixed point, 2 u64 RGBA fixed point.
+hipError_t tonemap(const void* in, int in_kind, size_t npix, void* out, int format, float scale_constant,
                    float brightness, float gamma, float vibrancy, hipStream_t s);
 
 // Random RGBA atomic increments for the binning roofline.

@@ -376,14 +376,16 @@ __global__ void bf16_to_f32_kernel(const bf16* __restrict__ a, float* __restrict
 // image: [R, G, B, alpha] of the tonemapped frame, log2(1 + density) / 16,
 // log2(1 + hits) / 16 and zero padding up to Cpad channels; also the FP32
 // base image the network adds its residual to.
-__global__ void features_kernel(const uint4* __restrict__ hist, const float4* __restrict__ tm, size_t npix, int Cpad,
-                                bf16* __restrict__ feat, float* __restrict__ base) {
+// `hist` holds u32 RGBA bins, or u64 RGBA bins when hist64 is set
+__global__ void features_kernel(const uint32_t* __restrict__ hist, bool hist64, const float4* __restrict__ tm,
+                                size_t npix, int Cpad, bf16* __restrict__ feat, float* __restrict__ base) {
     const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= npix) return;
-    const uint4 h = hist[p];
+    const float w = hist64 ? float(reinterpret_cast<const unsigned long long*>(hist)[4 * p + 3])
+                           : float(hist[4 * p + 3]);
     const float4 c = tm[p];
-    const float dens = float(h.w) * (1.0f / 255.0f);
-    float f[8] = {c.x, c.y, c.z, c.w, log2f(1.0f + dens) * (1.0f / 16.0f), log2f(1.0f + float(h.w > 0)) , 0.f, 0.f};
+    const float dens = w * (1.0f / 255.0f);
+    float f[8] = {c.x, c.y, c.z, c.w, log2f(1.0f + dens) * (1.0f / 16.0f), w > 0.0f ? 1.0f : 0.0f, 0.f, 0.f};
     for (int k = 0; k < Cpad; k++) feat[p * Cpad + k] = bf16(k < 6 ? f[k] : 0.f);
     base[p * 3 + 0] = c.x;
     base[p * 3 + 1] = c.y;
@@ -523,9 +525,10 @@ hipError_t bf16_to_f32(const void* a, float* b, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t features(const uint32_t* hist, const float* tm_rgba, size_t npix, int Cpad, void* feat, float* base,
+hipError_t features(const uint32_t* hist, bool hist64, const float* tm_rgba, size_t npix, int Cpad, void* feat,
+                    float* base,
                     hipStream_t s) {
-    features_kernel<<<nb(npix), 256, 0, s>>>((const uint4*)hist, (const float4*)tm_rgba, npix, Cpad, (bf16*)feat, base);
+    features_kernel<<<nb(npix), 256, 0, s>>>(hist, hist64, (const float4*)tm_rgba, npix, Cpad, (bf16*)feat, base);
     return hipGetLastError();
 }
 

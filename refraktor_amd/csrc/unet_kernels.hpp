@@ -35,7 +35,8 @@ hipError_t adam(float* w, float* m, float* v, const float* g, size_t n, float lr
                 int step, float wd, void* wb, hipStream_t s);
 hipError_t f32_to_bf16(const float* a, void* b, size_t n, hipStream_t s);
 hipError_t bf16_to_f32(const void* a, float* b, size_t n, hipStream_t s);
-hipError_t features(const uint32_t* hist, const float* tm_rgba, size_t npix, int Cpad, void* feat, float* base,
+hipError_t features(const uint32_t* hist, bool hist64, const float* tm_rgba, size_t npix, int Cpad, void* feat,
+                    float* base,
                     hipStream_t s);
 hipError_t clamp_rgb(const float* in, size_t n, float* out, hipStream_t s);
 // crops (img, y, x) from a stack of H x W images with pixel_bytes per pixel

@@ -86,7 +86,8 @@ class Denoiser:
 @dataclass
 class Frame:
     """one rendered frame kept on the device"""
-    hist: render.DeviceBuffer     # u32 RGBA
+    hist: render.DeviceBuffer     # RGBA fixed-point bins of hist_bits
+    hist_bits: int
     tm: render.DeviceBuffer       # float4 tonemapped (after DE)
     width: int
     height: int
@@ -103,21 +104,23 @@ def render_frame(db: render.VariationDB, f: dict, width: int, height: int, iters
     st.bin(iters=iters)
     inf = st.info()
     npx = width * height
-    hist = render.DeviceBuffer(npx * 16)
-    check(lib.rfk_memcpy_dtod(hist.ptr, inf.histogram, npx * 16))
+    hb = npx * 4 * inf.hist_bits // 8
+    hist = render.DeviceBuffer(hb)
+    check(lib.rfk_memcpy_dtod(hist.ptr, inf.histogram, hb))
     de = render.DeviceBuffer(npx * 16)
     check(lib.rfk_density_estimate(st.handle, de_radius, 0.0, 0.4, de.ptr))
     tm = render.DeviceBuffer(npx * 16)
     args = _lib.TonemapArgs(inf.quality, f["gamma"], f["brightness"], f["vibrancy"], 0.0)
     check(lib.rfk_tonemap_image(de.ptr, width, height, tm.ptr, _lib.PIXEL_FLOAT4, C.byref(args), None))
-    return Frame(hist, tm, width, height, inf.quality)
+    return Frame(hist, inf.hist_bits, tm, width, height, inf.quality)
 
 
 def features(frame: Frame) -> tuple[render.DeviceBuffer, render.DeviceBuffer]:
     npx = frame.width * frame.height
     feat = render.DeviceBuffer(npx * FEAT_C * 2)
     base = render.DeviceBuffer(npx * 12)
-    check(_lib.lib().rfk_denoiser_features(frame.hist.ptr, frame.tm.ptr, npx, feat.ptr, base.ptr, None))
+    check(_lib.lib().rfk_denoiser_features(frame.hist.ptr, frame.hist_bits, frame.tm.ptr, npx, feat.ptr, base.ptr,
+                                           None))
     return feat, base
 
 

@@ -106,9 +106,9 @@ class VariationDB:
         return Flame(self, None, handle=h.value)
 
     def compile(self, flame, precision: int = 0, threads_per_block: int = 256, atomic_width: int = 32,
-                bin_tile_log2: int = 0) -> "Kernel":
+                bin_tile_log2: int = 0, hist_bits: int = 32) -> "Kernel":
         f = flame if isinstance(flame, Flame) else self.flame(flame)
-        return Kernel(self, f, precision, threads_per_block, atomic_width, bin_tile_log2)
+        return Kernel(self, f, precision, threads_per_block, atomic_width, bin_tile_log2, hist_bits)
 
     def __del__(self):
         try:
@@ -187,10 +187,11 @@ class Flame:
 
 class Kernel:
     def __init__(self, db: VariationDB, flame: Flame, precision: int, tpb: int, atomic_width: int,
-                 bin_tile_log2: int = 0):
+                 bin_tile_log2: int = 0, hist_bits: int = 32):
         self.db, self.flame, self._lib = db, flame, db._lib
         self.precision = precision
-        cfg = _lib.ExecConfig(tpb, atomic_width, bin_tile_log2, 0)
+        self.hist_bits = hist_bits
+        cfg = _lib.ExecConfig(tpb, atomic_width, bin_tile_log2, hist_bits)
         h = C.c_void_p()
         log = C.create_string_buffer(1 << 16)
         rc = self._lib.rfk_compile(db.handle, flame.handle, precision, C.byref(cfg), C.byref(h), log, len(log))
@@ -261,8 +262,10 @@ class State:
         check(self._lib.rfk_abort(self.handle))
 
     def histogram(self) -> np.ndarray:
+        """(H, W, 4) R,G,B,W fixed-point bins: u32, or u64 for hist_bits=64 kernels"""
         inf = self.info()
-        return download(inf.histogram, np.uint32, (inf.height, inf.width, 4))
+        dt = np.uint64 if inf.hist_bits == 64 else np.uint32
+        return download(inf.histogram, dt, (inf.height, inf.width, 4))
 
     def params(self) -> np.ndarray:
         inf = self.info()
@@ -346,7 +349,9 @@ class Batch:
         return {name: getattr(inf, name) for name, _ in inf._fields_}
 
     def histogram(self, f: int) -> np.ndarray:
-        return download(self.frame(f)["histogram"], np.uint32, (self.height, self.width, 4))
+        inf = self.frame(f)
+        dt = np.uint64 if inf["hist_bits"] == 64 else np.uint32
+        return download(inf["histogram"], dt, (self.height, self.width, 4))
 
     def __del__(self):
         try:

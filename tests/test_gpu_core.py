@@ -80,25 +80,44 @@ def test_trajectory_free_running(small):
     assert np.mean(err[:4] < 1e-3) > 0.99
 
 
+def _bin_compare(st, pts, label):
+    st.clear_histogram()
+    st.bin_points(pts)
+    g = st.histogram()
+    o = oracle.bin_points(st.params()[0, :6], st.palettes()[0], pts, W, H)
+    if not np.array_equal(g, o):
+        bad = np.argwhere((g != o).any(axis=-1))
+        S = st.params()[0, :6].astype(np.float32)
+        sx = np.trunc(S[0] * pts[:, 0] + S[2] * pts[:, 1] + S[4])
+        sy = np.trunc(S[1] * pts[:, 0] + S[3] * pts[:, 1] + S[5])
+        lines = []
+        for yy, xx in bad[:5]:
+            near = np.nonzero((np.abs(sx - xx) <= 1) & (np.abs(sy - yy) <= 1))[0][:4]
+            lines.append(f"pixel ({yy},{xx}) gpu {g[yy, xx]} oracle {o[yy, xx]} points {pts[near].tolist()}")
+        pytest.fail(f"{label}: {len(bad)} pixels differ, gpu sum {g.sum(axis=(0, 1))} oracle sum "
+                    f"{o.sum(axis=(0, 1))}\n" + "\n".join(lines))
+
+
 def test_bin_points_bit_exact(small):
     f, k = small
     st = k.warmup(k.pack_samples(W, H), W, H, seed=2, fuse=0, walkers=1024)
     rng = np.random.default_rng(0)
     n = 200_000
-    pts = np.stack([rng.uniform(-2.5, 2.5, n), rng.uniform(-1.5, 1.5, n), rng.uniform(-0.2, 1.2, n),
-                    rng.uniform(-0.1, 1.0, n)], axis=1).astype(np.float32)
+    inside = np.stack([rng.uniform(-2.0, 2.0, n), rng.uniform(-1.1, 1.1, n), rng.uniform(0.0, 1.0, n),
+                       rng.uniform(0.0, 1.0, n)], axis=1).astype(np.float32)
+    _bin_compare(st, inside, "in-frame points")
+    wide = np.stack([rng.uniform(-2.5, 2.5, n), rng.uniform(-1.5, 1.5, n), rng.uniform(-0.2, 1.2, n),
+                     rng.uniform(-0.1, 1.0, n)], axis=1).astype(np.float32)
+    _bin_compare(st, wide, "points around the frame, colours and opacities out of range")
     # edge cases: non-finite, zero opacity, exact colour ends
-    pts[:10, 0] = np.nan
-    pts[10:20, 1] = np.inf
-    pts[20:30, 3] = 0.0
-    pts[30:40, 2] = 1.0
-    pts[40:50, 2] = 0.0
-    st.clear_histogram()
-    st.bin_points(pts)
-    g = st.histogram()
-    o = oracle.bin_points(st.params()[0, :6], st.palettes()[0], pts, W, H)
-    assert np.array_equal(g, o)
-    assert g[..., 3].sum() > 0
+    edge = inside[:60].copy()
+    edge[:10, 0] = np.nan
+    edge[10:20, 1] = np.inf
+    edge[20:30, 3] = 0.0
+    edge[30:40, 2] = 1.0
+    edge[40:50, 2] = 0.0
+    edge[50:60, 2] = np.nan
+    _bin_compare(st, edge, "edge cases")
 
 
 def test_render_counters_and_determinism(small):

@@ -11,22 +11,56 @@ pytestmark = pytest.mark.gpu
 W, H = 320, 180
 
 
-def _render(db, flm, tpb, aw, tile, seed=4):
-    k = db.compile(flm, threads_per_block=tpb, atomic_width=aw, bin_tile_log2=tile)
+def _render(db, flm, tpb, aw, tile, seed=4, hist_bits=32):
+    k = db.compile(flm, threads_per_block=tpb, atomic_width=aw, bin_tile_log2=tile, hist_bits=hist_bits)
     st = k.warmup(k.pack_samples(W, H), W, H, seed=seed, walkers=1 << 14)
     res = st.bin(iters=1 << 21)
-    return st.histogram(), res
+    return st, res
 
 
 @pytest.mark.parametrize("name", ["small", "1080"])
 def test_atomic_width_and_bin_tiles_bit_exact(gpu, db, name):
     flm = fl.flame_small(2) if name == "small" else fl.flame_1080(2)
-    ref, rres = _render(db, flm, 256, 32, 0)
+    rst, rres = _render(db, flm, 256, 32, 0)
+    ref = rst.histogram().astype(np.uint64)
     assert rres["total_draws"] > 0
-    for aw, tile in ((64, 0), (32, 10), (64, 10), (32, 6), (32, 12), (64, 12)):
-        got, res = _render(db, flm, 256, aw, tile)
+    for aw, tile, hb in ((64, 0, 32), (32, 10, 32), (64, 10, 32), (32, 6, 32), (32, 12, 32), (64, 12, 32),
+                         (32, 0, 64), (32, 11, 64)):
+        st, res = _render(db, flm, 256, aw, tile, hist_bits=hb)
         assert res["total_draws"] == rres["total_draws"]
-        np.testing.assert_array_equal(got, ref, err_msg=f"atomic {aw} tile {tile}")
+        got = st.histogram()
+        assert got.dtype == (np.uint64 if hb == 64 else np.uint32)
+        np.testing.assert_array_equal(got.astype(np.uint64), ref, err_msg=f"atomic {aw} tile {tile} bins {hb}")
+
+
+def test_64bit_bins_feed_the_same_image(gpu, db):
+    """density histogram, density estimation and tonemap read u64 bins as they read u32 bins"""
+    flm = fl.flame_1080(5)
+    a, _ = _render(db, flm, 256, 32, 0)
+    b, _ = _render(db, flm, 256, 32, 0, hist_bits=64)
+    ba, ma = a.density_histogram()
+    bb, mb = b.density_histogram()
+    np.testing.assert_array_equal(ba, bb)
+    assert ma == mb
+    np.testing.assert_array_equal(a.density_estimate(6.0), b.density_estimate(6.0))
+    np.testing.assert_array_equal(a.tonemap(2.2, 4.0, 1.0), b.tonemap(2.2, 4.0, 1.0))
+
+
+def test_64bit_bins_do_not_wrap(gpu, db):
+    """one pixel taking more than 2^32 / 255 full-opacity draws: u64 bins hold the count"""
+    k = db.compile(fl.flame_small(1), hist_bits=64)
+    st = k.warmup(k.pack_samples(64, 64), 64, 64, seed=1, walkers=1 << 12)
+    S = st.params()[0, :6]
+    # plane point that lands in pixel (10, 20): solve the screen affine
+    A = np.array([[S[0], S[2]], [S[1], S[3]]], dtype=np.float64)
+    x, y = np.linalg.solve(A, np.array([20.5 - S[4], 10.5 - S[5]]))
+    n = 17_000_000  # 255 * n > 2^32
+    pts = np.tile(np.array([[x, y, 0.5, 1.0]], dtype=np.float32), (n, 1))
+    st.clear_histogram()
+    st.bin_points(pts)
+    h = st.histogram()
+    assert int(h[10, 20, 3]) == 255 * n > (1 << 32)
+    assert int(h.sum(dtype=np.uint64)) == int(h[10, 20].sum(dtype=np.uint64))
 
 
 def test_bad_tile_sizes_rejected(gpu, db):
