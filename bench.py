@@ -25,6 +25,11 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 
+def check(rc: int) -> None:
+    from refraktor_amd import _lib
+    _lib.check(rc)
+
+
 def psnr(a, b) -> float:
     import numpy as np
     mse = float(np.mean((a[..., :3].astype(np.float64) - b[..., :3].astype(np.float64)) ** 2))
@@ -37,19 +42,28 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--tpb", type=int, default=256)
+    ap.add_argument("--tpb", type=int, default=1024)
+    ap.add_argument("--tile", type=int, default=12, help="LDS bin cache slots (log2), 0 = off")
+    ap.add_argument("--atomic-width", type=int, default=64)
+    ap.add_argument("--hist-bits", type=int, default=32)
     ap.add_argument("--walkers", type=int, default=1 << 20)
+    ap.add_argument("--low-iters", type=int, default=1 << 26, help="budget of the denoised path")
+    ap.add_argument("--weights", type=str, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
     ap.add_argument("--no-psnr", action="store_true", help="skip the converged reference render")
+    ap.add_argument("--no-denoise", action="store_true", help="plain path only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
                           "n_gpus": args.gpus}))
         return 2
 
+    import ctypes as C
+
     import numpy as np
 
     from refraktor_amd import _lib
+    from refraktor_amd import denoise
     from refraktor_amd import flame as fl
     from refraktor_amd import render
 
@@ -58,69 +72,94 @@ def main() -> int:
     DE_R = 9.0
     f = fl.flame_1080(args.seed)
     db = render.VariationDB()
-    k = db.compile(f, threads_per_block=args.tpb)
+    k = db.compile(f, threads_per_block=args.tpb, atomic_width=args.atomic_width, bin_tile_log2=args.tile,
+                   hist_bits=args.hist_bits)
+    k64 = db.compile(f, threads_per_block=args.tpb, bin_tile_log2=args.tile, hist_bits=64)
     samples = k.pack_samples(W, H)
     gamma, bright, vib = f["gamma"], f["brightness"], f["vibrancy"]
+    lib = _lib.lib()
+    npx = W * H
 
-    def frame(seed: int):
-        st = k.warmup(samples, W, H, seed=seed, fuse=20, walkers=args.walkers)
-        res = st.bin(iters=ITERS)
-        de = st.density_estimate(DE_R, 0.0, 0.4)
-        img = render.tonemap_image(de, gamma, bright, vib, st.info().quality, fmt=_lib.PIXEL_HALF4)
-        return st, res, img
+    def plain(seed: int, kern=k, iters=ITERS, fmt=_lib.PIXEL_HALF4):
+        st = kern.warmup(samples, W, H, seed=seed, fuse=20, walkers=args.walkers)
+        res = st.bin(iters=iters)
+        de = render.DeviceBuffer(npx * 16)
+        check(lib.rfk_density_estimate(st.handle, DE_R, 0.0, 0.4, de.ptr))
+        out = render.DeviceBuffer(npx * _lib.PIXEL_BYTES[fmt])
+        ta = _lib.TonemapArgs(st.info().quality, gamma, bright, vib, 0.0)
+        check(lib.rfk_tonemap_image(de.ptr, W, H, out.ptr, fmt, C.byref(ta), None))
+        return st, res, out
 
-    for i in range(args.warmup):
-        frame(args.seed + 1000 + i)
-    _lib.lib().rfk_synchronize()
-    t0 = time.perf_counter()
-    kernel_ms, draws, passes = [], [], []
-    last = None
-    for i in range(args.steps):
-        st, res, img = frame(args.seed + i)
-        kernel_ms.append(res["kernel_ms"])
-        draws.append(res["total_draws"])
-        passes.append(res["total_passes"])
-        last = (st, res, img)
-    _lib.lib().rfk_synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    fps = 1e3 / ms_per_step
+    weights = Path(args.weights) if args.weights else denoise.DEFAULT_WEIGHTS
+    den = denoise.Denoiser(weights=weights) if (not args.no_denoise and weights.exists()) else None
+    feat = render.DeviceBuffer(npx * denoise.FEAT_C * 2)
+    base = render.DeviceBuffer(npx * 12)
+    rgb = render.DeviceBuffer(npx * 12)
+
+    def denoised(seed: int):
+        st, res, tm = plain(seed, k, args.low_iters, _lib.PIXEL_FLOAT4)
+        inf = st.info()
+        check(lib.rfk_denoiser_features(inf.histogram, inf.hist_bits, tm.ptr, npx, feat.ptr, base.ptr, None))
+        den.forward(feat, base, 1, H, W, rgb)
+        return st, res, rgb
+
+    def timed(fn, steps: int, warm: int):
+        for i in range(warm):
+            fn(args.seed + 1000 + i)
+        lib.rfk_synchronize()
+        t0 = time.perf_counter()
+        kms, draws, passes, last = [], [], [], None
+        for i in range(steps):
+            st, res, img = fn(args.seed + i)
+            kms.append(res["kernel_ms"])
+            draws.append(res["total_draws"])
+            passes.append(res["total_passes"])
+            last = img
+        lib.rfk_synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        return ms, float(np.mean(kms)), float(np.mean(draws)), float(np.mean(passes)), last
+
+    ms_plain, kms, dr, ps, img_plain = timed(plain, args.steps, args.warmup)
+    ms_den = img_den = None
+    if den is not None:
+        ms_den, _, _, _, img_den = timed(denoised, args.steps, args.warmup)
+
+    psnr_plain = psnr_den = None
+    if not args.no_psnr:
+        # converged reference: 2^32 iterations into u64 bins (no wrap in hot pixels)
+        _, _, ref_buf = plain(args.seed + 5000, k64, 1 << 32, _lib.PIXEL_FLOAT4)
+        ref = ref_buf.download(np.float32, (H, W, 4))
+        psnr_plain = psnr(img_plain.download(np.float16, (H, W, 4)).astype(np.float32), ref)
+        if img_den is not None:
+            psnr_den = psnr(img_den.download(np.float32, (H, W, 3)), ref)
 
     # iterate kernel: algorithmic bytes = 16 B per draw (one RGBA u32 increment)
-    kms = float(np.mean(kernel_ms))
-    dr = float(np.mean(draws))
     achieved_gbs = dr * 16.0 / (kms * 1e-3) / 1e9
-    it_per_s = float(np.mean(passes)) / (kms * 1e-3)
-    # measured atomic peak at the same footprint (RGBA u32, 1080p)
+    it_per_s = ps / (kms * 1e-3)
     peak_inc = 0.0
     try:
-        import ctypes as C
         v = C.c_double()
-        _lib.check(_lib.lib().rfk_atomic_peak(W * H * 16, 1 << 28, C.byref(v)))
+        check(lib.rfk_atomic_peak(npx * 16, 1 << 28, C.byref(v)))
         peak_inc = v.value
     except Exception:
         pass
-
-    psnr_db = None
-    if not args.no_psnr:
-        st_ref = k.warmup(samples, W, H, seed=args.seed + 5000, fuse=20, walkers=args.walkers)
-        st_ref.bin(iters=1 << 32)
-        de = st_ref.density_estimate(DE_R, 0.0, 0.4)
-        ref = render.tonemap_image(de, gamma, bright, vib, st_ref.info().quality, fmt=_lib.PIXEL_FLOAT4)
-        psnr_db = psnr(last[2].astype(np.float32), ref)
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(f, W, H, args.seed, it_per_frame=ITERS)
 
+    fps_plain = 1e3 / ms_plain
+    fps_den = 1e3 / ms_den if ms_den else None
+    use_den = fps_den is not None and psnr_den is not None and psnr_den >= 40.0
+    value = fps_den if use_den else fps_plain
     line = {
         "metric": "1080p frames/sec at PSNR>=40dB vs converged plain render; iterations/sec",
-        "value": fps,
+        "value": value,
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": ms_den if use_den else ms_plain,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -128,11 +167,17 @@ def main() -> int:
         "data": "synthetic seeded flame (SplitMix64 seed %d)" % args.seed,
         "config": {"workload": "plain render 1920x1080, 8 xforms x 2-4 variations + final xform, 2^30 iterations, "
                                "DE radius 9, log-density tonemap, vibrancy 1",
-                   "width": W, "height": H, "iterations": ITERS, "walkers": last[0].info().walkers,
-                   "threads_per_block": args.tpb, "path": "plain"},
+                   "width": W, "height": H, "iterations": ITERS, "walkers": args.walkers,
+                   "threads_per_block": args.tpb, "atomic_width": args.atomic_width, "bin_tile_log2": args.tile,
+                   "hist_bits": args.hist_bits,
+                   "path": "denoised" if use_den else "plain"},
+        "paths": {"plain": {"fps": fps_plain, "ms": ms_plain, "iterations": ITERS, "psnr_db": psnr_plain},
+                  "denoised": None if fps_den is None else
+                  {"fps": fps_den, "ms": ms_den, "iterations": args.low_iters, "psnr_db": psnr_den,
+                   "speedup_vs_plain": fps_den / fps_plain, "weights": str(weights.name)}},
         "iterations_per_s": it_per_s,
         "iterate_kernel_ms": kms,
-        "psnr_db_vs_converged": psnr_db,
+        "psnr_db_vs_converged": psnr_den if use_den else psnr_plain,
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "rfk_iterate", "bytes_per_draw": 16, "draws_per_launch": dr,

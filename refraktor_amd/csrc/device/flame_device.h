@@ -258,7 +258,7 @@ RFK_INLINE v2 rfk_hammersley(u32 idx, u32 total) {
 // so bin / 255 is the reference's float4 bin in draw units. All roundings are
 // spelled out (explicit fmaf, __fmul_rn) so any compiler produces the same bits.
 struct rfk_draw_t {
-    u32 pixel;
+    u32 pixel, px, py;
     u32 r, g, b, w;
     bool hit;
 };
@@ -272,7 +272,9 @@ RFK_INLINE rfk_draw_t rfk_make_draw(const float* __restrict__ S, const u32* pal,
     float sy = truncf(fmaf(S[1], x, fmaf(S[3], y, S[5])));
     if (sx >= 0.0f && sy >= 0.0f && sx < float(width) && sy < float(height) && opacity > 0.0f) {
         d.hit = true;
-        d.pixel = u32(sy) * width + u32(sx);
+        d.px = u32(sx);
+        d.py = u32(sy);
+        d.pixel = d.py * width + d.px;
         float zc = fminf(fmaxf(z, 0.0f), 1.0f);
         float idx = __fmul_rn(zc, 255.0f);
         float mix = idx - truncf(idx);
@@ -342,6 +344,9 @@ struct rfk_iter_args {
     u32 flags;                 // RFK_FLAG_*
     u32 check_every;           // passes between bailout checks (power of two)
     u32 nsamples;              // rows of params (precalc kernel)
+    u64* queue;                // queue binning: [npasses][walkers] draw records
+    u32 tiles_x;               // queue binning: 64-pixel tiles per row
+    u32 queue_pad;
 };
 
 // Persistent batch launch: work items are (frame, block) pairs
@@ -371,5 +376,6 @@ struct rfk_batch_args {
 #define RFK_CNT_START 4
 #define RFK_CNT_STOP 5
 #define RFK_CNT_STOPPED 6
+#define RFK_CNT_ORIGIN 7   // queue binning: device clock at the first block of a bin call
 
 #endif  // RFK_DEVICE_H
