@@ -98,6 +98,8 @@ typedef struct rfk_exec_config {
     uint32_t atomic_width;       /* 32-bit bins: four u32 atomics per draw (32) or two u64 (64); 0 = 32 */
     uint32_t bin_tile_log2;      /* LDS bin cache of 2^n slots per block, 6..12 (0 = off) */
     uint32_t hist_bits;          /* histogram bin width: 32 (u32 RGBA) or 64 (u64 RGBA); 0 = 32 */
+    uint32_t bin_queue;          /* 1: draws go to a record queue, sorted by 64x64 screen tile and summed
+                                    per tile in LDS (no contended atomics); excludes bin_tile_log2 */
 } rfk_exec_config;
 int rfk_compile(const rfk_db* db, const rfk_flame* f, int precision, const rfk_exec_config* cfg,
                 rfk_kernel** out, char* log, size_t log_cap);
@@ -129,6 +131,12 @@ int rfk_warmup(rfk_kernel* k, void* stream, const double* samples, size_t n_real
 int rfk_warmup_ex(rfk_kernel* k, void* stream, const double* samples, size_t n_reals, uint32_t width, uint32_t height,
                   uint32_t seed, uint32_t stream_id, uint32_t fuse, uint32_t walkers, int temporal_multiplier,
                   rfk_state** out);
+/* Re-initialise an existing state of the same kernel for a new frame with
+ * its allocations kept (the reference allocates a new state per warmup):
+ * clears the histogram and walker passes, evaluates `samples`, runs precalc,
+ * seeds walkers and the fuse passes. Same image size and walker count.     */
+int rfk_rewarm(rfk_kernel* k, rfk_state* s, const double* samples, size_t n_reals, uint32_t seed, uint32_t stream_id,
+               uint32_t fuse);
 int rfk_state_free(rfk_state* s);
 
 /* ----------------------------------------------------------- autotune --
@@ -139,6 +147,7 @@ int rfk_state_free(rfk_state* s);
  * compile or launch keep their error code in `status`.                     */
 typedef struct rfk_tune_candidate {
     uint32_t threads_per_block, atomic_width, bin_tile_log2, walkers; /* in  */
+    uint32_t bin_queue;                                               /* in  */
     double iters_per_s, compile_ms;                                   /* out */
     int status;                                                       /* out */
 } rfk_tune_candidate;

@@ -25,7 +25,7 @@ PIXEL_BYTES = {0: 16, 1: 8, 2: 12, 3: 6, 4: 4}
 
 class ExecConfig(C.Structure):
     _fields_ = [("threads_per_block", C.c_uint32), ("atomic_width", C.c_uint32),
-                ("bin_tile_log2", C.c_uint32), ("hist_bits", C.c_uint32)]
+                ("bin_tile_log2", C.c_uint32), ("hist_bits", C.c_uint32), ("bin_queue", C.c_uint32)]
 
 
 class Bailout(C.Structure):
@@ -59,8 +59,8 @@ class BatchFrameInfo(C.Structure):
 
 class TuneCandidate(C.Structure):
     _fields_ = [("threads_per_block", C.c_uint32), ("atomic_width", C.c_uint32), ("bin_tile_log2", C.c_uint32),
-                ("walkers", C.c_uint32), ("iters_per_s", C.c_double), ("compile_ms", C.c_double),
-                ("status", C.c_int)]
+                ("walkers", C.c_uint32), ("bin_queue", C.c_uint32), ("iters_per_s", C.c_double),
+                ("compile_ms", C.c_double), ("status", C.c_int)]
 
 
 class TonemapArgs(C.Structure):
@@ -102,6 +102,7 @@ _SIGS = [
                              C.c_uint32, C.c_uint32, C.c_int, C.POINTER(_P)]),
     ("rfk_warmup_ex", C.c_int, [_P, _P, C.POINTER(C.c_double), C.c_size_t, C.c_uint32, C.c_uint32, C.c_uint32,
                                 C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(_P)]),
+    ("rfk_rewarm", C.c_int, [_P, _P, C.POINTER(C.c_double), C.c_size_t, C.c_uint32, C.c_uint32, C.c_uint32]),
     ("rfk_state_free", C.c_int, [_P]),
     ("rfk_autotune", C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_double), C.c_size_t, C.c_uint32, C.c_uint32,
                                C.c_uint64, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),

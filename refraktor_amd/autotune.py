@@ -27,13 +27,13 @@ class Shape:
     atomic_width: int = 32
     bin_tile_log2: int = 0
     walkers: int = 1 << 20
+    bin_queue: int = 0
 
 
-DEFAULT_GRID = [Shape(tpb, aw, bt, w)
-                for tpb in (256, 512, 1024)
-                for aw in (32, 64)
-                for bt in (0, 11)
-                for w in (1 << 18, 1 << 20)]
+# atomic binning with the LDS bin cache at a few shapes, and queue binning
+DEFAULT_GRID = ([Shape(tpb, aw, bt, w) for tpb in (256, 1024) for aw in (32, 64) for bt in (11, 12)
+                 for w in (1 << 20,)] +
+                [Shape(tpb, 64, 0, w, 1) for tpb in (256, 512, 1024) for w in (1 << 19, 1 << 20, 1 << 21)])
 
 
 def _cache_path() -> Path:
@@ -54,6 +54,7 @@ def measure(db: render.VariationDB, flame: render.Flame, samples: np.ndarray, wi
     for i, s in enumerate(grid):
         arr[i].threads_per_block, arr[i].atomic_width = s.threads_per_block, s.atomic_width
         arr[i].bin_tile_log2, arr[i].walkers = s.bin_tile_log2, s.walkers
+        arr[i].bin_queue = s.bin_queue
     samples = np.ascontiguousarray(samples, dtype=np.float64)
     best = C.c_size_t()
     check(_lib.lib().rfk_autotune(db.handle, flame.handle, precision, samples.ctypes.data_as(C.POINTER(C.c_double)),
@@ -82,7 +83,8 @@ def tune(db: render.VariationDB, flame: render.Flame, samples: np.ndarray, width
         return Shape(**cache[key]["shape"]), cache[key]["results"]
     results = measure(db, flame, samples, width, height, grid, probe_iters, precision)
     best = next(r for r in results if r["best"])
-    shape = Shape(best["threads_per_block"], best["atomic_width"], best["bin_tile_log2"], best["walkers"])
+    shape = Shape(best["threads_per_block"], best["atomic_width"], best["bin_tile_log2"], best["walkers"],
+                  best["bin_queue"])
     if use_cache:
         cache[key] = {"shape": asdict(shape), "results": results}
         try:

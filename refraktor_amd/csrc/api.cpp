@@ -71,6 +71,15 @@ struct rfk_state {
     uint32_t stream_id = 0;
     bool hist64 = false;
     size_t hist_words() const { return size_t(width) * height * (hist64 ? 8u : 4u); }
+    // warmup scratch: keyframe samples, their segments, affine kind table
+    double *d_samples = nullptr, *d_segments = nullptr;
+    int8_t* d_kind = nullptr;
+    uint32_t scratch_samples = 0;
+    // queue binning scratch (kernels compiled with bin_queue)
+    unsigned long long *queue = nullptr, *sorted = nullptr;
+    uint32_t *qcounts = nullptr, *qnseg = nullptr;
+    void* qsegments = nullptr;
+    uint32_t chunk_passes = 0, max_segments = 0;
 };
 
 struct rfk_batch {
@@ -97,6 +106,9 @@ struct rfk_batch {
 };
 
 namespace {
+
+// partitions of the queue binning count/scatter kernels (one workgroup each)
+constexpr uint32_t kQueueParts = 256;
 
 thread_local std::string g_last_error;
 
@@ -134,6 +146,8 @@ hipError_t dalloc(T** p, size_t count) {
 void free_state_buffers(rfk_state* s) {
     auto f = [](void* p) { if (p) (void)hipFree(p); };
     f(s->params); f(s->palettes); f(s->wx); f(s->wy); f(s->wc); f(s->wlast); f(s->block_pass); f(s->hist);
+    f(s->queue); f(s->sorted); f(s->qcounts); f(s->qnseg); f(s->qsegments);
+    f(s->d_samples); f(s->d_segments); f(s->d_kind);
     f(s->counters); f(s->dstats);
     if (s->counters_host) (void)hipHostFree(s->counters_host);
     if (s->stop_host) (void)hipHostFree(s->stop_host);
@@ -163,12 +177,37 @@ rfk_iter_args base_args(rfk_state* s) {
     a.total_walkers = s->walkers;
     a.check_every = 16;
     a.nsamples = s->blocks;
+    a.queue = reinterpret_cast<u64*>(s->queue);
+    a.tiles_x = rfk::sk::queue_tiles_x(s->width);
     return a;
 }
 
 int launch_iterate(rfk_state* s, rfk_iter_args& a) {
     void* argv[] = {&a};
     RFK_HIP(hipModuleLaunchKernel(s->k->f_iterate, s->blocks, 1, 1, s->tpb, 1, 1, 0, s->stream, argv, nullptr));
+    return RFK_OK;
+}
+
+// Queue binning: chunks of `chunk_passes` passes, each followed by the sort
+// and per-tile sum of its records. With an iteration budget every chunk is
+// enqueued at once (a bailout makes the remaining chunks record nothing);
+// without one the host checks the device's stop count after every chunk.
+int bin_queued(rfk_state* s, rfk_iter_args& a, uint64_t passes, bool bounded) {
+    for (uint64_t done = 0; done < passes;) {
+        const uint32_t n = uint32_t(std::min<uint64_t>(s->chunk_passes, passes - done));
+        a.npasses = n;
+        if (int rc = launch_iterate(s, a)) return rc;
+        RFK_HIP(rfk::sk::queue_bin(s->queue, size_t(n) * s->walkers, s->sorted, s->qcounts, kQueueParts,
+                                   s->qsegments, s->max_segments, s->qnseg, s->width, s->height, s->hist, s->hist64,
+                                   s->k->opt.atomic_width == 64, s->stream));
+        done += n;
+        if (!bounded && done < passes) {
+            RFK_HIP(hipMemcpyAsync(s->counters_host, s->counters, 8 * sizeof(unsigned long long),
+                                   hipMemcpyDeviceToHost, s->stream));
+            RFK_HIP(hipStreamSynchronize(s->stream));
+            if (s->counters_host[RFK_CNT_STOPPED] != 0) break;
+        }
+    }
     return RFK_OK;
 }
 
@@ -320,6 +359,8 @@ static int make_options(const rfk_exec_config* cfg, int precision, rfk::compile_
     const uint32_t hb = (cfg && cfg->hist_bits) ? cfg->hist_bits : 32u;
     if (hb != 32 && hb != 64) return fail(RFK_ERR_INVALID, "hist_bits must be 32 or 64");
     o.hist64 = hb == 64;
+    o.bin_queue = cfg && cfg->bin_queue;
+    if (o.bin_queue && o.bin_tile_log2) return fail(RFK_ERR_INVALID, "bin_queue and bin_tile_log2 exclude each other");
     if (o.bin_tile_log2 != 0 && (o.bin_tile_log2 < 6 || o.bin_tile_log2 > 12))
         return fail(RFK_ERR_INVALID, "bin_tile_log2 must be 0 or 6..12");
     // static LDS of one workgroup: palette, two point buffers, bin cache
@@ -436,6 +477,56 @@ int rfk_fix_rotation(double* samples, size_t sample_size, size_t flame_size, con
     return RFK_OK;
 }
 
+// Everything of a warmup that does not allocate: clear the histogram and pass
+// counters, fix rotations, Catmull-Rom + per-block sample evaluation,
+// precalc, walker seeding and the fuse passes.
+static int state_prepare(rfk_kernel* k, rfk_state* s, const double* samples, size_t n_reals, uint32_t seed,
+                         uint32_t stream_id, uint32_t fuse) {
+    const size_t sample_size = size_t(k->L.size_reals) + 768;
+    if (n_reals % sample_size != 0 || n_reals / sample_size < 4)
+        return fail(RFK_ERR_INVALID, "samples must hold >= 4 packed samples of " + std::to_string(sample_size) + " reals");
+    const uint32_t nsamp = uint32_t(n_reals / sample_size), nseg = nsamp - 3;
+    s->seed = seed;
+    s->stream_id = stream_id;
+    RFK_HIP(hipMemsetAsync(s->hist, 0, s->hist_words() * 4, s->stream));
+    RFK_HIP(hipMemsetAsync(s->block_pass, 0, size_t(s->blocks) * 4, s->stream));
+    if (s->wlast) RFK_HIP(hipMemsetAsync(s->wlast, 0xFF, size_t(s->walkers) * 4, s->stream));
+    // sample scratch grows with the number of keyframe samples
+    if (s->scratch_samples < nsamp) {
+        if (s->d_samples) (void)hipFree(s->d_samples);
+        if (s->d_segments) (void)hipFree(s->d_segments);
+        s->d_samples = nullptr;
+        s->d_segments = nullptr;
+        s->scratch_samples = 0;
+        RFK_HIP(hipMalloc(&s->d_samples, n_reals * sizeof(double)));
+        RFK_HIP(hipMalloc(&s->d_segments, size_t(nseg) * sample_size * 4 * sizeof(double)));
+        s->scratch_samples = nsamp;
+    }
+    std::vector<double> host(samples, samples + n_reals);
+    rfk::fix_rotation(host.data(), sample_size, size_t(k->L.size_reals), k->L.affine_indices.data(),
+                      k->L.affine_indices.size(), int(nsamp));
+    RFK_HIP(hipMemcpyAsync(s->d_samples, host.data(), n_reals * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    RFK_HIP(rfk::sk::catmull(s->d_samples, uint32_t(sample_size), nseg, s->d_segments, s->stream));
+    RFK_HIP(rfk::sk::eval_samples(s->d_segments, uint32_t(sample_size), s->size_reals, nseg, s->blocks, s->d_kind,
+                                  s->precision, s->params, s->param_stride, s->palettes, s->stream));
+    // per-sample precalc + jitter (generated), then walker seeding and fuse passes
+    rfk_iter_args a = base_args(s);
+    {
+        void* argv[] = {&a};
+        uint32_t tb = 64, g = (s->blocks + tb - 1) / tb;
+        RFK_HIP(hipModuleLaunchKernel(k->f_precalc, g, 1, 1, tb, 1, 1, 0, s->stream, argv, nullptr));
+    }
+    if (int rc = reset_counters(s)) return rc;
+    a.flags = RFK_FLAG_INIT;
+    a.npasses = fuse;
+    if (int rc = launch_iterate(s, a)) return rc;
+    // the host copy of the samples must outlive the upload
+    RFK_HIP(hipStreamSynchronize(s->stream));
+    s->passes_done = fuse;
+    s->quality = 0.0;
+    return RFK_OK;
+}
+
 int rfk_warmup(rfk_kernel* k, void* stream, const double* samples, size_t n_reals, uint32_t width, uint32_t height,
                uint32_t seed, uint32_t fuse, uint32_t walkers, int temporal_multiplier, rfk_state** out) {
     return rfk_warmup_ex(k, stream, samples, n_reals, width, height, seed, 0u, fuse, walkers, temporal_multiplier, out);
@@ -451,8 +542,6 @@ int rfk_warmup_ex(rfk_kernel* k, void* stream, const double* samples, size_t n_r
     if (n_reals % sample_size != 0 || n_reals / sample_size < 4)
         return fail(RFK_ERR_INVALID, "samples must hold >= 4 packed samples of " + std::to_string(sample_size) + " reals");
     if (int rc = check_device()) return rc;
-    const uint32_t nsamp = uint32_t(n_reals / sample_size);
-    const uint32_t nseg = nsamp - 3;
 
     auto s = std::make_unique<rfk_state>();
     s->k = k;
@@ -505,6 +594,18 @@ int rfk_warmup_ex(rfk_kernel* k, void* stream, const double* samples, size_t n_r
     RFK_HIPC(dalloc(&s->block_pass, s->blocks));
     s->hist64 = k->opt.hist64;
     RFK_HIPC(dalloc(&s->hist, s->hist_words()));
+    if (k->opt.bin_queue) {
+        // chunks of whole passes of all walkers, about 2^26 records (512 MiB) each
+        s->chunk_passes = std::max<uint32_t>(1u, uint32_t((1ull << 26) / s->walkers));
+        const size_t q = size_t(s->chunk_passes) * s->walkers;
+        const uint32_t ntiles = rfk::sk::queue_tiles(s->width, s->height);
+        s->max_segments = rfk::sk::queue_max_segments(q, ntiles);
+        RFK_HIPC(hipMalloc(&s->queue, q * 8));
+        RFK_HIPC(hipMalloc(&s->sorted, q * 8));
+        RFK_HIPC(dalloc(&s->qcounts, size_t(ntiles) * kQueueParts));
+        RFK_HIPC(hipMalloc(&s->qsegments, size_t(s->max_segments) * 16));
+        RFK_HIPC(dalloc(&s->qnseg, 4));
+    }
     RFK_HIPC(dalloc(&s->counters, 8));
     RFK_HIPC(dalloc(&s->dstats, 98));
     RFK_HIPC(hipHostMalloc(reinterpret_cast<void**>(&s->counters_host), 128 * sizeof(unsigned long long),
@@ -514,47 +615,23 @@ int rfk_warmup_ex(rfk_kernel* k, void* stream, const double* samples, size_t n_r
     RFK_HIPC(hipHostGetDevicePointer(reinterpret_cast<void**>(&s->stop_dev), s->stop_host, 0));
     RFK_HIPC(hipEventCreate(&s->ev0));
     RFK_HIPC(hipEventCreate(&s->ev1));
-    RFK_HIPC(hipMemsetAsync(s->hist, 0, s->hist_words() * 4, s->stream));
-    RFK_HIPC(hipMemsetAsync(s->block_pass, 0, size_t(s->blocks) * 4, s->stream));
-    if (s->wlast) RFK_HIPC(hipMemsetAsync(s->wlast, 0xFF, size_t(s->walkers) * 4, s->stream));
-
-    // host fix_rotation, upload, Catmull-Rom, per-sample evaluation
-    std::vector<double> host(samples, samples + n_reals);
-    rfk::fix_rotation(host.data(), sample_size, size_t(k->L.size_reals), k->L.affine_indices.data(),
-                      k->L.affine_indices.size(), int(nsamp));
     std::vector<int8_t> kind(size_t(k->L.size_reals), 0);
-    for (size_t a : k->L.affine_indices)
-        for (int c = 0; c < 4; c++) kind[a + size_t(c)] = int8_t(1 + c);
-    double *d_samples = nullptr, *d_segments = nullptr;
-    int8_t* d_kind = nullptr;
-    RFK_HIPC(hipMalloc(&d_samples, n_reals * sizeof(double)));
-    RFK_HIPC(hipMalloc(&d_segments, size_t(nseg) * sample_size * 4 * sizeof(double)));
-    RFK_HIPC(hipMalloc(&d_kind, kind.size() + 16));
-    RFK_HIPC(hipMemcpyAsync(d_samples, host.data(), n_reals * sizeof(double), hipMemcpyHostToDevice, s->stream));
-    RFK_HIPC(hipMemcpyAsync(d_kind, kind.data(), kind.size(), hipMemcpyHostToDevice, s->stream));
-    RFK_HIPC(rfk::sk::catmull(d_samples, uint32_t(sample_size), nseg, d_segments, s->stream));
-    RFK_HIPC(rfk::sk::eval_samples(d_segments, uint32_t(sample_size), s->size_reals, nseg, s->blocks, d_kind,
-                                   s->precision, s->params, s->param_stride, s->palettes, s->stream));
-
-    // per-sample precalc + jitter (generated), then walker seeding and fuse passes
-    rfk_iter_args a = base_args(S);
-    {
-        void* argv[] = {&a};
-        uint32_t tb = 64, g = (s->blocks + tb - 1) / tb;
-        RFK_HIPC(hipModuleLaunchKernel(k->f_precalc, g, 1, 1, tb, 1, 1, 0, s->stream, argv, nullptr));
-    }
-    if (int rc = reset_counters(S)) return cleanup_on_error(rc);
-    a.flags = RFK_FLAG_INIT;
-    a.npasses = fuse;
-    if (int rc = launch_iterate(S, a)) return cleanup_on_error(rc);
-    RFK_HIPC(hipStreamSynchronize(s->stream));
-    (void)hipFree(d_samples);
-    (void)hipFree(d_segments);
-    (void)hipFree(d_kind);
-    s->passes_done = fuse;
+    for (size_t ai : k->L.affine_indices)
+        for (int c = 0; c < 4; c++) kind[ai + size_t(c)] = int8_t(1 + c);
+    RFK_HIPC(hipMalloc(&s->d_kind, kind.size() + 16));
+    RFK_HIPC(hipMemcpy(s->d_kind, kind.data(), kind.size(), hipMemcpyHostToDevice));
 #undef RFK_HIPC
+    if (int rc = state_prepare(k, S, samples, n_reals, seed, stream_id, fuse)) return cleanup_on_error(rc);
     *out = s.release();
     return RFK_OK;
+}
+
+int rfk_rewarm(rfk_kernel* k, rfk_state* s, const double* samples, size_t n_reals, uint32_t seed, uint32_t stream_id,
+               uint32_t fuse) {
+    if (!k || !s || !samples) return fail(RFK_ERR_INVALID, "null argument");
+    if (s->k != k) return fail(RFK_ERR_INVALID, "state belongs to another kernel");
+    if (s->pending) return fail(RFK_ERR_INVALID, "a bin call is pending on this state");
+    return state_prepare(k, s, samples, n_reals, seed, stream_id, fuse);
 }
 
 int rfk_state_free(rfk_state* s) {
@@ -579,11 +656,18 @@ int rfk_bin_async(rfk_kernel* k, rfk_state* s, const rfk_bailout* bo) {
     a.flags = RFK_FLAG_DRAW;
     if (has_time || has_q) a.flags |= RFK_FLAG_BAILOUT;
     a.time_limit_ticks = has_time ? uint64_t(bo->millis * s->wallclock_khz) : 0;
-    a.quality_target = has_q ? uint64_t(bo->quality * double(s->width) * s->height * 255.0 / s->blocks) : 0;
+    // atomic binning checks quality per block (its share), queue binning
+    // against the whole call's hits
+    const double q_total = bo->quality * double(s->width) * s->height * 255.0;
+    a.quality_target = has_q ? uint64_t(k->opt.bin_queue ? q_total : q_total / s->blocks) : 0;
     *s->stop_host = 0;
     if (int rc = reset_counters(s)) return rc;
     RFK_HIP(hipEventRecord(s->ev0, s->stream));
-    if (int rc = launch_iterate(s, a)) return rc;
+    if (!k->opt.bin_queue) {
+        if (int rc = launch_iterate(s, a)) return rc;
+    } else if (int rc = bin_queued(s, a, passes, bo->iters != 0)) {
+        return rc;
+    }
     RFK_HIP(hipEventRecord(s->ev1, s->stream));
     // density statistics after every bin, as the reference does (flame_compiler.cpp:928-937)
     RFK_HIP(hipMemsetAsync(s->dstats, 0, 98 * sizeof(unsigned long long), s->stream));
@@ -703,7 +787,7 @@ int rfk_autotune(const rfk_db* db, const rfk_flame* f, int precision, const doub
         rfk_tune_candidate& c = cands[i];
         c.iters_per_s = 0.0;
         c.compile_ms = 0.0;
-        rfk_exec_config cfg{c.threads_per_block, c.atomic_width, c.bin_tile_log2, 0u};
+        rfk_exec_config cfg{c.threads_per_block, c.atomic_width, c.bin_tile_log2, 0u, c.bin_queue};
         rfk_kernel* k = nullptr;
         c.status = rfk_compile(db, f, precision, &cfg, &k, nullptr, 0);
         if (c.status != RFK_OK) continue;
@@ -752,6 +836,7 @@ int rfk_batch_create(rfk_kernel* k, void* stream, const double* samples, size_t 
                      uint32_t fuse, rfk_batch** out) {
     if (!k || !samples || !iters || !out || frames == 0) return fail(RFK_ERR_INVALID, "bad argument");
     if (k->L.chaos_base >= 0) return fail(RFK_ERR_INVALID, "batches need a kernel without xaos");
+    if (k->opt.bin_queue) return fail(RFK_ERR_INVALID, "batches bin with atomics: compile without bin_queue");
     if (width == 0 || height == 0 || width > 32768 || height > 32768) return fail(RFK_ERR_INVALID, "bad image size");
     const size_t sample_size = size_t(k->L.size_reals) + 768;
     if (n_reals % sample_size != 0 || n_reals / sample_size < 4)

@@ -205,6 +205,7 @@ std::string make_source(const variation_db& db, const flame& f, const flame_layo
     os << "#define RFK_ATOMIC64 " << (o.atomic_width == 64 ? 1 : 0) << "\n";
     os << "#define RFK_BIN_TILE_LOG2 " << o.bin_tile_log2 << "\n";
     os << "#define RFK_HIST64 " << (o.hist64 ? 1 : 0) << "\n";
+    os << "#define RFK_BIN_QUEUE " << (o.bin_queue ? 1 : 0) << "\n";
     os << device_runtime_source() << "\n";
     os << "// ---------------------------------------------------------------- flame\n";
     os << make_flame_functions(db, f, L) << "\n";

@@ -17,6 +17,7 @@ struct compile_options {
     uint32_t atomic_width = 32;
     uint32_t bin_tile_log2 = 0;  // 0: no LDS bin cache, else 2^n cached bins per block
     bool hist64 = false;         // u64 RGBA bins instead of u32
+    bool bin_queue = false;      // draws to a record queue (queue_bin.hip) instead of atomics
 };
 
 // Generated source: configuration macros, the device runtime, the per-flame

@@ -26,6 +26,26 @@
 #ifndef RFK_HIST64
 #define RFK_HIST64 0
 #endif
+#ifndef RFK_BIN_QUEUE
+#define RFK_BIN_QUEUE 0
+#endif
+
+// Queue binning: instead of atomics, every pass of every walker stores one
+// 64-bit draw record at queue[pass][walker] (coalesced, no atomics); the
+// records are then counting-sorted by 64x64-pixel screen tile and summed per
+// tile in LDS by the library's queue kernels. Record layout:
+//   bits  0..31  R | G << 8 | B << 16 | W << 24   (fixed-point increments)
+//   bits 32..43  pixel inside the tile (y * 64 + x)
+//   bits 44..63  tile index (ty * tiles_x + tx); all ones = no draw
+#define RFK_QTILE_LOG2 6
+#define RFK_QEMPTY 0xFFFFFFFFFFFFFFFFull
+
+RFK_INLINE u64 rfk_queue_record(const rfk_draw_t& d, u32 tiles_x) {
+    const u32 tile = (d.py >> RFK_QTILE_LOG2) * tiles_x + (d.px >> RFK_QTILE_LOG2);
+    const u32 local = ((d.py & 63u) << RFK_QTILE_LOG2) | (d.px & 63u);
+    const u32 col = d.r | (d.g << 8) | (d.b << 16) | (d.w << 24);
+    return u64(col) | (u64(local) << 32) | (u64(tile) << 44);
+}
 
 // bin element: u32 fixed point (wraps past 2^32 / 255 full-opacity draws in
 // one pixel) or u64 for budgets that can put more than that into a pixel
@@ -137,6 +157,15 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, const Real* __r
         sm.bad = 0;
         sm.stop = 0;
         sm.start = wall_clock64();
+#if RFK_BIN_QUEUE
+        // a bin call spans several launches: its clock starts with the first
+        // block of the first launch, and a stop seen by any block ends it
+        if (bailout) {
+            atomicCAS(&a.counters[RFK_CNT_ORIGIN], 0ull, sm.start);
+            if (__hip_atomic_load(&a.counters[RFK_CNT_STOPPED], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull)
+                sm.stop = 1u;
+        }
+#endif
     }
 #if RFK_BIN_TILE_LOG2
     if (draw) rfk_tile_clear(sm);
@@ -173,7 +202,13 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, const Real* __r
     u32x4 pick_cache = {0u, 0u, 0u, 0u}, perm_cache = {0u, 0u, 0u, 0u};
 
     const u32 ntotal = nfuse + npasses;
-    for (u32 i = 0; i < ntotal; i++) {
+#if RFK_BIN_QUEUE
+    __syncthreads();
+    const u32 nrun = sm.stop ? 0u : ntotal;
+#else
+    const u32 nrun = ntotal;
+#endif
+    for (u32 i = 0; i < nrun; i++) {
         const u32 pass = pass0 + i;
         rs.begin(pass);
         v3 pt = {x, y, c};
@@ -215,8 +250,15 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, const Real* __r
             rfk_dispatch(RFK_NUM_XFORMS, P, d, rs);
 #endif
             rfk_draw_t dr = rfk_make_draw(S, sm.pal, a.width, a.height, float(d.x), float(d.y), float(d.z), float(op));
+#if RFK_BIN_QUEUE
+            u64 rec = RFK_QEMPTY;
+#endif
             if (dr.hit) {
-#if RFK_BIN_TILE_LOG2
+#if RFK_BIN_QUEUE
+                // opacities above 1 overflow the 8-bit fields: bin those directly
+                if ((dr.r | dr.g | dr.b | dr.w) <= 255u) rec = rfk_queue_record(dr, a.tiles_x);
+                else rfk_commit_draw(hist, dr);
+#elif RFK_BIN_TILE_LOG2
                 rfk_tile_draw(sm, hist, dr);
 #else
                 rfk_commit_draw(hist, dr);
@@ -224,6 +266,9 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, const Real* __r
                 hits += dr.w;
                 draws++;
             }
+#if RFK_BIN_QUEUE
+            a.queue[(size_t)(i - nfuse) * a.total_walkers + gid] = rec;
+#endif
         }
 #if RFK_BIN_TILE_LOG2
         if (draw && ((i + 1u) % RFK_TILE_FLUSH) == 0u) {
@@ -276,8 +321,21 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, const Real* __r
             if (t == 0) {
                 bool stop = a.stop_flag &&
                             __hip_atomic_load((u32*)a.stop_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u;
+#if RFK_BIN_QUEUE
+                // whole-call criteria: clock since the call's first block,
+                // hits of finished blocks plus this block's, any block stopped
+                const u64 origin = __hip_atomic_load(&a.counters[RFK_CNT_ORIGIN], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                if (a.time_limit_ticks && wall_clock64() - origin >= a.time_limit_ticks) stop = true;
+                if (a.quality_target && __hip_atomic_load(&a.counters[RFK_CNT_HITS], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT) + sm.hits >= a.quality_target)
+                    stop = true;
+                if (__hip_atomic_load(&a.counters[RFK_CNT_STOPPED], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull)
+                    stop = true;
+#else
                 if (a.time_limit_ticks && wall_clock64() - sm.start >= a.time_limit_ticks) stop = true;
                 if (a.quality_target && sm.hits >= a.quality_target) stop = true;
+#endif
                 sm.stop = stop ? 1u : 0u;
             }
             __syncthreads();
@@ -289,6 +347,12 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, const Real* __r
         __syncthreads();
         rfk_tile_flush(sm, hist);
     }
+#endif
+#if RFK_BIN_QUEUE
+    // passes a bailout skipped leave no draws behind
+    if (draw)
+        for (u32 j = (done > nfuse ? done : nfuse); j < ntotal; j++)
+            a.queue[(size_t)(j - nfuse) * a.total_walkers + gid] = RFK_QEMPTY;
 #endif
 
     if (wx) {

@@ -41,6 +41,17 @@ hipError_t density_estimate(const uint32_t* hist, bool hist64, uint32_t width, u
 hipError_t tonemap(const void* in, int in_kind, size_t npix, void* out, int format, float scale_constant,
                    float brightness, float gamma, float vibrancy, hipStream_t s);
 
+// Queue binning (queue_bin.hip): fold n draw records (64x64-pixel tile
+// layout, see flame_kernels.h) into the histogram. Scratch: sorted (n
+// records), counts (ntiles * nparts u32), segments (max_segments * 16 B),
+// nseg (one u32). packed64: 32-bit bins updated as two 64-bit pairs.
+uint32_t queue_tiles_x(uint32_t width);
+uint32_t queue_tiles(uint32_t width, uint32_t height);
+uint32_t queue_max_segments(size_t records, uint32_t ntiles);
+hipError_t queue_bin(const unsigned long long* records, size_t n, unsigned long long* sorted, uint32_t* counts,
+                     uint32_t nparts, void* segments, uint32_t max_segments, uint32_t* nseg, uint32_t width,
+                     uint32_t height, uint32_t* hist, bool hist64, bool packed64, hipStream_t s);
+
 // Random RGBA atomic increments for the binning roofline.
 hipError_t atomic_peak(uint32_t* buf, uint64_t nbins, uint32_t grid, uint32_t block, uint32_t iters, uint32_t seed,
                        hipStream_t s);

@@ -170,9 +170,27 @@ def make_pairs(seeds, width: int, height: int, low_iters: int, high_iters: int, 
     return Dataset(feat, base, tgt, n, height, width)
 
 
+def _crop_offsets(rng, data: Dataset, batch: int, crop: int, importance) -> np.ndarray:
+    """(image, y, x) of `batch` crops; with `importance` ([n][gy][gx] coarse
+    weights over 16-pixel cells) crops favour bright, detailed regions"""
+    if importance is None:
+        return np.stack([rng.integers(0, data.n, batch), rng.integers(0, data.height - crop + 1, batch),
+                         rng.integers(0, data.width - crop + 1, batch)], axis=1).astype(np.int32)
+    out = np.empty((batch, 3), np.int32)
+    for b in range(batch):
+        i = int(rng.integers(0, data.n))
+        w = importance[i].ravel()
+        cell = int(rng.choice(w.size, p=w / w.sum()))
+        cy, cx = divmod(cell, importance[i].shape[1])
+        y = int(np.clip(cy * 16 + 8 - crop // 2 + rng.integers(-crop // 4, crop // 4 + 1), 0, data.height - crop))
+        x = int(np.clip(cx * 16 + 8 - crop // 2 + rng.integers(-crop // 4, crop // 4 + 1), 0, data.width - crop))
+        out[b] = (i, y, x)
+    return out
+
+
 def train(den: Denoiser, data: Dataset, steps: int, batch: int = 8, crop: int = 128, lr: float = 1e-3,
-          seed: int = 0, log=print) -> list[float]:
-    """Adam on random crops of the dataset"""
+          seed: int = 0, log=print, importance=None) -> list[float]:
+    """Adam on random crops of the dataset (cosine learning-rate decay)"""
     rng = np.random.default_rng(seed)
     lib = _lib.lib()
     cp = crop * crop
@@ -182,8 +200,7 @@ def train(den: Denoiser, data: Dataset, steps: int, batch: int = 8, crop: int = 
     losses = []
     t0 = time.time()
     for step in range(steps):
-        offs = np.stack([rng.integers(0, data.n, batch), rng.integers(0, data.height - crop + 1, batch),
-                         rng.integers(0, data.width - crop + 1, batch)], axis=1).astype(np.int32)
+        offs = _crop_offsets(rng, data, batch, crop, importance)
         op = offs.ctypes.data
         check(lib.rfk_nn_gather_crops(data.feat.ptr, data.height, data.width, FEAT_C * 2, op, batch, crop, crop,
                                       bf.ptr, None))

@@ -106,9 +106,9 @@ class VariationDB:
         return Flame(self, None, handle=h.value)
 
     def compile(self, flame, precision: int = 0, threads_per_block: int = 256, atomic_width: int = 32,
-                bin_tile_log2: int = 0, hist_bits: int = 32) -> "Kernel":
+                bin_tile_log2: int = 0, hist_bits: int = 32, bin_queue: bool = False) -> "Kernel":
         f = flame if isinstance(flame, Flame) else self.flame(flame)
-        return Kernel(self, f, precision, threads_per_block, atomic_width, bin_tile_log2, hist_bits)
+        return Kernel(self, f, precision, threads_per_block, atomic_width, bin_tile_log2, hist_bits, bin_queue)
 
     def __del__(self):
         try:
@@ -166,9 +166,9 @@ class Flame:
         return buf.value.decode()
 
     def compile_check(self, precision: int = 0, threads_per_block: int = 256, atomic_width: int = 32,
-                      bin_tile_log2: int = 0) -> int:
+                      bin_tile_log2: int = 0, hist_bits: int = 32, bin_queue: bool = False) -> int:
         """hiprtc-compile the generated source without a device; returns code-object bytes"""
-        cfg = _lib.ExecConfig(threads_per_block, atomic_width, bin_tile_log2, 0)
+        cfg = _lib.ExecConfig(threads_per_block, atomic_width, bin_tile_log2, hist_bits, int(bin_queue))
         n = C.c_size_t()
         log = C.create_string_buffer(1 << 16)
         rc = self._lib.rfk_compile_check(self.db.handle, self.handle, precision, C.byref(cfg), C.byref(n), log,
@@ -187,11 +187,11 @@ class Flame:
 
 class Kernel:
     def __init__(self, db: VariationDB, flame: Flame, precision: int, tpb: int, atomic_width: int,
-                 bin_tile_log2: int = 0, hist_bits: int = 32):
+                 bin_tile_log2: int = 0, hist_bits: int = 32, bin_queue: bool = False):
         self.db, self.flame, self._lib = db, flame, db._lib
         self.precision = precision
         self.hist_bits = hist_bits
-        cfg = _lib.ExecConfig(tpb, atomic_width, bin_tile_log2, hist_bits)
+        cfg = _lib.ExecConfig(tpb, atomic_width, bin_tile_log2, hist_bits, int(bin_queue))
         h = C.c_void_p()
         log = C.create_string_buffer(1 << 16)
         rc = self._lib.rfk_compile(db.handle, flame.handle, precision, C.byref(cfg), C.byref(h), log, len(log))
@@ -237,6 +237,13 @@ class Kernel:
 class State:
     def __init__(self, kernel: Kernel, handle: int):
         self.kernel, self._lib, self.handle = kernel, kernel._lib, handle
+
+    def rewarm(self, samples: np.ndarray, seed: int, fuse: int = 20, stream_id: int = 0) -> "State":
+        """new frame on the same allocations (same kernel, size and walkers)"""
+        samples = np.ascontiguousarray(samples, dtype=np.float64)
+        check(self._lib.rfk_rewarm(self.kernel.handle, self.handle, samples.ctypes.data_as(C.POINTER(C.c_double)),
+                                   samples.size, seed, stream_id, fuse))
+        return self
 
     def info(self) -> _lib.StateInfo:
         inf = _lib.StateInfo()

@@ -32,9 +32,23 @@ def training_flame(seed: int) -> dict:
     return fl.flame_keyframe(seed) if seed % 2 == 0 else fl.flame_1080(seed)
 
 
+# launch shape of the renders: 1024 walkers per block, LDS bin cache, 64-bit
+# bins so the converged targets cannot wrap
+SHAPE = dict(threads_per_block=1024, atomic_width=64, bin_tile_log2=12, hist_bits=64)
+
+
 def compile_all(db: render.VariationDB, flames: list[dict], threads: int = 16) -> list:
     with ThreadPoolExecutor(max_workers=threads) as ex:
-        return list(ex.map(lambda f: db.compile(f), flames))
+        return list(ex.map(lambda f: db.compile(f, **SHAPE), flames))
+
+
+def coarse_importance(rgb: np.ndarray) -> np.ndarray:
+    """16x16-cell weights: mean brightness plus local contrast, floored"""
+    h, w = rgb.shape[0] // 16 * 16, rgb.shape[1] // 16 * 16
+    cells = rgb[:h, :w].mean(axis=-1).reshape(h // 16, 16, w // 16, 16)
+    m = cells.mean(axis=(1, 3))
+    s = cells.std(axis=(1, 3))
+    return (m + 2.0 * s + 0.02).astype(np.float64)
 
 
 def make_pairs(db, seeds, width, height, low_iters, high_iters, de_radius, threads=16, log=print):
@@ -47,6 +61,7 @@ def make_pairs(db, seeds, width, height, low_iters, high_iters, de_radius, threa
     base = render.DeviceBuffer(n * npx * 12)
     tgt = render.DeviceBuffer(n * npx * 12)
     lib = _lib.lib()
+    importance = []
     t0 = time.time()
     for i, (s, f, k) in enumerate(zip(seeds, flames, kernels)):
         lo = denoise.render_frame(db, f, width, height, low_iters, seed=s * 7 + 1, de_radius=de_radius, kernel=k)
@@ -56,9 +71,10 @@ def make_pairs(db, seeds, width, height, low_iters, high_iters, de_radius, threa
         check(lib.rfk_memcpy_dtod(feat.ptr + i * npx * denoise.FEAT_C * 2, fe.ptr, npx * denoise.FEAT_C * 2))
         check(lib.rfk_memcpy_dtod(base.ptr + i * npx * 12, ba.ptr, npx * 12))
         check(lib.rfk_memcpy_dtod(tgt.ptr + i * npx * 12, t.ptr, npx * 12))
+        importance.append(coarse_importance(t.download(np.float32, (height, width, 3))))
         if i % 16 == 0 or i == n - 1:
             log(f"[pairs] {i + 1}/{n} rendered, {time.time() - t0:.1f} s")
-    return denoise.Dataset(feat, base, tgt, n, height, width)
+    return denoise.Dataset(feat, base, tgt, n, height, width), np.stack(importance)
 
 
 def evaluate(den: denoise.Denoiser, data: denoise.Dataset, idx: list[int]) -> dict:
@@ -101,13 +117,15 @@ def main(argv=None) -> int:
 
     db = render.VariationDB()
     seeds = list(range(TRAIN_SEED0, TRAIN_SEED0 + args.flames + args.holdout))
-    data = make_pairs(db, seeds, args.width, args.height, args.low, args.high, args.de_radius, args.threads)
+    data, importance = make_pairs(db, seeds, args.width, args.height, args.low, args.high, args.de_radius,
+                                  args.threads)
     ntrain = args.flames
     train_view = denoise.Dataset(data.feat, data.base, data.target, ntrain, data.height, data.width)
     den = denoise.Denoiser(seed=1, weights=args.init)
     before = evaluate(den, data, list(range(ntrain, ntrain + args.holdout))) if args.init else None
     t0 = time.time()
-    losses = denoise.train(den, train_view, args.steps, batch=args.batch, crop=args.crop, lr=args.lr)
+    losses = denoise.train(den, train_view, args.steps, batch=args.batch, crop=args.crop, lr=args.lr,
+                           importance=importance[:ntrain])
     train_s = time.time() - t0
     den.save(args.out)
     after = evaluate(den, data, list(range(ntrain, ntrain + args.holdout)))

@@ -120,6 +120,21 @@ def test_bin_points_bit_exact(small):
     _bin_compare(st, edge, "edge cases")
 
 
+def test_rewarm_matches_fresh_warmup(small):
+    """a state re-warmed for a new frame renders what a new state renders"""
+    f, k = small
+    s = k.pack_samples(W, H)
+    fresh = k.warmup(s, W, H, seed=11, fuse=10, walkers=8192)
+    fresh.bin(iters=8192 * 32)
+    st = k.warmup(s, W, H, seed=3, fuse=7, walkers=8192)
+    st.bin(iters=8192 * 5)
+    st.rewarm(s, seed=11, fuse=10)
+    r = st.bin(iters=8192 * 32)
+    assert r["total_passes"] == 8192 * 32
+    np.testing.assert_array_equal(st.histogram(), fresh.histogram())
+    np.testing.assert_array_equal(st.walkers(), fresh.walkers())
+
+
 def test_render_counters_and_determinism(small):
     f, k = small
     out = []

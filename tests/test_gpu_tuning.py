@@ -11,10 +11,11 @@ pytestmark = pytest.mark.gpu
 W, H = 320, 180
 
 
-def _render(db, flm, tpb, aw, tile, seed=4, hist_bits=32):
-    k = db.compile(flm, threads_per_block=tpb, atomic_width=aw, bin_tile_log2=tile, hist_bits=hist_bits)
-    st = k.warmup(k.pack_samples(W, H), W, H, seed=seed, walkers=1 << 14)
-    res = st.bin(iters=1 << 21)
+def _render(db, flm, tpb, aw, tile, seed=4, hist_bits=32, queue=False, walkers=1 << 14, iters=1 << 21):
+    k = db.compile(flm, threads_per_block=tpb, atomic_width=aw, bin_tile_log2=tile, hist_bits=hist_bits,
+                   bin_queue=queue)
+    st = k.warmup(k.pack_samples(W, H), W, H, seed=seed, walkers=walkers)
+    res = st.bin(iters=iters)
     return st, res
 
 
@@ -31,6 +32,45 @@ def test_atomic_width_and_bin_tiles_bit_exact(gpu, db, name):
         got = st.histogram()
         assert got.dtype == (np.uint64 if hb == 64 else np.uint32)
         np.testing.assert_array_equal(got.astype(np.uint64), ref, err_msg=f"atomic {aw} tile {tile} bins {hb}")
+
+
+@pytest.mark.parametrize("name", ["small", "1080"])
+def test_queue_binning_bit_exact(gpu, db, name):
+    """records sorted by screen tile and summed in LDS give the atomic histogram"""
+    flm = fl.flame_small(3) if name == "small" else fl.flame_1080(3)
+    rst, rres = _render(db, flm, 256, 32, 0)
+    ref = rst.histogram().astype(np.uint64)
+    for aw, hb, tpb in ((32, 32, 256), (64, 32, 256), (32, 64, 1024)):
+        st, res = _render(db, flm, tpb, aw, 0, hist_bits=hb, queue=True)
+        assert res["total_draws"] == rres["total_draws"] and res["total_passes"] == rres["total_passes"]
+        np.testing.assert_array_equal(st.histogram().astype(np.uint64), ref, err_msg=f"queue aw {aw} bins {hb}")
+
+
+def test_queue_binning_many_chunks(gpu, db):
+    """2^20 walkers x 200 passes: four record chunks of 64 passes and a short one"""
+    flm = fl.flame_1080(4)
+    args = dict(walkers=1 << 20, iters=200 << 20)
+    a, ra = _render(db, flm, 1024, 64, 12, **args)
+    b, rb = _render(db, flm, 1024, 64, 0, queue=True, **args)
+    assert ra["total_passes"] == rb["total_passes"] == 200 << 20
+    np.testing.assert_array_equal(a.histogram(), b.histogram())
+
+
+def test_queue_binning_bailouts(gpu, db):
+    k = db.compile(fl.flame_1080(2), threads_per_block=1024, atomic_width=64, bin_queue=True)
+    s = k.pack_samples(W, H)
+    # quality: stops once the whole call reached the target mean density
+    st = k.warmup(s, W, H, seed=3, walkers=1 << 16)
+    r = st.bin(quality=20.0)
+    assert r["quality"] >= 20.0 and r["total_passes"] % (1 << 16) == 0
+    assert r["quality"] < 60.0  # stopped within a few bailout checks
+    # time: a few milliseconds of device clock
+    st = k.warmup(s, W, H, seed=3, walkers=1 << 16)
+    r = st.bin(millis=5.0)
+    assert 0 < r["total_passes"] and r["kernel_ms"] < 500.0
+    # the histogram holds exactly the counted draws
+    h = st.histogram()
+    assert int(h[..., 3].sum(dtype=np.uint64)) == r["hit_sum"]
 
 
 def test_64bit_bins_feed_the_same_image(gpu, db):
