@@ -240,6 +240,7 @@ typedef struct rfk_state_info {
     void* stream;             /* hipStream_t                                 */
     double quality;           /* accumulated mean density over all bin calls */
     uint32_t hist_bits;       /* 32 or 64                                    */
+    uint32_t* walker_last;    /* device, walkers: last xform (xaos flames), else NULL */
 } rfk_state_info;
 int rfk_state_get_info(const rfk_state* s, rfk_state_info* out);
 /* Zero the histogram and counters (walkers keep their positions). */

@@ -117,13 +117,15 @@ enum {
 typedef struct {
     const int* prog;
     int n_xforms, has_final;
+    int chaos_base;  /* offset of the xaos rows (n x (1 + n) reals), -1 without xaos */
     int xform_start[256];
 } flame_prog;
 
-static void prog_index(flame_prog* fp, const int* prog, int n_xforms, int has_final) {
+static void prog_index(flame_prog* fp, const int* prog, int n_xforms, int has_final, int chaos_base) {
     fp->prog = prog;
     fp->n_xforms = n_xforms;
     fp->has_final = has_final;
+    fp->chaos_base = chaos_base;
     int i = 0;
     for (int x = 0; x < n_xforms + has_final; x++) {
         fp->xform_start[x] = i;
@@ -348,6 +350,29 @@ static u32 select_xform(const flame_prog* fp, const float* P, float ratio) {
     return last_nonzero;
 }
 
+/* xaos selection (flame.tpl:127-142): row `last` of the xaos table (its
+ * first real is the row's weight sum), cumulative scan as select_xform;
+ * no previous xform yet (last >= n) uses the plain weights */
+static u32 select_xform_chaos(const flame_prog* fp, const float* P, u32 last, float ratio) {
+    const int n = fp->n_xforms;
+    if (last >= (u32)n) return select_xform(fp, P, ratio);
+    const float* row = P + fp->chaos_base + (size_t)last * (n + 1);
+    float rsum = 0.0f;
+    u32 last_nonzero = 0;
+    ratio = ratio * row[0];
+    for (int i = 0; i < n; i++) {
+        float w = row[i + 1];
+        if (i + 1 < n) {
+            if (w != 0.0f && rsum + w >= ratio) return (u32)i;
+            rsum = rsum + w;
+            if (w != 0.0f) last_nonzero = (u32)i;
+        } else {
+            return w != 0.0f ? (u32)i : last_nonzero;
+        }
+    }
+    return last_nonzero;
+}
+
 static inline int badvalue(double v) { return !(fabs(v) <= 1e10); }
 
 /* -------------------------------------------------------------- draw */
@@ -387,6 +412,59 @@ void orc_bin_points(const float* S, const u32* pal, const float* pts, size_t n, 
  * GPU permutes them), so only one step of error is compared per pass.
  * trace: per pass per walker (x, y, c, opacity); xids: per pass per wave.
  * hist: draw into it (NULL: no drawing). Returns the number of draws. */
+/* Xaos flames: every walker picks its own xform from the row of its last
+ * one (lane stream: the pick is the pass's first draw), walkers keep their
+ * own point (no permutation); xids per pass per walker. */
+static u64 run_block_chaos(const flame_prog* fp, const float* P, const u32* pal, u32 block, u32 tpb, u32 seed,
+                           u32 stream, u32 pass0, u32 npasses, double* x, double* y, double* c, u32* last,
+                           double* trace, u32* xids, const float* gpu_trace, const u32* gpu_xids, u32 trace_stride,
+                           u32* hist, u32 width, u32 height) {
+    u64 draws = 0;
+    float S[6];
+    for (int k = 0; k < 6; k++) S[k] = P[k];
+    for (u32 i = 0; i < npasses; i++) {
+        u32 pass = pass0 + i;
+        for (u32 t = 0; t < tpb; t++) {
+            size_t g = (size_t)block * tpb + t;
+            if (gpu_trace && i > 0) {
+                const float* gt = gpu_trace + ((size_t)(i - 1) * trace_stride + g) * 4;
+                x[t] = gt[0];
+                y[t] = gt[1];
+                c[t] = gt[2];
+                last[t] = gpu_xids[(size_t)(i - 1) * trace_stride + g];
+            }
+            lane_rng rs;
+            rng_init(&rs, seed, stream, (u32)g, TAG_LANE);
+            rng_begin(&rs, pass);
+            u32 xid = select_xform_chaos(fp, P, last[t], (float)rand01(&rs));
+            if (xids) xids[(size_t)i * trace_stride + g] = xid;
+            double px = x[t], py = y[t], pz = c[t];
+            double op = apply_xform(fp, (int)xid, P, &px, &py, &pz, &rs);
+            if (badvalue((float)px) || badvalue((float)py)) {
+                px = rand01(&rs) * 2.0 - 1.0;
+                py = rand01(&rs) * 2.0 - 1.0;
+                op = 0.0;
+            }
+            if (hist) {
+                double dx = px, dy = py, dz = pz;
+                if (fp->has_final) apply_xform(fp, fp->n_xforms, P, &dx, &dy, &dz, &rs);
+                u32 pix, inc[4];
+                if (make_draw(S, pal, width, height, (float)dx, (float)dy, (float)dz, (float)op, &pix, inc)) {
+                    for (int k = 0; k < 4; k++) hist[4ull * pix + k] += inc[k];
+                    draws++;
+                }
+            }
+            if (trace) {
+                double* tr = trace + ((size_t)i * trace_stride + g) * 4;
+                tr[0] = px; tr[1] = py; tr[2] = pz; tr[3] = op;
+            }
+            x[t] = px; y[t] = py; c[t] = pz;
+            last[t] = xid;
+        }
+    }
+    return draws;
+}
+
 static u64 run_block(const flame_prog* fp, const float* P, const u32* pal, u32 block, u32 tpb, u32 tpb_bits,
                      u32 seed, u32 stream, u32 pass0, u32 npasses, double* x, double* y, double* c, double* trace,
                      u32* xids, const float* gpu_trace, u32 trace_stride, u32* hist, u32 width, u32 height,
@@ -467,13 +545,22 @@ static u32 bits_of(u32 tpb) {
 }
 
 /* Trace `npasses` passes for every block (see run_block). x/y/c: all walkers. */
-int orc_trace(const int* prog, int n_xforms, int has_final, const float* params, u32 stride, u32 blocks, u32 tpb,
-              u32 seed, u32 stream, u32 pass0, u32 npasses, double* x, double* y, double* c, double* trace, u32* xids,
-              const float* gpu_trace) {
+int orc_trace(const int* prog, int n_xforms, int has_final, int chaos_base, const float* params, u32 stride,
+              u32 blocks, u32 tpb, u32 seed, u32 stream, u32 pass0, u32 npasses, double* x, double* y, double* c,
+              u32* last, double* trace, u32* xids, const float* gpu_trace, const u32* gpu_xids) {
     flame_prog fp;
-    prog_index(&fp, prog, n_xforms, has_final);
+    prog_index(&fp, prog, n_xforms, has_final, chaos_base);
     u32 bits = bits_of(tpb);
     int ok = 1;
+    if (chaos_base >= 0) {
+        if (!last) return -1;
+#pragma omp parallel for schedule(dynamic)
+        for (long b = 0; b < (long)blocks; b++)
+            run_block_chaos(&fp, params + (size_t)b * stride, NULL, (u32)b, tpb, seed, stream, pass0, npasses,
+                            x + (size_t)b * tpb, y + (size_t)b * tpb, c + (size_t)b * tpb, last + (size_t)b * tpb,
+                            trace, xids, gpu_trace, gpu_xids, blocks * tpb, NULL, 0, 0);
+        return 0;
+    }
 #pragma omp parallel for schedule(dynamic)
     for (long b = 0; b < (long)blocks; b++) {
         double* ox = (double*)malloc(sizeof(double) * tpb * 3);
@@ -516,11 +603,11 @@ void orc_seed(const float* params, u32 stride, u32 blocks, u32 tpb, u32 width, u
 
 /* Full render on host cores: seed, fuse passes, then `passes` drawn passes,
  * per-thread private histograms summed at the end. Returns total draws. */
-u64 orc_render(const int* prog, int n_xforms, int has_final, const float* params, u32 stride, const u32* palettes,
-               u32 blocks, u32 tpb, u32 width, u32 height, u32 seed, u32 stream, u32 fuse, u32 passes, u32* hist,
-               int threads) {
+u64 orc_render(const int* prog, int n_xforms, int has_final, int chaos_base, const float* params, u32 stride,
+               const u32* palettes, u32 blocks, u32 tpb, u32 width, u32 height, u32 seed, u32 stream, u32 fuse,
+               u32 passes, u32* hist, int threads) {
     flame_prog fp;
-    prog_index(&fp, prog, n_xforms, has_final);
+    prog_index(&fp, prog, n_xforms, has_final, chaos_base);
     u32 bits = bits_of(tpb);
     size_t total = (size_t)blocks * tpb, npx = (size_t)width * height;
     double* x = (double*)malloc(sizeof(double) * total);
@@ -540,6 +627,16 @@ u64 orc_render(const int* prog, int n_xforms, int has_final, const float* params
         for (long b = 0; b < (long)blocks; b++) {
             const float* P = params + (size_t)b * stride;
             double *bx = x + (size_t)b * tpb, *by = y + (size_t)b * tpb, *bc = c + (size_t)b * tpb;
+            if (chaos_base >= 0) {
+                u32* last = (u32*)malloc(sizeof(u32) * tpb);
+                for (u32 t = 0; t < tpb; t++) last[t] = 0xFFFFFFFFu;
+                run_block_chaos(&fp, P, NULL, (u32)b, tpb, seed, stream, 0, fuse, bx, by, bc, last, NULL, NULL, NULL,
+                                NULL, 0, NULL, 0, 0);
+                draws += run_block_chaos(&fp, P, palettes + (size_t)b * 256, (u32)b, tpb, seed, stream, fuse, passes,
+                                         bx, by, bc, last, NULL, NULL, NULL, NULL, 0, local, width, height);
+                free(last);
+                continue;
+            }
             run_block(&fp, P, NULL, (u32)b, tpb, bits, seed, stream, 0, fuse, bx, by, bc, NULL, NULL, NULL, 0, NULL,
                       0, 0, ox, ox + tpb, ox + 2 * tpb);
             draws += run_block(&fp, P, palettes + (size_t)b * 256, (u32)b, tpb, bits, seed, stream, fuse, passes, bx,

@@ -432,13 +432,13 @@ def clib():
         P = C.c_void_p
         lib.orc_philox.argtypes = [P, P, P]
         lib.orc_bin_points.argtypes = [P, P, P, C.c_size_t, P, C.c_uint32, C.c_uint32]
-        lib.orc_trace.argtypes = [P, C.c_int, C.c_int, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                                  C.c_uint32, C.c_uint32, P, P, P, P, P, P]
+        lib.orc_trace.argtypes = [P, C.c_int, C.c_int, C.c_int, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P, P, P, P, P]
         lib.orc_trace.restype = C.c_int
         lib.orc_seed.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                  C.c_uint32, P, P, P]
-        lib.orc_render.argtypes = [P, C.c_int, C.c_int, P, C.c_uint32, P, C.c_uint32, C.c_uint32, C.c_uint32,
-                                   C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P, C.c_int]
+        lib.orc_render.argtypes = [P, C.c_int, C.c_int, C.c_int, P, C.c_uint32, P, C.c_uint32, C.c_uint32,
+                                   C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P, C.c_int]
         lib.orc_render.restype = C.c_uint64
         _clib = lib
     return _clib
@@ -473,17 +473,24 @@ def seed_walkers(params32, stride, blocks, tpb, width, height, seed, stream=0):
     return x, y, c
 
 
-def trace(L: Layout, params32, stride, blocks, tpb, seed, pass0, npasses, x, y, c, gpu_trace=None, stream=0):
+def trace(L: Layout, params32, stride, blocks, tpb, seed, pass0, npasses, x, y, c, gpu_trace=None, stream=0,
+          last=None, gpu_xids=None):
+    """per-pass walker states; xids per wave, or per walker for xaos flames
+    (`last`: each walker's previous xform, 0xFFFFFFFF for none)"""
     prog = L.program()
     n = blocks * tpb
     x, y, c = (np.ascontiguousarray(v, dtype=np.float64).copy() for v in (x, y, c))
     tr = np.zeros((npasses, n, 4))
-    xids = np.zeros((npasses, n // 64), dtype=np.uint32)
+    xids = np.zeros((npasses, n if L.chaos else n // 64), dtype=np.uint32)
     p = np.ascontiguousarray(params32, dtype=np.float32)
     g = None if gpu_trace is None else np.ascontiguousarray(gpu_trace, dtype=np.float32)
-    rc = clib().orc_trace(_ptr(prog), L.n_xforms, int(L.has_final), _ptr(p), stride, blocks, tpb, seed, stream, pass0,
-                          npasses, _ptr(x), _ptr(y), _ptr(c), _ptr(tr), _ptr(xids),
-                          None if g is None else _ptr(g))
+    gx = None if gpu_xids is None else np.ascontiguousarray(gpu_xids, dtype=np.uint32)
+    lst = None
+    if L.chaos:
+        lst = np.full(n, 0xFFFFFFFF, np.uint32) if last is None else np.ascontiguousarray(last, np.uint32).copy()
+    rc = clib().orc_trace(_ptr(prog), L.n_xforms, int(L.has_final), L.chaos_base, _ptr(p), stride, blocks, tpb, seed,
+                          stream, pass0, npasses, _ptr(x), _ptr(y), _ptr(c), None if lst is None else _ptr(lst),
+                          _ptr(tr), _ptr(xids), None if g is None else _ptr(g), None if gx is None else _ptr(gx))
     assert rc == 0
     return tr, xids
 
@@ -494,8 +501,8 @@ def render(L: Layout, params32, stride, palettes, blocks, tpb, width, height, se
     hist = np.zeros((height, width, 4), dtype=np.uint32)
     p = np.ascontiguousarray(params32, dtype=np.float32)
     pal = np.ascontiguousarray(palettes, dtype=np.uint32)
-    draws = clib().orc_render(_ptr(prog), L.n_xforms, int(L.has_final), _ptr(p), stride, _ptr(pal), blocks, tpb,
-                              width, height, seed, stream, fuse, passes, _ptr(hist), threads)
+    draws = clib().orc_render(_ptr(prog), L.n_xforms, int(L.has_final), L.chaos_base, _ptr(p), stride, _ptr(pal),
+                              blocks, tpb, width, height, seed, stream, fuse, passes, _ptr(hist), threads)
     return hist, int(draws)
 
 

@@ -48,7 +48,8 @@ class StateInfo(C.Structure):
                 ("param_stride", C.c_uint32), ("seed", C.c_uint32), ("passes_done", C.c_uint64),
                 ("histogram", C.c_void_p), ("params", C.c_void_p), ("palettes", C.c_void_p),
                 ("walker_x", C.c_void_p), ("walker_y", C.c_void_p), ("walker_c", C.c_void_p),
-                ("stream", C.c_void_p), ("quality", C.c_double), ("hist_bits", C.c_uint32)]
+                ("stream", C.c_void_p), ("quality", C.c_double), ("hist_bits", C.c_uint32),
+                ("walker_last", C.c_void_p)]
 
 
 class BatchFrameInfo(C.Structure):

@@ -763,6 +763,7 @@ int rfk_state_get_info(const rfk_state* s, rfk_state_info* out) {
     out->stream = s->stream;
     out->quality = s->quality;
     out->hist_bits = s->hist64 ? 64u : 32u;
+    out->walker_last = s->wlast;
     return RFK_OK;
 }
 

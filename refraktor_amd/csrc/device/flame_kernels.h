@@ -69,6 +69,7 @@ struct rfk_shared {
     u32 pal[256];
     Real pts[2][3][RFK_TPB];
     u64 hits, draws, bad, start;
+    u64 published;  // hits already added to the call's counter (queue binning)
     u32 stop, item;
 #if RFK_BIN_TILE_LOG2
     u32 tkey[RFK_TILE];
@@ -153,6 +154,7 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, const Real* __r
     for (u32 i = t; i < 256u; i += RFK_TPB) sm.pal[i] = pal[i];
     if (t == 0) {
         sm.hits = 0;
+        sm.published = 0;
         sm.draws = 0;
         sm.bad = 0;
         sm.stop = 0;
@@ -327,9 +329,11 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, const Real* __r
                 const u64 origin = __hip_atomic_load(&a.counters[RFK_CNT_ORIGIN], __ATOMIC_RELAXED,
                                                      __HIP_MEMORY_SCOPE_AGENT);
                 if (a.time_limit_ticks && wall_clock64() - origin >= a.time_limit_ticks) stop = true;
-                if (a.quality_target && __hip_atomic_load(&a.counters[RFK_CNT_HITS], __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_AGENT) + sm.hits >= a.quality_target)
-                    stop = true;
+                // publish this block's new hits and compare the call's running total
+                const u64 delta = sm.hits - sm.published;
+                sm.published = sm.hits;
+                const u64 total = atomicAdd(&a.counters[RFK_CNT_HITS], delta) + delta;
+                if (a.quality_target && total >= a.quality_target) stop = true;
                 if (__hip_atomic_load(&a.counters[RFK_CNT_STOPPED], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull)
                     stop = true;
 #else
@@ -390,7 +394,7 @@ extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate(rfk_iter_args 
                                a.total_walkers, pass0, 0u, a.npasses, a.flags, a.wx, a.wy, a.wc, a.wlast);
     if (threadIdx.x == 0) {
         a.block_pass[block] = pass0 + done;
-        atomicAdd(&a.counters[RFK_CNT_HITS], sm.hits);
+        atomicAdd(&a.counters[RFK_CNT_HITS], sm.hits - sm.published);
         atomicAdd(&a.counters[RFK_CNT_PASSES], (u64)done * RFK_TPB);
         atomicMin(&a.counters[RFK_CNT_START], sm.start);
         atomicMax(&a.counters[RFK_CNT_STOP], wall_clock64());

@@ -172,6 +172,23 @@ def flame_keyframe(seed: int) -> dict:
     return make_flame(xfs, random_palette(rng), vibrancy=1.0)
 
 
+def flame_xaos(seed: int) -> dict:
+    """5 xforms of 1-2 variations with a xaos matrix: entries U[0,2], a third
+    of them zero, no all-zero row"""
+    rng = SplitMix64(seed)
+    n = 5
+    xfs = _xforms(rng, n, _multi_variation(2, 1))
+    chaos = []
+    for i in range(n):
+        row = [0.0 if rng.uniform() < 1.0 / 3.0 else 2.0 * rng.uniform() for _ in range(n)]
+        if not any(row):
+            row[i] = 1.0
+        chaos.append(row)
+    f = make_flame(xfs, random_palette(rng), vibrancy=1.0)
+    f["chaos"] = chaos
+    return f
+
+
 @dataclass
 class Workload:
     name: str

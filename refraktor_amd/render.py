@@ -289,6 +289,13 @@ class State:
                          download(inf.walker_y, np.float32, (inf.walkers,)),
                          download(inf.walker_c, np.float32, (inf.walkers,))], axis=1)
 
+    def walkers_last(self) -> np.ndarray:
+        """each walker's last xform (xaos flames; 0xFFFFFFFF before the first pass)"""
+        inf = self.info()
+        if not inf.walker_last:
+            raise ValueError("not a xaos flame")
+        return download(inf.walker_last, np.uint32, (inf.walkers,))
+
     def trace(self, passes: int) -> tuple[np.ndarray, np.ndarray]:
         inf = self.info()
         tr = DeviceBuffer(passes * inf.walkers * 16)

@@ -32,9 +32,9 @@ def training_flame(seed: int) -> dict:
     return fl.flame_keyframe(seed) if seed % 2 == 0 else fl.flame_1080(seed)
 
 
-# launch shape of the renders: 1024 walkers per block, LDS bin cache, 64-bit
+# launch shape of the renders: 1024 walkers per block, queue binning, 64-bit
 # bins so the converged targets cannot wrap
-SHAPE = dict(threads_per_block=1024, atomic_width=64, bin_tile_log2=12, hist_bits=64)
+SHAPE = dict(threads_per_block=1024, atomic_width=64, bin_queue=True, hist_bits=64)
 
 
 def compile_all(db: render.VariationDB, flames: list[dict], threads: int = 16) -> list:

@@ -189,3 +189,120 @@ def test_density_histogram_matches_oracle(small):
     bins, mx = st.density_histogram()
     ob, omx = oracle.density_histogram(st.histogram())
     assert np.array_equal(bins, ob) and mx == omx
+
+
+def test_xaos_trajectory_one_step(db, gpu):
+    """per-walker xaos picks (waterfall over the wave's xforms) vs the oracle"""
+    f = fl.flame_xaos(2)
+    k = db.compile(f)
+    st = k.warmup(k.pack_samples(W, H), W, H, seed=9, fuse=3, walkers=4096)
+    inf = st.info()
+    x0, last0 = st.walkers(), st.walkers_last()
+    assert (last0 < 5).all()  # the fuse passes picked an xform for everyone
+    tr, xid = st.trace(16)
+    assert xid.shape == (16, inf.walkers)
+    otr, oxid = oracle.trace(_layout(f), st.params(), inf.param_stride, inf.blocks, inf.threads_per_block, 9, 3, 16,
+                             x0[:, 0], x0[:, 1], x0[:, 2], gpu_trace=tr, last=last0, gpu_xids=xid)
+    assert np.array_equal(xid, oxid)
+    # rows with zero entries are never taken: xform j after i only if chaos[i][j] != 0
+    ch = np.array(f["chaos"])
+    prev = np.concatenate([last0[None, :], xid[:-1]])
+    assert (ch[prev, xid] != 0).all()
+    err = np.abs(tr[..., :3] - otr[..., :3]) / np.maximum(1.0, np.abs(otr[..., :3]))
+    assert np.mean(err < 1e-4) > 0.999, np.quantile(err, [0.5, 0.99, 0.999])
+
+
+def test_f64_trajectory_one_step(db, gpu):
+    """double-precision kernel: the per-step error bound tightens by orders of magnitude"""
+    f = fl.flame_1080(5)
+    k = db.compile(f, precision=1)
+    st = k.warmup(k.pack_samples(W, H), W, H, seed=4, fuse=3, walkers=2048)
+    inf = st.info()
+    x0 = st.walkers()
+    tr, xid = st.trace(8)
+    P = st.params().astype(np.float32)
+    otr, oxid = oracle.trace(_layout(f), P, inf.param_stride, inf.blocks, inf.threads_per_block, 4, 3, 8,
+                             x0[:, 0], x0[:, 1], x0[:, 2], gpu_trace=tr)
+    # the oracle evaluates in double from float32 parameters; the f64 kernel
+    # from double parameters: compare the xform choices and a loose bound
+    assert np.mean(xid == oxid) > 0.99
+    err = np.abs(tr[..., :2] - otr[..., :2]) / np.maximum(1.0, np.abs(otr[..., :2]))
+    assert np.median(err) < 1e-5
+
+
+def test_animated_warmup_params_match_oracle(db, gpu):
+    """Catmull-Rom over four different keyframe samples, per-block times and polar affines"""
+    kf = [fl.flame_keyframe(40 + i) for i in range(2)]
+    for k_ in kf[1:]:
+        k_["palette"] = kf[0]["palette"]
+    aligned = db.align(kf)
+    k = db.compile(aligned[0])
+    packed = [a.pack_samples(W, H, 0.0, 0.0, 1) for a in aligned]
+    samples = np.concatenate([packed[0], packed[0], packed[1], packed[1]])
+    st = k.warmup(samples, W, H, seed=6, fuse=0, walkers=64 * 256)
+    inf = st.info()
+    L = oracle.Layout(aligned[0].to_json(), oracle.load_variation_params(render.CONFIG_DIR))
+    ss = L.size_reals + 768
+    fixed = oracle.fix_rotation(samples.copy(), ss, L.size_reals, L.affine_indices, 4)
+    seg = oracle.catmull(fixed, ss)
+    P, pals = oracle.eval_samples(seg, L, inf.blocks)
+    P = oracle.precalc(P, L, 6, 0)
+    n = L.size_reals
+    np.testing.assert_allclose(st.params()[:, :n], P[:, :n], rtol=2e-6, atol=2e-5)
+    assert np.array_equal(st.palettes(), pals)
+    # the weights really move between the two keyframes
+    w0 = L.xforms[0]["weight"]
+    assert abs(float(P[0, w0]) - float(P[-1, w0])) > 1e-3
+
+
+def test_density_estimate_matches_oracle(small):
+    f, k = small
+    st = k.warmup(k.pack_samples(96, 64), 96, 64, seed=3, fuse=10, walkers=4096)
+    st.bin(iters=4096 * 16)
+    h = st.histogram()
+    for max_r, min_r, curve in ((0.0, 0.0, 0.4), (3.0, 0.0, 0.4), (6.0, 1.0, 0.6)):
+        got = st.density_estimate(max_r, min_r, curve).astype(np.float64)
+        want = oracle.density_estimate(h, max_r, min_r, curve)
+        np.testing.assert_allclose(got, want, rtol=2e-4, atol=1e-4 * want.max())
+
+
+def test_bailouts(small):
+    f, k = small
+    s = k.pack_samples(W, H)
+    # exact iteration budget, rounded up to whole passes of all walkers
+    st = k.warmup(s, W, H, seed=1, fuse=5, walkers=8192)
+    r = st.bin(iters=8192 * 10 + 1)
+    assert r["total_passes"] == 8192 * 11
+    # quality target: every block stops at its share of the target density
+    st = k.warmup(s, W, H, seed=1, fuse=5, walkers=8192)
+    r = st.bin(quality=4.0)
+    assert r["quality"] >= 4.0 * 0.99 and r["quality"] < 8.0
+    # time limit on the device clock
+    st = k.warmup(s, W, H, seed=1, fuse=5, walkers=8192)
+    r = st.bin(millis=3.0)
+    assert r["total_passes"] > 0 and r["elapsed_ms"] < 100.0
+    # abort from the host while the kernel runs (no other bailout: it would run ~forever)
+    st = k.warmup(s, W, H, seed=1, fuse=5, walkers=8192)
+    st.bin_async(millis=60000.0)
+    import time
+    time.sleep(0.05)
+    st.abort()
+    r = st.bin_wait()
+    assert r["total_passes"] > 0 and r["kernel_ms"] < 30000.0
+
+
+def test_temporal_multiplier_blocks(db, gpu):
+    """tm blocks per base block, each its own temporal sample along the segment"""
+    kf = [fl.flame_keyframe(60), fl.flame_keyframe(61)]
+    kf[1]["palette"] = kf[0]["palette"]
+    aligned = db.align(kf)
+    k = db.compile(aligned[0])
+    packed = [a.pack_samples(W, H, 0.0, 0.0, 1) for a in aligned]
+    samples = np.concatenate([packed[0], packed[0], packed[1], packed[1]])
+    st = k.warmup(samples, W, H, seed=2, fuse=5, walkers=4096, temporal_multiplier=4)
+    inf = st.info()
+    assert inf.blocks == 4 * 4096 // inf.threads_per_block
+    P = st.params()
+    assert len({tuple(np.round(r[:12], 5)) for r in P}) > 1  # rows differ along the segment
+    r = st.bin(iters=inf.walkers * 8)
+    assert r["total_passes"] == inf.walkers * 8

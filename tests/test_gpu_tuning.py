@@ -38,9 +38,12 @@ def test_atomic_width_and_bin_tiles_bit_exact(gpu, db, name):
 def test_queue_binning_bit_exact(gpu, db, name):
     """records sorted by screen tile and summed in LDS give the atomic histogram"""
     flm = fl.flame_small(3) if name == "small" else fl.flame_1080(3)
-    rst, rres = _render(db, flm, 256, 32, 0)
-    ref = rst.histogram().astype(np.uint64)
+    refs = {}
+    for tpb in (256, 1024):  # walkers per block change the streams: one reference each
+        rst, rres = _render(db, flm, tpb, 32, 0)
+        refs[tpb] = (rst.histogram().astype(np.uint64), rres)
     for aw, hb, tpb in ((32, 32, 256), (64, 32, 256), (32, 64, 1024)):
+        ref, rres = refs[tpb]
         st, res = _render(db, flm, tpb, aw, 0, hist_bits=hb, queue=True)
         assert res["total_draws"] == rres["total_draws"] and res["total_passes"] == rres["total_passes"]
         np.testing.assert_array_equal(st.histogram().astype(np.uint64), ref, err_msg=f"queue aw {aw} bins {hb}")
