@@ -98,8 +98,10 @@ def main() -> int:
         out = images.get(fmt)
         if out is None:
             out = images[fmt] = render.DeviceBuffer(npx * _lib.PIXEL_BYTES[fmt])
-        ta = _lib.TonemapArgs(st.info().quality, gamma, bright, vib, 0.0)
-        check(lib.rfk_tonemap_image(de.ptr, W, H, out.ptr, fmt, C.byref(ta), None))
+        inf = st.info()
+        ta = _lib.TonemapArgs(inf.quality, gamma, bright, vib, 0.0)
+        # everything of a frame on the state's stream: the next re-warm waits for it
+        check(lib.rfk_tonemap_image(de.ptr, W, H, out.ptr, fmt, C.byref(ta), inf.stream))
         return st, res, out
 
     weights = Path(args.weights) if args.weights else denoise.DEFAULT_WEIGHTS
@@ -111,8 +113,8 @@ def main() -> int:
     def denoised(seed: int):
         st, res, tm = plain(seed, k, args.low_iters, _lib.PIXEL_FLOAT4)
         inf = st.info()
-        check(lib.rfk_denoiser_features(inf.histogram, inf.hist_bits, tm.ptr, npx, feat.ptr, base.ptr, None))
-        den.forward(feat, base, 1, H, W, rgb)
+        check(lib.rfk_denoiser_features(inf.histogram, inf.hist_bits, tm.ptr, npx, feat.ptr, base.ptr, inf.stream))
+        den.forward(feat, base, 1, H, W, rgb, stream=inf.stream)
         return st, res, rgb
 
     def timed(fn, steps: int, warm: int):

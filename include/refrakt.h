@@ -274,6 +274,11 @@ typedef struct rfk_tonemap_args {
 /* Tonemap the state's fixed-point histogram. */
 int rfk_tonemap(rfk_state* s, void* out /* device */, int pixel_format, const rfk_tonemap_args* a);
 /* Tonemap a float4 RGBA image in draw units (e.g. the density-estimated one). */
+/* density_hdr (assets/kernels/tonemap.cu:159-233): remap a
+ * tonemapped float4 / half4 image (device) by the CDF of the render's
+ * 97-bin density histogram (host, from rfk_density_histogram) into float4. */
+int rfk_density_hdr(const void* tonemapped, int pixel_format, size_t npix, const uint64_t* bins97 /* host */,
+                    float* out_rgba, void* stream);
 int rfk_tonemap_image(const float* in_rgba, uint32_t width, uint32_t height, void* out, int pixel_format,
                       const rfk_tonemap_args* a, void* stream);
 

@@ -522,3 +522,39 @@ def prepare(flame: dict, config_dir, width: int, height: int, blocks: int, seed:
     P32 = np.zeros((blocks, stride), dtype=np.float32)
     P32[:, :L.size_reals] = P
     return L, P32, pals
+
+
+def density_hdr(img: np.ndarray, bins97) -> np.ndarray:
+    """CDF remap of a tonemapped RGBA image (assets/kernels/tonemap.cu:159-233):
+    v *= (cdf(alpha * 96 + 1) / 0.5)^2 in HSV, alpha -> 1; the upper bin is
+    clamped to the last one (the reference reads one past it at alpha = 1)"""
+    img = np.asarray(img, dtype=np.float64)
+    b = np.asarray(bins97, dtype=np.float64)
+    cdf = (np.cumsum(b) / b.sum() if b.sum() > 0 else np.zeros(97)).astype(np.float32).astype(np.float64)
+    power = img[..., 3] * 96.0 + 1.0
+    lo = np.minimum(np.floor(power).astype(np.int64), 96)
+    hi = np.minimum(lo + 1, 96)
+    frac = power - np.floor(power)
+    v = cdf[lo] + frac * (cdf[hi] - cdf[lo])
+    r, g, bb = img[..., 0], img[..., 1], img[..., 2]
+    mx = np.maximum(r, np.maximum(g, bb))
+    mn = np.minimum(r, np.minimum(g, bb))
+    d = mx - mn
+    with np.errstate(divide="ignore", invalid="ignore"):
+        h = np.where(mx == r, np.mod((g - bb) / d, 6.0), np.where(mx == g, (bb - r) / d + 2.0, (r - g) / d + 4.0))
+        s = np.where(mx > 0, d / mx, 0.0)
+    h = np.where(d > 0, h, 0.0)
+    h = np.where(h < 0, h + 6.0, h)
+    val = mx * (v / 0.5) ** 2
+    i = np.floor(h).astype(np.int64) % 6
+    f = h - np.floor(h)
+    p, q, t = val * (1 - s), val * (1 - s * f), val * (1 - s * (1 - f))
+    out = np.zeros(img.shape[:-1] + (4,))
+    choices = [(val, t, p), (q, val, p), (p, val, t), (p, q, val), (t, p, val), (val, p, q)]
+    for k, (cr, cg, cb) in enumerate(choices):
+        m = i == k
+        out[..., 0] = np.where(m, cr, out[..., 0])
+        out[..., 1] = np.where(m, cg, out[..., 1])
+        out[..., 2] = np.where(m, cb, out[..., 2])
+    out[..., 3] = 1.0
+    return out

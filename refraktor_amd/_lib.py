@@ -126,6 +126,7 @@ _SIGS = [
     ("rfk_density_estimate", C.c_int, [_P, C.c_float, C.c_float, C.c_float, _P]),
     ("rfk_tonemap", C.c_int, [_P, _P, C.c_int, C.POINTER(TonemapArgs)]),
     ("rfk_tonemap_image", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_int, C.POINTER(TonemapArgs), _P]),
+    ("rfk_density_hdr", C.c_int, [_P, C.c_int, C.c_size_t, C.POINTER(C.c_uint64), _P, _P]),
     ("rfk_catmull_coefficients", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     ("rfk_fix_rotation", C.c_int, [C.POINTER(C.c_double), C.c_size_t, C.c_size_t, C.POINTER(C.c_size_t),
                                    C.c_size_t, C.c_int]),

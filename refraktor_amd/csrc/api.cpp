@@ -75,6 +75,9 @@ struct rfk_state {
     double *d_samples = nullptr, *d_segments = nullptr;
     int8_t* d_kind = nullptr;
     uint32_t scratch_samples = 0;
+    double* staging = nullptr;  // pinned host copy of the samples being uploaded
+    size_t staging_reals = 0;
+    hipEvent_t ev_staging = nullptr;
     // queue binning scratch (kernels compiled with bin_queue)
     unsigned long long *queue = nullptr, *sorted = nullptr;
     uint32_t *qcounts = nullptr, *qnseg = nullptr;
@@ -148,6 +151,9 @@ void free_state_buffers(rfk_state* s) {
     f(s->params); f(s->palettes); f(s->wx); f(s->wy); f(s->wc); f(s->wlast); f(s->block_pass); f(s->hist);
     f(s->queue); f(s->sorted); f(s->qcounts); f(s->qnseg); f(s->qsegments);
     f(s->d_samples); f(s->d_segments); f(s->d_kind);
+    if (s->ev_staging) (void)hipEventSynchronize(s->ev_staging);
+    if (s->staging) (void)hipHostFree(s->staging);
+    if (s->ev_staging) (void)hipEventDestroy(s->ev_staging);
     f(s->counters); f(s->dstats);
     if (s->counters_host) (void)hipHostFree(s->counters_host);
     if (s->stop_host) (void)hipHostFree(s->stop_host);
@@ -211,10 +217,12 @@ int bin_queued(rfk_state* s, rfk_iter_args& a, uint64_t passes, bool bounded) {
     return RFK_OK;
 }
 
+// zero the counters, the running minimum start clock at ~0 (fill kernels: no
+// host buffer is read after this returns)
 int reset_counters(rfk_state* s) {
-    unsigned long long init[8] = {0, 0, 0, 0, ~0ull, 0, 0, 0};
-    std::memcpy(s->counters_host, init, sizeof init);
-    RFK_HIP(hipMemcpyAsync(s->counters, s->counters_host, sizeof init, hipMemcpyHostToDevice, s->stream));
+    uint32_t* c = reinterpret_cast<uint32_t*>(s->counters);
+    RFK_HIP(rfk::sk::fill_u32(c, 0u, 16, s->stream));
+    RFK_HIP(rfk::sk::fill_u32(c + 2 * RFK_CNT_START, 0xFFFFFFFFu, 2, s->stream));
     return RFK_OK;
 }
 
@@ -502,10 +510,24 @@ static int state_prepare(rfk_kernel* k, rfk_state* s, const double* samples, siz
         RFK_HIP(hipMalloc(&s->d_segments, size_t(nseg) * sample_size * 4 * sizeof(double)));
         s->scratch_samples = nsamp;
     }
-    std::vector<double> host(samples, samples + n_reals);
-    rfk::fix_rotation(host.data(), sample_size, size_t(k->L.size_reals), k->L.affine_indices.data(),
+    // pinned staging copy of the samples; the previous upload from it must be done
+    if (s->staging_reals < n_reals) {
+        if (s->staging) {
+            (void)hipEventSynchronize(s->ev_staging);
+            (void)hipHostFree(s->staging);
+        }
+        s->staging = nullptr;
+        s->staging_reals = 0;
+        RFK_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->staging), n_reals * sizeof(double), hipHostMallocDefault));
+        s->staging_reals = n_reals;
+    } else {
+        RFK_HIP(hipEventSynchronize(s->ev_staging));
+    }
+    std::memcpy(s->staging, samples, n_reals * sizeof(double));
+    rfk::fix_rotation(s->staging, sample_size, size_t(k->L.size_reals), k->L.affine_indices.data(),
                       k->L.affine_indices.size(), int(nsamp));
-    RFK_HIP(hipMemcpyAsync(s->d_samples, host.data(), n_reals * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    RFK_HIP(hipMemcpyAsync(s->d_samples, s->staging, n_reals * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    RFK_HIP(hipEventRecord(s->ev_staging, s->stream));
     RFK_HIP(rfk::sk::catmull(s->d_samples, uint32_t(sample_size), nseg, s->d_segments, s->stream));
     RFK_HIP(rfk::sk::eval_samples(s->d_segments, uint32_t(sample_size), s->size_reals, nseg, s->blocks, s->d_kind,
                                   s->precision, s->params, s->param_stride, s->palettes, s->stream));
@@ -520,8 +542,6 @@ static int state_prepare(rfk_kernel* k, rfk_state* s, const double* samples, siz
     a.flags = RFK_FLAG_INIT;
     a.npasses = fuse;
     if (int rc = launch_iterate(s, a)) return rc;
-    // the host copy of the samples must outlive the upload
-    RFK_HIP(hipStreamSynchronize(s->stream));
     s->passes_done = fuse;
     s->quality = 0.0;
     return RFK_OK;
@@ -615,6 +635,7 @@ int rfk_warmup_ex(rfk_kernel* k, void* stream, const double* samples, size_t n_r
     RFK_HIPC(hipHostGetDevicePointer(reinterpret_cast<void**>(&s->stop_dev), s->stop_host, 0));
     RFK_HIPC(hipEventCreate(&s->ev0));
     RFK_HIPC(hipEventCreate(&s->ev1));
+    RFK_HIPC(hipEventCreateWithFlags(&s->ev_staging, hipEventDisableTiming));
     std::vector<int8_t> kind(size_t(k->L.size_reals), 0);
     for (size_t ai : k->L.affine_indices)
         for (int c = 0; c < 4; c++) kind[ai + size_t(c)] = int8_t(1 + c);
@@ -1165,6 +1186,23 @@ int rfk_tonemap_image(const float* in_rgba, uint32_t width, uint32_t height, voi
                           static_cast<hipStream_t>(stream));
 }
 
+int rfk_density_hdr(const void* tonemapped, int pixel_format, size_t npix, const uint64_t* bins97, float* out_rgba,
+                    void* stream) {
+    if (!tonemapped || !bins97 || !out_rgba) return fail(RFK_ERR_INVALID, "null argument");
+    if (pixel_format != RFK_PIXEL_FLOAT4 && pixel_format != RFK_PIXEL_HALF4)
+        return fail(RFK_ERR_INVALID, "density_hdr reads float4 or half4 images");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    unsigned long long* d_bins = nullptr;
+    RFK_HIP(hipMalloc(&d_bins, 97 * sizeof(unsigned long long)));
+    hipError_t e = hipMemcpyAsync(d_bins, bins97, 97 * sizeof(unsigned long long), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = rfk::sk::density_hdr(tonemapped, pixel_format == RFK_PIXEL_HALF4, npix, d_bins, out_rgba, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_bins);
+    if (e != hipSuccess) return fail(RFK_ERR_HIP, std::string("density_hdr: ") + hipGetErrorString(e));
+    return RFK_OK;
+}
+
 int rfk_catmull_coefficients(const double* samples, uint32_t sample_size, uint32_t nseg, double* segments,
                              void* stream) {
     if (!samples || !segments) return fail(RFK_ERR_INVALID, "null argument");
@@ -1218,19 +1256,25 @@ int rfk_free(void* p) {
     RFK_HIP(hipFree(p));
     return RFK_OK;
 }
+// The helpers wait for all device work first: states run on their own
+// non-blocking streams, which the null stream does not order against.
 int rfk_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    RFK_HIP(hipDeviceSynchronize());
     RFK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return RFK_OK;
 }
 int rfk_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    RFK_HIP(hipDeviceSynchronize());
     RFK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return RFK_OK;
 }
 int rfk_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+    RFK_HIP(hipDeviceSynchronize());
     RFK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return RFK_OK;
 }
 int rfk_memset(void* dst, int value, size_t bytes) {
+    RFK_HIP(hipDeviceSynchronize());
     RFK_HIP(hipMemset(dst, value, bytes));
     return RFK_OK;
 }

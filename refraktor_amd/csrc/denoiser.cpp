@@ -130,9 +130,17 @@ int alloc_params(rfk_denoiser* d) {
     return RFK_OK;
 }
 
+// New parameters restart the optimiser: Adam's moments are zeroed and its step
+// count reset, so the bias correction of the first steps applies again (the
+// moments are not part of the weight file).
 int upload_weights(rfk_denoiser* d, const std::vector<float>& w, const std::vector<float>& b) {
     NN_HIP(hipMemcpy(d->w, w.data(), w.size() * 4, hipMemcpyHostToDevice));
     NN_HIP(hipMemcpy(d->b, b.data(), b.size() * 4, hipMemcpyHostToDevice));
+    NN_HIP(hipMemset(d->mw, 0, d->nw * 4));
+    NN_HIP(hipMemset(d->vw, 0, d->nw * 4));
+    NN_HIP(hipMemset(d->mb, 0, d->nb * 4));
+    NN_HIP(hipMemset(d->vb, 0, d->nb * 4));
+    d->step = 0;
     NN_HIP(rfk::nn::f32_to_bf16(d->w, d->wbf, d->nw, nullptr));
     NN_HIP(hipDeviceSynchronize());
     return RFK_OK;
@@ -389,7 +397,6 @@ int rfk_denoiser_load(rfk_denoiser* d, const char* path) {
     f.read(reinterpret_cast<char*>(w.data()), std::streamsize(w.size() * 4));
     f.read(reinterpret_cast<char*>(b.data()), std::streamsize(b.size() * 4));
     if (!f) return fail(RFK_ERR_PARSE, "truncated weight file");
-    d->step = int(hdr[7]);
     return upload_weights(d, w, b);
 }
 

@@ -293,6 +293,73 @@ __global__ void tonemap_kernel(const void* __restrict__ in, size_t npix, void* _
     }
 }
 
+// -------------------------------------------------------------- density HDR
+// CDF remap of a tonemapped image (tonemap.cu:159-233): the density
+// histogram's CDF at the pixel's log-density (alpha * 96 + 1, lerped between
+// bins, clamped to the last bin) scales the HSV value by (cdf / 0.5)^2;
+// alpha becomes 1.
+__device__ __forceinline__ float4 rgb_to_hsv_dev(float4 c) {
+    const float mx = fmaxf(c.x, fmaxf(c.y, c.z)), mn = fminf(c.x, fminf(c.y, c.z)), d = mx - mn;
+    float h = 0.0f;
+    if (d > 0.0f) {
+        if (mx == c.x) h = fmodf((c.y - c.z) / d, 6.0f);
+        else if (mx == c.y) h = (c.z - c.x) / d + 2.0f;
+        else h = (c.x - c.y) / d + 4.0f;
+        if (h < 0.0f) h += 6.0f;
+    }
+    return {h, mx > 0.0f ? d / mx : 0.0f, mx, c.w};  // h in sextants [0, 6)
+}
+
+__device__ __forceinline__ float4 hsv_to_rgb_dev(float4 hsv) {
+    const float h = hsv.x, s = hsv.y, v = hsv.z;
+    const int i = int(floorf(h)) % 6;
+    const float f = h - floorf(h);
+    const float p = v * (1.0f - s), q = v * (1.0f - s * f), t = v * (1.0f - s * (1.0f - f));
+    switch (i) {
+        case 0: return {v, t, p, hsv.w};
+        case 1: return {q, v, p, hsv.w};
+        case 2: return {p, v, t, hsv.w};
+        case 3: return {p, q, v, hsv.w};
+        case 4: return {t, p, v, hsv.w};
+        default: return {v, p, q, hsv.w};
+    }
+}
+
+template <bool Half>
+__global__ void density_hdr_kernel(const void* __restrict__ in, size_t npix, const unsigned long long* __restrict__ bins97,
+                                   float4* __restrict__ out) {
+    constexpr int kBins = 97;
+    __shared__ float cdf[kBins];
+    if (threadIdx.x == 0) {
+        double total = 0.0;
+        for (int i = 0; i < kBins; i++) total += double(bins97[i]);
+        double run = 0.0;
+        for (int i = 0; i < kBins; i++) {
+            run += double(bins97[i]);
+            cdf[i] = total > 0.0 ? float(run / total) : 0.0f;
+        }
+    }
+    __syncthreads();
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= npix) return;
+    float4 c;
+    if (Half) {
+        const half4_t h = reinterpret_cast<const half4_t*>(in)[g];
+        c = {__half2float(h.x), __half2float(h.y), __half2float(h.z), __half2float(h.w)};
+    } else {
+        c = reinterpret_cast<const float4*>(in)[g];
+    }
+    const float power = c.w * float(kBins - 1) + 1.0f;
+    const int lo = min(int(power), kBins - 1), hi = min(lo + 1, kBins - 1);
+    const float frac = power - floorf(power);
+    const float v = lerpf(cdf[lo], cdf[hi], frac);
+    float4 hsv = rgb_to_hsv_dev(c);
+    const float gain = v / 0.5f;
+    hsv.z *= gain * gain;
+    const float4 o = hsv_to_rgb_dev(hsv);
+    out[g] = {o.x, o.y, o.z, 1.0f};
+}
+
 // --------------------------------------------------------------- atomics
 __global__ void atomic_peak_kernel(uint32_t* __restrict__ buf, uint64_t nbins, uint32_t iters, uint32_t seed) {
     uint32_t h = (blockIdx.x * blockDim.x + threadIdx.x) * 0x9E3779B1u ^ seed;
@@ -416,6 +483,16 @@ hipError_t tonemap(const void* in, int in_kind, size_t npix, void* out, int form
         default: return hipErrorInvalidValue;
     }
 #undef RFK_TM
+    return hipGetLastError();
+}
+
+hipError_t density_hdr(const void* in, bool in_half, size_t npix, const unsigned long long* bins97, float* out,
+                       hipStream_t s) {
+    if (!npix) return hipSuccess;
+    if (in_half)
+        density_hdr_kernel<true><<<nblocks(npix), kBlock, 0, s>>>(in, npix, bins97, reinterpret_cast<float4*>(out));
+    else
+        density_hdr_kernel<false><<<nblocks(npix), kBlock, 0, s>>>(in, npix, bins97, reinterpret_cast<float4*>(out));
     return hipGetLastError();
 }
 

@@ -52,6 +52,11 @@ hipError_t queue_bin(const unsigned long long* records, size_t n, unsigned long 
                      uint32_t nparts, void* segments, uint32_t max_segments, uint32_t* nseg, uint32_t width,
                      uint32_t height, uint32_t* hist, bool hist64, bool packed64, hipStream_t s);
 
+// density_hdr (tonemap.cu:159-233) on a tonemapped float4 / half4 image,
+// with the 97-bin density histogram (device u64) of the same render.
+hipError_t density_hdr(const void* in, bool in_half, size_t npix, const unsigned long long* bins97, float* out,
+                       hipStream_t s);
+
 // Random RGBA atomic increments for the binning roofline.
 hipError_t atomic_peak(uint32_t* buf, uint64_t nbins, uint32_t grid, uint32_t block, uint32_t iters, uint32_t seed,
                        hipStream_t s);

@@ -66,8 +66,9 @@ class Denoiser:
         return w, b
 
     def forward(self, feat: render.DeviceBuffer, base: render.DeviceBuffer, n: int, h: int, w: int,
-                out: render.DeviceBuffer) -> None:
-        check(self._lib.rfk_denoiser_forward(self.handle, feat.ptr, base.ptr, n, h, w, out.ptr, None))
+                out: render.DeviceBuffer, stream=None) -> None:
+        """enqueued on `stream` (a state's stream keeps the frame's work in order)"""
+        check(self._lib.rfk_denoiser_forward(self.handle, feat.ptr, base.ptr, n, h, w, out.ptr, stream))
 
     def train_step(self, feat, base, target, n: int, h: int, w: int, lr: float) -> float:
         loss = C.c_float()

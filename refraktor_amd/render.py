@@ -398,3 +398,16 @@ def tonemap_image(img: np.ndarray, gamma: float, brightness: float, vibrancy: fl
     args = _lib.TonemapArgs(quality, gamma, brightness, vibrancy, 0.0)
     check(_lib.lib().rfk_tonemap_image(src.ptr, w, h, out.ptr, fmt, C.byref(args), None))
     return _host_pixels(out, fmt, h, w)
+
+
+def density_hdr(img: np.ndarray, bins97: np.ndarray) -> np.ndarray:
+    """density_hdr CDF remap of a host float4 tonemapped image on the device"""
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    h, w = img.shape[:2]
+    src = DeviceBuffer(img.nbytes)
+    src.upload(img)
+    out = DeviceBuffer(h * w * 16)
+    b = np.ascontiguousarray(bins97, dtype=np.uint64)
+    check(_lib.lib().rfk_density_hdr(src.ptr, _lib.PIXEL_FLOAT4, h * w, b.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                     out.ptr, None))
+    return out.download(np.float32, (h, w, 4))

@@ -14,7 +14,8 @@ import argparse
 import json
 import sys
 import time
-from concurrent.futures import ThreadPoolExecutor
+import multiprocessing as mp
+from concurrent.futures import ProcessPoolExecutor
 from pathlib import Path
 
 import numpy as np
@@ -37,9 +38,23 @@ def training_flame(seed: int) -> dict:
 SHAPE = dict(threads_per_block=1024, atomic_width=64, bin_queue=True, hist_bits=64)
 
 
-def compile_all(db: render.VariationDB, flames: list[dict], threads: int = 16) -> list:
-    with ThreadPoolExecutor(max_workers=threads) as ex:
-        return list(ex.map(lambda f: db.compile(f, **SHAPE), flames))
+def _compile_only(flame: dict) -> int:
+    """hiprtc compile into the code-object cache; runs in a worker process
+    that never touches the GPU"""
+    db = render.VariationDB()
+    s = {k: v for k, v in SHAPE.items() if k != "threads_per_block"}
+    return db.flame(flame).compile_check(0, SHAPE["threads_per_block"], **s)
+
+
+def compile_all(db: render.VariationDB, flames: list[dict], threads: int = 16, log=print) -> list:
+    """hiprtc in `threads` worker processes (it does not compile in parallel
+    inside one process), then load every kernel from the cache here"""
+    t0 = time.time()
+    with ProcessPoolExecutor(max_workers=threads, mp_context=mp.get_context("spawn")) as ex:
+        for i, _ in enumerate(ex.map(_compile_only, flames, chunksize=4)):
+            if i % 32 == 31:
+                log(f"[pairs] compiled {i + 1}/{len(flames)} in {time.time() - t0:.1f} s")
+    return [db.compile(f, **SHAPE) for f in flames]
 
 
 def coarse_importance(rgb: np.ndarray) -> np.ndarray:
@@ -54,7 +69,7 @@ def coarse_importance(rgb: np.ndarray) -> np.ndarray:
 def make_pairs(db, seeds, width, height, low_iters, high_iters, de_radius, threads=16, log=print):
     flames = [training_flame(s) for s in seeds]
     t0 = time.time()
-    kernels = compile_all(db, flames, threads)
+    kernels = compile_all(db, flames, threads, log)
     log(f"[pairs] compiled {len(kernels)} flame kernels in {time.time() - t0:.1f} s")
     n, npx = len(seeds), width * height
     feat = render.DeviceBuffer(n * npx * denoise.FEAT_C * 2)

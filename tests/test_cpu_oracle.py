@@ -197,3 +197,26 @@ def test_oracle_render_counts():
     hst, draws = oracle.render(L, P, P.shape[1], pal, 8, 256, 80, 45, 2, 5, 32)
     assert draws <= 8 * 256 * 32
     assert int(hst[..., 3].astype(np.int64).sum()) == draws * 255  # opacity 1 everywhere
+
+
+def test_density_hdr_closed_forms():
+    """gain (cdf/0.5)^2 on the HSV value; hue and saturation kept (colorsys)"""
+    import colorsys
+    rng = np.random.default_rng(3)
+    img = rng.uniform(0, 1, (64, 4))
+    bins = np.full(97, 10)  # uniform CDF: cdf(k) = (k + 1) / 97
+    out = oracle.density_hdr(img[None], bins)[0]
+    for px, o in zip(img, out):
+        power = px[3] * 96 + 1
+        lo = min(int(power), 96)
+        hi = min(lo + 1, 96)
+        v = ((lo + 1) + (power - np.floor(power)) * ((hi + 1) - (lo + 1))) / 97
+        h, s, val = colorsys.rgb_to_hsv(*px[:3])
+        want = colorsys.hsv_to_rgb(h, s, val * (v / 0.5) ** 2)
+        np.testing.assert_allclose(o[:3], want, rtol=1e-5, atol=1e-6)
+        assert o[3] == 1.0
+    # at cdf = 0.5 the image is unchanged; gray stays gray
+    bins = np.zeros(97)
+    bins[0] = bins[96] = 1  # cdf: 0.5 up to bin 95, 1 at bin 96
+    px = np.array([[[0.3, 0.3, 0.3, 10 / 96.0], [0.2, 0.5, 0.9, 20 / 96.0]]])
+    np.testing.assert_allclose(oracle.density_hdr(px, bins)[0, :, :3], px[0, :, :3], atol=1e-7)

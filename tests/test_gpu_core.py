@@ -306,3 +306,14 @@ def test_temporal_multiplier_blocks(db, gpu):
     assert len({tuple(np.round(r[:12], 5)) for r in P}) > 1  # rows differ along the segment
     r = st.bin(iters=inf.walkers * 8)
     assert r["total_passes"] == inf.walkers * 8
+
+
+def test_density_hdr_matches_oracle(small):
+    f, k = small
+    st = k.warmup(k.pack_samples(W, H), W, H, seed=3, fuse=10, walkers=8192)
+    st.bin(iters=8192 * 64)
+    img = st.tonemap(2.2, 4.0, 1.0)
+    bins, _ = st.density_histogram()
+    got = render.density_hdr(img, bins)
+    want = oracle.density_hdr(img, bins)
+    np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-5)

@@ -65,7 +65,7 @@ def test_queue_binning_bailouts(gpu, db):
     # quality: stops once the whole call reached the target mean density
     st = k.warmup(s, W, H, seed=3, walkers=1 << 16)
     r = st.bin(quality=20.0)
-    assert r["quality"] >= 20.0 and r["total_passes"] % (1 << 16) == 0
+    assert r["quality"] >= 20.0 and r["total_passes"] % 1024 == 0  # blocks stop at their own checks
     assert r["quality"] < 60.0  # stopped within a few bailout checks
     # time: a few milliseconds of device clock
     st = k.warmup(s, W, H, seed=3, walkers=1 << 16)
