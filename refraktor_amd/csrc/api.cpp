@@ -80,7 +80,7 @@ struct rfk_state {
     hipEvent_t ev_staging = nullptr;
     // queue binning scratch (kernels compiled with bin_queue)
     unsigned long long *queue = nullptr, *sorted = nullptr;
-    uint32_t *qcounts = nullptr, *qnseg = nullptr;
+    uint32_t *qcounts = nullptr, *qnseg = nullptr, *qtiles = nullptr;
     void* qsegments = nullptr;
     uint32_t chunk_passes = 0, max_segments = 0;
 };
@@ -109,9 +109,6 @@ struct rfk_batch {
 };
 
 namespace {
-
-// partitions of the queue binning count/scatter kernels (one workgroup each)
-constexpr uint32_t kQueueParts = 256;
 
 thread_local std::string g_last_error;
 
@@ -149,7 +146,7 @@ hipError_t dalloc(T** p, size_t count) {
 void free_state_buffers(rfk_state* s) {
     auto f = [](void* p) { if (p) (void)hipFree(p); };
     f(s->params); f(s->palettes); f(s->wx); f(s->wy); f(s->wc); f(s->wlast); f(s->block_pass); f(s->hist);
-    f(s->queue); f(s->sorted); f(s->qcounts); f(s->qnseg); f(s->qsegments);
+    f(s->queue); f(s->sorted); f(s->qcounts); f(s->qnseg); f(s->qsegments); f(s->qtiles);
     f(s->d_samples); f(s->d_segments); f(s->d_kind);
     if (s->ev_staging) (void)hipEventSynchronize(s->ev_staging);
     if (s->staging) (void)hipHostFree(s->staging);
@@ -184,13 +181,17 @@ rfk_iter_args base_args(rfk_state* s) {
     a.check_every = 16;
     a.nsamples = s->blocks;
     a.queue = reinterpret_cast<u64*>(s->queue);
+    a.qcounts = s->qcounts;
     a.tiles_x = rfk::sk::queue_tiles_x(s->width);
+    a.ntiles = rfk::sk::queue_tiles(s->width, s->height);
     return a;
 }
 
 int launch_iterate(rfk_state* s, rfk_iter_args& a) {
     void* argv[] = {&a};
-    RFK_HIP(hipModuleLaunchKernel(s->k->f_iterate, s->blocks, 1, 1, s->tpb, 1, 1, 0, s->stream, argv, nullptr));
+    // queue binning kernels count records per screen tile in dynamic LDS
+    const unsigned lds = s->k->opt.bin_queue ? a.ntiles * 4u : 0u;
+    RFK_HIP(hipModuleLaunchKernel(s->k->f_iterate, s->blocks, 1, 1, s->tpb, 1, 1, lds, s->stream, argv, nullptr));
     return RFK_OK;
 }
 
@@ -203,7 +204,7 @@ int bin_queued(rfk_state* s, rfk_iter_args& a, uint64_t passes, bool bounded) {
         const uint32_t n = uint32_t(std::min<uint64_t>(s->chunk_passes, passes - done));
         a.npasses = n;
         if (int rc = launch_iterate(s, a)) return rc;
-        RFK_HIP(rfk::sk::queue_bin(s->queue, size_t(n) * s->walkers, s->sorted, s->qcounts, kQueueParts,
+        RFK_HIP(rfk::sk::queue_bin(s->queue, size_t(n) * s->tpb, s->blocks, s->sorted, s->qcounts, s->qtiles,
                                    s->qsegments, s->max_segments, s->qnseg, s->width, s->height, s->hist, s->hist64,
                                    s->k->opt.atomic_width == 64, s->stream));
         done += n;
@@ -615,14 +616,16 @@ int rfk_warmup_ex(rfk_kernel* k, void* stream, const double* samples, size_t n_r
     s->hist64 = k->opt.hist64;
     RFK_HIPC(dalloc(&s->hist, s->hist_words()));
     if (k->opt.bin_queue) {
-        // chunks of whole passes of all walkers, about 2^26 records (512 MiB) each
-        s->chunk_passes = std::max<uint32_t>(1u, uint32_t((1ull << 26) / s->walkers));
+        // chunks of whole passes of all walkers, about 2^27 records (1 GiB) each
+        s->chunk_passes = std::max<uint32_t>(1u, uint32_t((1ull << 27) / s->walkers));
         const size_t q = size_t(s->chunk_passes) * s->walkers;
         const uint32_t ntiles = rfk::sk::queue_tiles(s->width, s->height);
+        if (q >= (1ull << 32)) return cleanup_on_error(fail(RFK_ERR_INVALID, "too many walkers for queue binning"));
         s->max_segments = rfk::sk::queue_max_segments(q, ntiles);
         RFK_HIPC(hipMalloc(&s->queue, q * 8));
         RFK_HIPC(hipMalloc(&s->sorted, q * 8));
-        RFK_HIPC(dalloc(&s->qcounts, size_t(ntiles) * kQueueParts));
+        RFK_HIPC(dalloc(&s->qcounts, size_t(ntiles) * s->blocks));
+        RFK_HIPC(dalloc(&s->qtiles, size_t(ntiles) * 2));
         RFK_HIPC(hipMalloc(&s->qsegments, size_t(s->max_segments) * 16));
         RFK_HIPC(dalloc(&s->qnseg, 4));
     }

@@ -344,9 +344,10 @@ struct rfk_iter_args {
     u32 flags;                 // RFK_FLAG_*
     u32 check_every;           // passes between bailout checks (power of two)
     u32 nsamples;              // rows of params (precalc kernel)
-    u64* queue;                // queue binning: [npasses][walkers] draw records
+    u64* queue;                // queue binning: [blocks][npasses][RFK_TPB] draw records
+    u32* qcounts;              // queue binning: [tiles][blocks] records per screen tile
     u32 tiles_x;               // queue binning: 64-pixel tiles per row
-    u32 queue_pad;
+    u32 ntiles;                // queue binning: tiles in the image
 };
 
 // Persistent batch launch: work items are (frame, block) pairs

@@ -31,9 +31,10 @@
 #endif
 
 // Queue binning: instead of atomics, every pass of every walker stores one
-// 64-bit draw record at queue[pass][walker] (coalesced, no atomics); the
-// records are then counting-sorted by 64x64-pixel screen tile and summed per
-// tile in LDS by the library's queue kernels. Record layout:
+// 64-bit draw record at queue[block][pass][lane] (coalesced, no atomics) and
+// the block counts its records per 64x64-pixel screen tile in LDS; the
+// library's queue kernels then counting-sort the records by tile and sum
+// them per tile in LDS. Record layout:
 //   bits  0..31  R | G << 8 | B << 16 | W << 24   (fixed-point increments)
 //   bits 32..43  pixel inside the tile (y * 64 + x)
 //   bits 44..63  tile index (ty * tiles_x + tx); all ones = no draw
@@ -46,6 +47,25 @@ RFK_INLINE u64 rfk_queue_record(const rfk_draw_t& d, u32 tiles_x) {
     const u32 col = d.r | (d.g << 8) | (d.b << 16) | (d.w << 24);
     return u64(col) | (u64(local) << 32) | (u64(tile) << 44);
 }
+
+#if RFK_BIN_QUEUE
+// per-block record counts per tile (dynamic LDS, a.ntiles entries)
+extern __shared__ u32 rfk_qcount[];
+
+// count a wave's records: lanes sharing the first valid lane's tile add once
+RFK_INLINE void rfk_queue_count(u32 tile, bool valid) {
+    const u64 vm = __builtin_amdgcn_ballot_w64(valid);
+    if (vm == 0ull) return;
+    const u32 lead = __builtin_amdgcn_readlane(tile, __builtin_ctzll(vm));
+    const u64 same = __builtin_amdgcn_ballot_w64(valid && tile == lead);
+    if (valid && tile == lead) {
+        if (__builtin_amdgcn_mbcnt_hi(u32(same >> 32), __builtin_amdgcn_mbcnt_lo(u32(same), 0u)) == 0u)
+            atomicAdd(&rfk_qcount[lead], u32(__builtin_popcountll(same)));
+    } else if (valid) {
+        atomicAdd(&rfk_qcount[tile], 1u);
+    }
+}
+#endif
 
 // bin element: u32 fixed point (wraps past 2^32 / 255 full-opacity draws in
 // one pixel) or u64 for budgets that can put more than that into a pixel
@@ -205,6 +225,8 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, const Real* __r
 
     const u32 ntotal = nfuse + npasses;
 #if RFK_BIN_QUEUE
+    if (draw)
+        for (u32 k = t; k < a.ntiles; k += RFK_TPB) rfk_qcount[k] = 0u;
     __syncthreads();
     const u32 nrun = sm.stop ? 0u : ntotal;
 #else
@@ -269,7 +291,8 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, const Real* __r
                 draws++;
             }
 #if RFK_BIN_QUEUE
-            a.queue[(size_t)(i - nfuse) * a.total_walkers + gid] = rec;
+            a.queue[((size_t)block * npasses + (i - nfuse)) * RFK_TPB + t] = rec;
+            rfk_queue_count(u32(rec >> 44), rec != RFK_QEMPTY);
 #endif
         }
 #if RFK_BIN_TILE_LOG2
@@ -354,9 +377,12 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, const Real* __r
 #endif
 #if RFK_BIN_QUEUE
     // passes a bailout skipped leave no draws behind
-    if (draw)
+    if (draw) {
         for (u32 j = (done > nfuse ? done : nfuse); j < ntotal; j++)
-            a.queue[(size_t)(j - nfuse) * a.total_walkers + gid] = RFK_QEMPTY;
+            a.queue[((size_t)block * npasses + (j - nfuse)) * RFK_TPB + t] = RFK_QEMPTY;
+        __syncthreads();
+        for (u32 k = t; k < a.ntiles; k += RFK_TPB) a.qcounts[(size_t)k * gridDim.x + block] = rfk_qcount[k];
+    }
 #endif
 
     if (wx) {

@@ -1,16 +1,19 @@
 // Queue binning: the iterate kernel stores one 64-bit record per walker and
-// pass (see flame_kernels.h for the layout); these kernels fold a chunk of
-// records into the histogram without contended global atomics:
+// pass (layout in flame_kernels.h), block-major: block b's records of a chunk
+// are contiguous, and the block counts its records per 64x64-pixel screen
+// tile in LDS (counts [tile][block]). These kernels fold a chunk of records
+// into the histogram without contended global atomics:
 //
-//   1. count    per (tile, partition) record counts; a partition is a fixed
-//               contiguous range of records handled by one workgroup
-//   2. scan     exclusive scan over [tile][partition] -> write offsets, and
-//               the table of (tile, segment) work items for step 4
-//   3. scatter  counting sort by tile: each partition sorts 4096-record
-//               pieces in LDS and writes tile runs to their offsets
-//   4. bin      one workgroup per (tile, segment of <= 2^18 records) sums
-//               the records in a 64x64 LDS tile, then adds the non-zero
-//               pixels into the histogram with one atomic per channel
+//   1. qscan_col  per tile: exclusive prefix of its counts over the blocks
+//                 (in place) and the tile's total
+//   2. qseg       exclusive scan of the tile totals -> tile starts, and the
+//                 (tile, segment of <= 2^18 records) work items of step 4
+//   3. qscatter   counting sort by tile: each block's records, 8192 at a time,
+//                 ranked in LDS and written as tile runs at their offsets
+//   4. qbin       one workgroup per work item sums its records in a 64x64
+//                 LDS tile, then adds the non-zero pixels into the histogram:
+//                 plain read-add-write when the work item is the tile's only
+//                 one, one atomic per channel otherwise
 //
 // All integer sums: the histogram equals the one direct atomics give, bit
 // for bit, whatever the order.
@@ -26,19 +29,18 @@ namespace {
 
 typedef unsigned long long u64;
 
-constexpr uint32_t kTileLog2 = 6;                 // 64x64-pixel tiles
+constexpr uint32_t kTileLog2 = 6;  // 64x64-pixel tiles
 constexpr uint32_t kTilePix = 1u << (2 * kTileLog2);
 constexpr uint32_t kNoTile = 0xFFFFFu;
-constexpr uint32_t kPartThreads = 1024;
-constexpr uint32_t kPiece = 4096;                 // records sorted in LDS at a time
-constexpr uint32_t kSegment = 1u << 18;           // records per bin work item (u32 LDS sums cannot wrap)
+constexpr uint32_t kThreads = 1024;
+constexpr uint32_t kPiece = 8192;        // records sorted in LDS at a time
+constexpr uint32_t kSegment = 1u << 18;  // records per bin work item (u32 LDS sums cannot wrap)
 
 __device__ __forceinline__ uint32_t rec_tile(u64 r) { return uint32_t(r >> 44); }
 
-// Lanes of a wave that share the wave leader's key add once, by the first of
-// them; the others (usually a minority for real flames, the whole wave for a
-// concentrated one) add on their own. Returns the rank of the lane's record
-// among those counted for its key in this call.
+// Lanes of a wave that share the first valid lane's key add once, by the
+// first of them; the others add on their own. Returns the rank of the lane's
+// record among those counted for its key in this call.
 __device__ __forceinline__ uint32_t wave_count(uint32_t* counters, uint32_t key, bool valid) {
     const u64 vm = __ballot(valid);
     if (vm == 0ull) return 0u;
@@ -59,125 +61,110 @@ __device__ __forceinline__ uint32_t wave_count(uint32_t* counters, uint32_t key,
     return rank;
 }
 
-__global__ void __launch_bounds__(kPartThreads)
-    qcount_kernel(const u64* __restrict__ rec, size_t n, size_t per_part, uint32_t ntiles, uint32_t nparts,
-                  uint32_t* __restrict__ counts) {
-    extern __shared__ uint32_t s_cnt[];
-    for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) s_cnt[t] = 0;
+// Block-wide exclusive scan of one value per thread (blockDim.x = 1024).
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t* total) {
+    uint32_t incl = v;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(incl, d);
+        if (__lane_id() >= d) incl += u;
+    }
+    if (__lane_id() == 63) s_wave[threadIdx.x >> 6] = incl;
     __syncthreads();
-    const size_t b0 = size_t(blockIdx.x) * per_part, b1 = std::min(n, b0 + per_part);
-    for (size_t base = b0; base < b1; base += blockDim.x) {
-        const size_t i = base + threadIdx.x;
-        const u64 r = i < b1 ? rec[i] : ~0ull;
-        const uint32_t tile = rec_tile(r);
-        (void)wave_count(s_cnt, tile, tile < ntiles);
+    uint32_t base = 0, all = 0;
+    for (uint32_t w = 0; w < kThreads / 64; w++) {
+        const uint32_t s = s_wave[w];
+        if (w < (threadIdx.x >> 6)) base += s;
+        all += s;
     }
     __syncthreads();
-    for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) counts[size_t(t) * nparts + blockIdx.x] = s_cnt[t];
+    if (total) *total = all;
+    return base + incl - v;
 }
 
-// One workgroup. counts [ntiles][nparts] -> exclusive offsets in place;
-// segments: (tile, begin, end) per work item of step 4, nseg[0] = count.
-__global__ void __launch_bounds__(1024)
-    qscan_kernel(uint32_t* __restrict__ counts, uint32_t ntiles, uint32_t nparts, uint4* __restrict__ segments,
-                 uint32_t max_segments, uint32_t* __restrict__ nseg) {
-    __shared__ uint32_t s_sum[1024];
-    __shared__ uint32_t s_segs[1024];
-    const uint32_t n = ntiles * nparts;
-    const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
-    const uint32_t lo = threadIdx.x * per, hi = std::min(n, lo + per);
+// grid = ntiles: column t of counts [tile][nparts] -> exclusive prefix in
+// place, tile_total[t] = its sum.
+__global__ void __launch_bounds__(kThreads)
+    qscan_col_kernel(uint32_t* __restrict__ counts, uint32_t nparts, uint32_t* __restrict__ tile_total) {
+    __shared__ uint32_t s_wave[kThreads / 64];
+    uint32_t* col = counts + size_t(blockIdx.x) * nparts;
+    const uint32_t per = (nparts + kThreads - 1) / kThreads;
+    const uint32_t lo = threadIdx.x * per, hi = std::min(nparts, lo + per);
     uint32_t sum = 0;
-    for (uint32_t i = lo; i < hi; i++) sum += counts[i];
-    s_sum[threadIdx.x] = sum;
-    __syncthreads();
-    // inclusive Hillis-Steele scan of the 1024 partial sums
-    for (uint32_t d = 1; d < blockDim.x; d <<= 1) {
-        const uint32_t v = threadIdx.x >= d ? s_sum[threadIdx.x - d] : 0u;
-        __syncthreads();
-        s_sum[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = threadIdx.x ? s_sum[threadIdx.x - 1] : 0u;
+    for (uint32_t i = lo; i < hi; i++) sum += col[i];
+    uint32_t total = 0;
+    uint32_t run = block_exclusive_scan(sum, s_wave, &total);
     for (uint32_t i = lo; i < hi; i++) {
-        const uint32_t c = counts[i];
-        counts[i] = run;
+        const uint32_t c = col[i];
+        col[i] = run;
         run += c;
     }
-    const uint32_t total = s_sum[blockDim.x - 1];
-    __syncthreads();
-    // segments: tile t covers [off(t,0), off(t+1,0)) split into kSegment pieces;
-    // tiles are handed out in contiguous ranges, one range per thread
-    const uint32_t tper = (ntiles + blockDim.x - 1) / blockDim.x;
-    const uint32_t t0 = threadIdx.x * tper, t1 = std::min(ntiles, t0 + tper);
-    uint32_t nmine = 0;
-    for (uint32_t t = t0; t < t1; t++) {
-        const uint32_t b = counts[size_t(t) * nparts];
-        const uint32_t e = t + 1 < ntiles ? counts[size_t(t + 1) * nparts] : total;
-        nmine += (e - b + kSegment - 1) / kSegment;
-    }
-    s_segs[threadIdx.x] = nmine;
-    __syncthreads();
-    for (uint32_t d = 1; d < blockDim.x; d <<= 1) {
-        const uint32_t v = threadIdx.x >= d ? s_segs[threadIdx.x - d] : 0u;
-        __syncthreads();
-        s_segs[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t k = threadIdx.x ? s_segs[threadIdx.x - 1] : 0u;
-    for (uint32_t t = t0; t < t1; t++) {
-        const uint32_t b = counts[size_t(t) * nparts];
-        const uint32_t e = t + 1 < ntiles ? counts[size_t(t + 1) * nparts] : total;
-        for (uint32_t s = b; s < e; s += kSegment) {
-            if (k < max_segments) segments[k] = make_uint4(t, s, std::min(e, s + kSegment), 0u);
-            k++;
-        }
-    }
-    if (threadIdx.x == blockDim.x - 1) nseg[0] = std::min(s_segs[blockDim.x - 1], max_segments);
+    if (threadIdx.x == 0) tile_total[blockIdx.x] = total;
 }
 
-// LDS: cursor[ntiles] (global write position of this partition per tile),
-// lcount[ntiles], lstart[ntiles], piece[kPiece] records.
-__global__ void __launch_bounds__(kPartThreads)
-    qscatter_kernel(const u64* __restrict__ rec, size_t n, size_t per_part, uint32_t ntiles, uint32_t nparts,
-                    const uint32_t* __restrict__ offsets, u64* __restrict__ sorted) {
+// One workgroup: tile_start = exclusive scan of tile_total; work items
+// (tile, begin, end, exclusive) in tile order; nseg[0] = their count.
+__global__ void __launch_bounds__(kThreads)
+    qseg_kernel(const uint32_t* __restrict__ tile_total, uint32_t ntiles, uint32_t* __restrict__ tile_start,
+                uint4* __restrict__ segments, uint32_t max_segments, uint32_t* __restrict__ nseg) {
+    __shared__ uint32_t s_wave[kThreads / 64];
+    const uint32_t per = (ntiles + kThreads - 1) / kThreads;
+    const uint32_t lo = threadIdx.x * per, hi = std::min(ntiles, lo + per);
+    uint32_t sum = 0, segs = 0;
+    for (uint32_t t = lo; t < hi; t++) {
+        sum += tile_total[t];
+        segs += (tile_total[t] + kSegment - 1) / kSegment;
+    }
+    uint32_t run = block_exclusive_scan(sum, s_wave, nullptr);
+    uint32_t nall = 0;
+    uint32_t k = block_exclusive_scan(segs, s_wave, &nall);
+    for (uint32_t t = lo; t < hi; t++) {
+        const uint32_t n = tile_total[t];
+        tile_start[t] = run;
+        const uint32_t nt = (n + kSegment - 1) / kSegment;
+        for (uint32_t s = 0; s < nt; s++, k++) {
+            if (k < max_segments)
+                segments[k] = make_uint4(t, run + s * kSegment, run + std::min(n, (s + 1) * kSegment), nt == 1 ? 1u : 0u);
+        }
+        run += n;
+    }
+    if (threadIdx.x == 0) nseg[0] = std::min(nall, max_segments);
+}
+
+// grid = nparts: partition p = records [p * per_part, (p + 1) * per_part).
+// LDS: cursor[ntiles], lcount[ntiles], lstart[ntiles], piece[kPiece].
+__global__ void __launch_bounds__(kThreads)
+    qscatter_kernel(const u64* __restrict__ rec, size_t per_part, uint32_t ntiles, uint32_t nparts,
+                    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ tile_start,
+                    u64* __restrict__ sorted) {
     extern __shared__ uint32_t smem[];
     uint32_t* cursor = smem;
     uint32_t* lcount = cursor + ntiles;
     uint32_t* lstart = lcount + ntiles;
     u64* piece = reinterpret_cast<u64*>(lstart + ntiles + (ntiles & 1u));
-    __shared__ uint32_t s_wave[kPartThreads / 64];
-    for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) cursor[t] = offsets[size_t(t) * nparts + blockIdx.x];
-    const size_t b0 = size_t(blockIdx.x) * per_part, b1 = std::min(n, b0 + per_part);
-    constexpr uint32_t kPer = kPiece / kPartThreads;
+    __shared__ uint32_t s_wave[kThreads / 64];
+    for (uint32_t t = threadIdx.x; t < ntiles; t += kThreads)
+        cursor[t] = tile_start[t] + offsets[size_t(t) * nparts + blockIdx.x];
+    const size_t b0 = size_t(blockIdx.x) * per_part, b1 = b0 + per_part;
+    constexpr uint32_t kPer = kPiece / kThreads;
+    const uint32_t per = (ntiles + kThreads - 1) / kThreads;
+    const uint32_t lo = threadIdx.x * per, hi = std::min(ntiles, lo + per);
     for (size_t base = b0; base < b1; base += kPiece) {
-        for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) lcount[t] = 0;
+        for (uint32_t t = threadIdx.x; t < ntiles; t += kThreads) lcount[t] = 0;
         __syncthreads();
         u64 r[kPer];
         uint32_t rank[kPer];
 #pragma unroll
         for (uint32_t j = 0; j < kPer; j++) {
-            const size_t i = base + size_t(j) * kPartThreads + threadIdx.x;
+            const size_t i = base + size_t(j) * kThreads + threadIdx.x;
             r[j] = i < b1 ? rec[i] : ~0ull;
             const uint32_t tile = rec_tile(r[j]);
             rank[j] = wave_count(lcount, tile, tile < ntiles);
         }
         __syncthreads();
-        // exclusive scan of lcount -> lstart (ntiles entries, blockDim.x threads)
-        const uint32_t per = (ntiles + blockDim.x - 1) / blockDim.x;
-        const uint32_t lo = threadIdx.x * per, hi = std::min(ntiles, lo + per);
         uint32_t sum = 0;
         for (uint32_t t = lo; t < hi; t++) sum += lcount[t];
-        // wave-level inclusive scan of the per-thread sums, then across waves
-        uint32_t incl = sum;
-        for (uint32_t d = 1; d < 64; d <<= 1) {
-            const uint32_t v = __shfl_up(incl, d);
-            if (__lane_id() >= d) incl += v;
-        }
-        if (__lane_id() == 63) s_wave[threadIdx.x >> 6] = incl;
-        __syncthreads();
-        uint32_t wave_base = 0;
-        for (uint32_t w = 0; w < (threadIdx.x >> 6); w++) wave_base += s_wave[w];
-        uint32_t run = wave_base + incl - sum;
+        uint32_t used = 0;
+        uint32_t run = block_exclusive_scan(sum, s_wave, &used);
         for (uint32_t t = lo; t < hi; t++) {
             lstart[t] = run;
             run += lcount[t];
@@ -189,24 +176,30 @@ __global__ void __launch_bounds__(kPartThreads)
             if (tile < ntiles) piece[lstart[tile] + rank[j]] = r[j];
         }
         __syncthreads();
-        uint32_t used = 0;
-        for (uint32_t w = 0; w < kPartThreads / 64; w++) used += s_wave[w];
         // sorted order: consecutive threads write consecutive addresses of a run
-        for (uint32_t k = threadIdx.x; k < used; k += kPartThreads) {
+        for (uint32_t k = threadIdx.x; k < used; k += kThreads) {
             const u64 v = piece[k];
             const uint32_t tile = rec_tile(v);
             sorted[size_t(cursor[tile]) + (k - lstart[tile])] = v;
         }
         __syncthreads();
-        for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) cursor[t] += lcount[t];
+        for (uint32_t t = threadIdx.x; t < ntiles; t += kThreads) cursor[t] += lcount[t];
         __syncthreads();
     }
 }
 
 template <class HistT>
 __device__ __forceinline__ void add_bin(HistT* __restrict__ hist, size_t p, uint32_t r, uint32_t g, uint32_t b,
-                                        uint32_t w, bool packed64) {
+                                        uint32_t w, bool packed64, bool exclusive) {
     HistT* h = hist + 4 * p;
+    if (exclusive) {
+        // the only work item of its tile: nobody else touches these bins now
+        h[0] += HistT(r);
+        h[1] += HistT(g);
+        h[2] += HistT(b);
+        h[3] += HistT(w);
+        return;
+    }
     if constexpr (sizeof(HistT) == 4) {
         if (packed64) {
             u64* h64 = reinterpret_cast<u64*>(h);
@@ -222,20 +215,20 @@ __device__ __forceinline__ void add_bin(HistT* __restrict__ hist, size_t p, uint
 }
 
 template <class HistT>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kThreads)
     qbin_kernel(const u64* __restrict__ sorted, const uint4* __restrict__ segments, const uint32_t* __restrict__ nseg,
                 uint32_t width, uint32_t height, uint32_t tiles_x, HistT* __restrict__ hist, bool packed64) {
     __shared__ uint32_t acc[4][kTilePix];
     if (blockIdx.x >= nseg[0]) return;
     const uint4 sg = segments[blockIdx.x];
-    for (uint32_t p = threadIdx.x; p < kTilePix; p += blockDim.x) {
+    for (uint32_t p = threadIdx.x; p < kTilePix; p += kThreads) {
         acc[0][p] = 0u;
         acc[1][p] = 0u;
         acc[2][p] = 0u;
         acc[3][p] = 0u;
     }
     __syncthreads();
-    for (uint32_t i = sg.y + threadIdx.x; i < sg.z; i += blockDim.x) {
+    for (uint32_t i = sg.y + threadIdx.x; i < sg.z; i += kThreads) {
         const u64 v = sorted[i];
         const uint32_t local = uint32_t(v >> 32) & (kTilePix - 1u);
         const uint32_t c = uint32_t(v);
@@ -246,11 +239,12 @@ __global__ void __launch_bounds__(1024)
     }
     __syncthreads();
     const uint32_t tx = sg.x % tiles_x, ty = sg.x / tiles_x;
-    for (uint32_t p = threadIdx.x; p < kTilePix; p += blockDim.x) {
+    const bool exclusive = sg.w != 0u;
+    for (uint32_t p = threadIdx.x; p < kTilePix; p += kThreads) {
         const uint32_t r = acc[0][p], g = acc[1][p], b = acc[2][p], w = acc[3][p];
         if ((r | g | b | w) == 0u) continue;
         const uint32_t x = (tx << kTileLog2) + (p & 63u), y = (ty << kTileLog2) + (p >> kTileLog2);
-        if (x < width && y < height) add_bin(hist, size_t(y) * width + x, r, g, b, w, packed64);
+        if (x < width && y < height) add_bin(hist, size_t(y) * width + x, r, g, b, w, packed64, exclusive);
     }
 }
 
@@ -262,28 +256,29 @@ uint32_t queue_max_segments(size_t records, uint32_t ntiles) {
     return uint32_t((records + kSegment - 1) / kSegment) + ntiles;
 }
 
-hipError_t queue_bin(const unsigned long long* records, size_t n, unsigned long long* sorted, uint32_t* counts,
-                     uint32_t nparts, void* segments, uint32_t max_segments, uint32_t* nseg, uint32_t width,
-                     uint32_t height, uint32_t* hist, bool hist64, bool packed64, hipStream_t s) {
-    if (!n) return hipSuccess;
+hipError_t queue_bin(const unsigned long long* records, size_t per_part, uint32_t nparts, unsigned long long* sorted,
+                     uint32_t* counts, uint32_t* tile_scratch, void* segments, uint32_t max_segments, uint32_t* nseg,
+                     uint32_t width, uint32_t height, uint32_t* hist, bool hist64, bool packed64, hipStream_t s) {
+    if (!per_part || !nparts) return hipSuccess;
     const uint32_t tiles_x = queue_tiles_x(width), ntiles = queue_tiles(width, height);
     if (ntiles >= kNoTile) return hipErrorInvalidValue;
-    // `parts` contiguous partitions of whole 4096-record pieces; the count
-    // table is [tile][parts] (at most ntiles * nparts entries)
-    const size_t per_part = ((n + nparts - 1) / nparts + kPiece - 1) / kPiece * kPiece;
-    const uint32_t parts = uint32_t((n + per_part - 1) / per_part);
-    qcount_kernel<<<parts, kPartThreads, ntiles * sizeof(uint32_t), s>>>(records, n, per_part, ntiles, parts,
-                                                                          counts);
-    qscan_kernel<<<1, 1024, 0, s>>>(counts, ntiles, parts, reinterpret_cast<uint4*>(segments), max_segments, nseg);
+    uint32_t* tile_total = tile_scratch;
+    uint32_t* tile_start = tile_scratch + ntiles;
+    qscan_col_kernel<<<ntiles, kThreads, 0, s>>>(counts, nparts, tile_total);
+    qseg_kernel<<<1, kThreads, 0, s>>>(tile_total, ntiles, tile_start, reinterpret_cast<uint4*>(segments),
+                                       max_segments, nseg);
     const size_t lds = size_t(3 * ntiles + 1) * sizeof(uint32_t) + kPiece * sizeof(u64) + 8;
-    qscatter_kernel<<<parts, kPartThreads, lds, s>>>(records, n, per_part, ntiles, parts, counts, sorted);
-    const uint32_t grid = max_segments;
+    // more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per workgroup)
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&qscatter_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+    qscatter_kernel<<<nparts, kThreads, lds, s>>>(records, per_part, ntiles, nparts, counts, tile_start, sorted);
     if (hist64)
-        qbin_kernel<<<grid, 1024, 0, s>>>(sorted, reinterpret_cast<const uint4*>(segments), nseg, width, height,
-                                          tiles_x, reinterpret_cast<u64*>(hist), false);
+        qbin_kernel<<<max_segments, kThreads, 0, s>>>(sorted, reinterpret_cast<const uint4*>(segments), nseg, width,
+                                                      height, tiles_x, reinterpret_cast<u64*>(hist), false);
     else
-        qbin_kernel<<<grid, 1024, 0, s>>>(sorted, reinterpret_cast<const uint4*>(segments), nseg, width, height,
-                                          tiles_x, hist, packed64);
+        qbin_kernel<<<max_segments, kThreads, 0, s>>>(sorted, reinterpret_cast<const uint4*>(segments), nseg, width,
+                                                      height, tiles_x, hist, packed64);
     return hipGetLastError();
 }
 

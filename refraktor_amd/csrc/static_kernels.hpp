@@ -41,16 +41,19 @@ hipError_t density_estimate(const uint32_t* hist, bool hist64, uint32_t width, u
 hipError_t tonemap(const void* in, int in_kind, size_t npix, void* out, int format, float scale_constant,
                    float brightness, float gamma, float vibrancy, hipStream_t s);
 
-// Queue binning (queue_bin.hip): fold n draw records (64x64-pixel tile
-// layout, see flame_kernels.h) into the histogram. Scratch: sorted (n
-// records), counts (ntiles * nparts u32), segments (max_segments * 16 B),
-// nseg (one u32). packed64: 32-bit bins updated as two 64-bit pairs.
+// Queue binning (queue_bin.hip): fold nparts * per_part draw records
+// (partition p = the records of iterate block p, 64x64-pixel tile layout,
+// see flame_kernels.h) into the histogram. counts: [ntiles][nparts] records
+// per tile and partition, as the iterate kernel wrote them (overwritten).
+// Scratch: sorted (all records), tile_scratch (2 * ntiles u32), segments
+// (max_segments * 16 B), nseg (one u32). packed64: 32-bit bins updated as
+// two 64-bit pairs.
 uint32_t queue_tiles_x(uint32_t width);
 uint32_t queue_tiles(uint32_t width, uint32_t height);
 uint32_t queue_max_segments(size_t records, uint32_t ntiles);
-hipError_t queue_bin(const unsigned long long* records, size_t n, unsigned long long* sorted, uint32_t* counts,
-                     uint32_t nparts, void* segments, uint32_t max_segments, uint32_t* nseg, uint32_t width,
-                     uint32_t height, uint32_t* hist, bool hist64, bool packed64, hipStream_t s);
+hipError_t queue_bin(const unsigned long long* records, size_t per_part, uint32_t nparts, unsigned long long* sorted,
+                     uint32_t* counts, uint32_t* tile_scratch, void* segments, uint32_t max_segments, uint32_t* nseg,
+                     uint32_t width, uint32_t height, uint32_t* hist, bool hist64, bool packed64, hipStream_t s);
 
 // density_hdr (tonemap.cu:159-233) on a tonemapped float4 / half4 image,
 // with the 97-bin density histogram (device u64) of the same render.
