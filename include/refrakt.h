@@ -322,6 +322,10 @@ int rfk_denoiser_train_step(rfk_denoiser* d, const void* feat, const float* base
  * dw = dL/dW for dL/dy = dy, dx = dL/dx. act: 0 none, 1 ReLU.             */
 int rfk_nn_conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N, int H, int W, const void* wgt,
                            const float* bias, int Cout, int act, void* y, float* yf, void* stream);
+/* as rfk_nn_conv3x3_forward with x1 read through a nearest 2x upsample
+ * (x1_mode 1: x1 is H/2 x W/2) or a 2x2 average pool (2: x1 is 2H x 2W)  */
+int rfk_nn_conv3x3_forward_ex(const void* x1, int c1, int x1_mode, const void* x2, int c2, int N, int H, int W,
+                              const void* wgt, const float* bias, int Cout, int act, void* y, float* yf, void* stream);
 int rfk_nn_conv3x3_wgrad(const void* x1, int c1, const void* x2, int c2, const void* dy, int Cout, int N, int H,
                          int W, float* dw, void* stream);
 int rfk_nn_conv3x3_dgrad(const void* dy, int Cout, int N, int H, int W, const void* wgt, int Cin, void* dx,

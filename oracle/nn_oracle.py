@@ -175,3 +175,33 @@ def unet_loss_and_grads(feat, base, target, wflat, bflat, bf16_acts=True):
     g = back(1, A["a0"], A["e0"], pool_bwd(g) + g_e0, 1)
     back(0, x, A["a0"], g, 1)
     return loss, np.concatenate([v.ravel() for v in gw]), np.concatenate(gb), out
+
+
+def unet_forward(feat, base, wflat, bflat):
+    """inference output clamp(base + U-Net(feat)[..., :3], 0, 1) in float64
+    with BF16-rounded activations (pooled values rounded too, as the fused
+    GPU path rounds them when it reads them)"""
+    ws, bs = split_params(wflat, bflat)
+    q = to_bf16
+    relu = lambda a: np.maximum(a, 0.0)  # noqa: E731
+
+    def conv(i, x):
+        return conv3x3(x, ws[i], bs[i])
+
+    x = feat.astype(np.float64)
+    a0 = q(relu(conv(0, x)))
+    e0 = q(relu(conv(1, a0)))
+    a1 = q(relu(conv(2, q(pool(e0)))))
+    e1 = q(relu(conv(3, a1)))
+    a2 = q(relu(conv(4, q(pool(e1)))))
+    e2 = q(relu(conv(5, a2)))
+    a3 = q(relu(conv(6, q(pool(e2)))))
+    b = q(relu(conv(7, a3)))
+    c2 = q(relu(conv(8, np.concatenate([up(b), e2], axis=-1))))
+    d2 = q(relu(conv(9, c2)))
+    c1 = q(relu(conv(10, np.concatenate([up(d2), e1], axis=-1))))
+    d1 = q(relu(conv(11, c1)))
+    c0 = q(relu(conv(12, np.concatenate([up(d1), e0], axis=-1))))
+    d0 = q(relu(conv(13, c0)))
+    r = conv(14, d0)
+    return np.clip(base + r[..., :3], 0.0, 1.0)

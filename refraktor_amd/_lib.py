@@ -144,6 +144,8 @@ _SIGS = [
                                           C.POINTER(C.c_float), _P]),
     ("rfk_nn_conv3x3_forward", C.c_int, [_P, C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int,
                                          C.c_int, _P, _P, _P]),
+    ("rfk_nn_conv3x3_forward_ex", C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P,
+                                            C.c_int, C.c_int, _P, _P, _P]),
     ("rfk_nn_conv3x3_wgrad", C.c_int, [_P, C.c_int, _P, C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P]),
     ("rfk_nn_conv3x3_dgrad", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, C.c_int, _P, _P]),
     ("rfk_nn_gather_crops", C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.c_int, C.c_int, C.c_int, _P, _P]),

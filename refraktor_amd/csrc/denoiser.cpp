@@ -182,21 +182,49 @@ struct level_shape {
 };
 
 // Forward pass; activations are kept under their names for the backward pass.
-int forward(rfk_denoiser* d, int N, int H, int W, const void* feat, float* rout, hipStream_t s) {
+// fused (inference): pools and upsamples are read through by the next
+// convolution instead of being stored; training keeps every activation for
+// the backward pass.
+int forward(rfk_denoiser* d, int N, int H, int W, const void* feat, float* rout, hipStream_t s, bool fused = false) {
     int rc = 0;
     const level_shape L[4] = {{H, W}, {H / 2, W / 2}, {H / 4, W / 4}, {H / 8, W / 8}};
     auto A = [&](const std::string& name, int lvl, int C) -> void* {
         buf& b = act_buf(d, N, H, W, name, size_t(N) * L[lvl].H * L[lvl].W * C * 2, rc);
         return b.p;
     };
-    auto conv = [&](int li, const void* x1, const void* x2, int lvl, void* y, float* yf) -> int {
+    auto conv = [&](int li, const void* x1, const void* x2, int lvl, void* y, float* yf, int mode = 0) -> int {
         const conv_def& c = d->convs[size_t(li)];
         NN_HIP(rfk::nn::conv3x3_forward(x1, c.c1, x2, c.c2, N, L[lvl].H, L[lvl].W,
                                         static_cast<char*>(d->wbf) + c.w_off * 2, d->b + c.b_off, c.cout, c.act, 0.f,
-                                        y, yf, s));
+                                        y, yf, s, mode));
         return RFK_OK;
     };
     const int* ch = d->ch;
+    if (fused) {
+        void *a0 = A("a0", 0, ch[0]), *e0 = A("e0", 0, ch[0]);
+        void *a1 = A("a1", 1, ch[1]), *e1 = A("e1", 1, ch[1]);
+        void *a2 = A("a2", 2, ch[2]), *e2 = A("e2", 2, ch[2]);
+        void *a3 = A("a3", 3, ch[3]), *bb = A("b", 3, ch[3]);
+        void *c2 = A("c2", 2, ch[2]), *d2 = A("d2", 2, ch[2]);
+        void *c1 = A("c1", 1, ch[1]), *d1 = A("d1", 1, ch[1]);
+        void *c0 = A("c0", 0, ch[0]), *d0 = A("d0", 0, ch[0]);
+        if (rc) return rc;
+        if ((rc = conv(0, feat, nullptr, 0, a0, nullptr))) return rc;
+        if ((rc = conv(1, a0, nullptr, 0, e0, nullptr))) return rc;
+        if ((rc = conv(2, e0, nullptr, 1, a1, nullptr, 2))) return rc;
+        if ((rc = conv(3, a1, nullptr, 1, e1, nullptr))) return rc;
+        if ((rc = conv(4, e1, nullptr, 2, a2, nullptr, 2))) return rc;
+        if ((rc = conv(5, a2, nullptr, 2, e2, nullptr))) return rc;
+        if ((rc = conv(6, e2, nullptr, 3, a3, nullptr, 2))) return rc;
+        if ((rc = conv(7, a3, nullptr, 3, bb, nullptr))) return rc;
+        if ((rc = conv(8, bb, e2, 2, c2, nullptr, 1))) return rc;
+        if ((rc = conv(9, c2, nullptr, 2, d2, nullptr))) return rc;
+        if ((rc = conv(10, d2, e1, 1, c1, nullptr, 1))) return rc;
+        if ((rc = conv(11, c1, nullptr, 1, d1, nullptr))) return rc;
+        if ((rc = conv(12, d1, e0, 0, c0, nullptr, 1))) return rc;
+        if ((rc = conv(13, c0, nullptr, 0, d0, nullptr))) return rc;
+        return conv(14, d0, nullptr, 0, nullptr, rout);
+    }
     void *a0 = A("a0", 0, ch[0]), *e0 = A("e0", 0, ch[0]), *p0 = A("p0", 1, ch[0]);
     void *a1 = A("a1", 1, ch[1]), *e1 = A("e1", 1, ch[1]), *p1 = A("p1", 2, ch[1]);
     void *a2 = A("a2", 2, ch[2]), *e2 = A("e2", 2, ch[2]), *p2 = A("p2", 3, ch[2]);
@@ -442,7 +470,7 @@ int rfk_denoiser_forward(rfk_denoiser* d, const void* feat, const float* base, i
     const size_t npix = size_t(N) * H * W;
     float* r = static_cast<float*>(act_buf(d, N, H, W, "r", npix * kOut * 4, rc).p);
     if (rc) return rc;
-    if ((rc = forward(d, N, H, W, feat, r, s))) return rc;
+    if ((rc = forward(d, N, H, W, feat, r, s, true))) return rc;
     NN_HIP(rfk::nn::residual_out(base, 3, r, kOut, npix, out_rgb, s));
     NN_HIP(rfk::nn::clamp_rgb(out_rgb, npix * 3, out_rgb, s));
     return RFK_OK;
@@ -481,8 +509,15 @@ int rfk_denoiser_train_step(rfk_denoiser* d, const void* feat, const float* base
 // Single-layer entry points (used by the gradient checks).
 int rfk_nn_conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N, int H, int W, const void* wgt,
                            const float* bias, int Cout, int act, void* y, float* yf, void* stream) {
+    return rfk_nn_conv3x3_forward_ex(x1, c1, 0, x2, c2, N, H, W, wgt, bias, Cout, act, y, yf, stream);
+}
+
+int rfk_nn_conv3x3_forward_ex(const void* x1, int c1, int x1_mode, const void* x2, int c2, int N, int H, int W,
+                              const void* wgt, const float* bias, int Cout, int act, void* y, float* yf, void* stream) {
+    if (!x1 || !wgt || (!y && !yf)) return fail(RFK_ERR_INVALID, "null argument");
+    if (x1_mode < 0 || x1_mode > 2) return fail(RFK_ERR_INVALID, "x1_mode must be 0, 1 or 2");
     NN_HIP(rfk::nn::conv3x3_forward(x1, c1, x2, c2, N, H, W, wgt, bias, Cout, act, 0.f, y, yf,
-                                    static_cast<hipStream_t>(stream)));
+                                    static_cast<hipStream_t>(stream), x1_mode));
     return RFK_OK;
 }
 

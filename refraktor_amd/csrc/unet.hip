@@ -20,8 +20,6 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int KC = 32;       // input channels staged per step
 constexpr int PAD = 8;       // LDS row padding (bf16) against bank conflicts
-constexpr int TPX = 64;      // output pixels per workgroup (one row segment)
-constexpr int TCO = 64;      // output channels per workgroup
 
 __device__ __forceinline__ float act_fwd(float v, int act, float slope) {
     if (act == 1) return v > 0.f ? v : 0.f;
@@ -38,46 +36,76 @@ __device__ __forceinline__ bf16x8 zero8() {
 }
 
 // ------------------------------------------------------------- forward
-// Output tile: 64 consecutive pixels of one row x 64 output channels; wave w
-// computes pixels [32*(w&1), +32) x channels [32*(w>>1), +32) as one 32x32
-// MFMA accumulator. Input = virtual concat of x1 (c1 channels) and x2 (c2).
-__global__ __launch_bounds__(256) void conv3x3_fwd_kernel(const bf16* __restrict__ x1, int c1,
+// Output tile of a workgroup: 128*R consecutive pixels of one row x 32*C
+// output channels; wave w computes pixels [32R*w, +32R) x all 32C channels as
+// R x C 32x32 MFMA accumulators (A = pixels x K from the input halo, B =
+// K x channels from the weights, K = KCH input channels of one tap per step).
+// Input = virtual concat of x1 (c1 channels) and x2 (c2). x1 may be read
+// through a 2x upsample (x1 stored at H/2 x W/2, nearest) or a 2x2 average
+// pool (x1 stored at 2H x 2W), so the decoder and encoder skip the
+// intermediate tensors. Shapes: (R, C, KCH) = (2,1,32) and (2,1,16) for
+// Cout <= 32, (1,2,32) for <= 64, (1,3,16) for <= 96, (1,4,16) above.
+__device__ __forceinline__ bf16x8 pool8(const bf16* p00, const bf16* p01, const bf16* p10, const bf16* p11) {
+    const bf16x8 a = load8(p00), b = load8(p01), c = load8(p10), d = load8(p11);
+    bf16x8 o;
+#pragma unroll
+    for (int i = 0; i < 8; i++) o[i] = bf16(0.25f * (float(a[i]) + float(b[i]) + float(c[i]) + float(d[i])));
+    return o;
+}
+
+template <int R, int C, int KCH>
+__global__ __launch_bounds__(256) void conv3x3_fwd_kernel(const bf16* __restrict__ x1, int c1, int mode1,
                                                           const bf16* __restrict__ x2, int c2, int N, int H, int W,
                                                           const bf16* __restrict__ wgt, const float* __restrict__ bias,
                                                           int Cout, int act, float slope, bf16* __restrict__ y,
                                                           float* __restrict__ yf) {
-    __shared__ __attribute__((aligned(16))) bf16 s_in[3][TPX + 2][KC + PAD];
-    __shared__ __attribute__((aligned(16))) bf16 s_w[9][TCO][KC + PAD];
+    constexpr int BPX = 128 * R, TC = 32 * C, KP = KCH + PAD, G = KCH / 8;
+    __shared__ __attribute__((aligned(16))) bf16 s_in[3][BPX + 2][KP];
+    __shared__ __attribute__((aligned(16))) bf16 s_w[9][TC][KP];
     const int Cin = c1 + c2;
-    const int x0 = int(blockIdx.x) * TPX;
+    const int x0 = int(blockIdx.x) * BPX;
     const int yy = int(blockIdx.y);
-    const int ntile = (Cout + TCO - 1) / TCO;
+    const int ntile = (Cout + TC - 1) / TC;
     const int n = int(blockIdx.z) / ntile;
-    const int co0 = (int(blockIdx.z) % ntile) * TCO;
+    const int co0 = (int(blockIdx.z) % ntile) * TC;
     const int t = int(threadIdx.x);
     const int wv = t >> 6, l = t & 63;
-    const int ph = wv & 1, chh = wv >> 1;
     const int m = l & 31, kh = l >> 5;
+    const int H1 = mode1 == 1 ? H / 2 : (mode1 == 2 ? 2 * H : H), W1 = mode1 == 1 ? W / 2 : (mode1 == 2 ? 2 * W : W);
 
-    f32x16 acc;
+    f32x16 acc[R][C];
 #pragma unroll
-    for (int i = 0; i < 16; i++) acc[i] = 0.f;
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int c = 0; c < C; c++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) acc[r][c][i] = 0.f;
 
-    for (int c0 = 0; c0 < Cin; c0 += KC) {
-        // stage the input halo tile: 3 rows x 66 columns x 32 channels
-        for (int v = t; v < 3 * (TPX + 2) * (KC / 8); v += 256) {
-            const int g = v % (KC / 8), col = (v / (KC / 8)) % (TPX + 2), r = v / ((KC / 8) * (TPX + 2));
-            const int gy = yy + r - 1, gx = x0 + col - 1, c = c0 + g * 8;
+    for (int c0 = 0; c0 < Cin; c0 += KCH) {
+        // stage the input halo tile: 3 rows x (BPX + 2) columns x KCH channels
+        for (int v = t; v < 3 * (BPX + 2) * G; v += 256) {
+            const int g = v % G, col = (v / G) % (BPX + 2), rr = v / (G * (BPX + 2));
+            const int gy = yy + rr - 1, gx = x0 + col - 1, c = c0 + g * 8;
             bf16x8 val = zero8();
             if (gy >= 0 && gy < H && gx >= 0 && gx < W && c < Cin) {
-                const size_t pix = (size_t(n) * H + gy) * W + gx;
-                val = c < c1 ? load8(x1 + pix * c1 + c) : load8(x2 + pix * c2 + (c - c1));
+                if (c < c1) {
+                    if (mode1 == 1) {
+                        val = load8(x1 + ((size_t(n) * H1 + gy / 2) * W1 + gx / 2) * c1 + c);
+                    } else if (mode1 == 2) {
+                        const size_t r0 = (size_t(n) * H1 + 2 * gy) * W1 + 2 * gx, r1 = r0 + W1;
+                        val = pool8(x1 + r0 * c1 + c, x1 + (r0 + 1) * c1 + c, x1 + r1 * c1 + c, x1 + (r1 + 1) * c1 + c);
+                    } else {
+                        val = load8(x1 + ((size_t(n) * H + gy) * W + gx) * c1 + c);
+                    }
+                } else {
+                    val = load8(x2 + ((size_t(n) * H + gy) * W + gx) * c2 + (c - c1));
+                }
             }
-            *reinterpret_cast<bf16x8*>(&s_in[r][col][g * 8]) = val;
+            *reinterpret_cast<bf16x8*>(&s_in[rr][col][g * 8]) = val;
         }
         // stage weights [tap][co][ci]
-        for (int v = t; v < 9 * TCO * (KC / 8); v += 256) {
-            const int g = v % (KC / 8), co = (v / (KC / 8)) % TCO, tap = v / ((KC / 8) * TCO);
+        for (int v = t; v < 9 * TC * G; v += 256) {
+            const int g = v % G, co = (v / G) % TC, tap = v / (G * TC);
             const int c = c0 + g * 8, o = co0 + co;
             bf16x8 val = zero8();
             if (o < Cout && c < Cin) val = load8(wgt + (size_t(o) * 9 + tap) * Cin + c);
@@ -88,28 +116,41 @@ __global__ __launch_bounds__(256) void conv3x3_fwd_kernel(const bf16* __restrict
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap % 3;
 #pragma unroll
-            for (int ks = 0; ks < KC / 16; ks++) {
-                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&s_in[dy][ph * 32 + m + dx][ks * 16 + kh * 8]);
-                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&s_w[tap][chh * 32 + m][ks * 16 + kh * 8]);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            for (int ks = 0; ks < KCH / 16; ks++) {
+                bf16x8 b[C];
+#pragma unroll
+                for (int c = 0; c < C; c++) b[c] = *reinterpret_cast<const bf16x8*>(&s_w[tap][c * 32 + m][ks * 16 + kh * 8]);
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const bf16x8 av =
+                        *reinterpret_cast<const bf16x8*>(&s_in[dy][wv * 32 * R + r * 32 + m + dx][ks * 16 + kh * 8]);
+#pragma unroll
+                    for (int c = 0; c < C; c++) acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b[c], acc[r][c], 0, 0, 0);
+                }
             }
         }
         __syncthreads();
     }
     // D[row = pixel][col = channel]: lane holds col m, rows 8i + 4kh + j
-    const int co = co0 + chh * 32 + m;
-    if (co >= Cout) return;
-    const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int c = 0; c < C; c++) {
+        const int co = co0 + c * 32 + m;
+        if (co >= Cout) continue;
+        const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int px = x0 + ph * 32 + i * 8 + kh * 4 + j;
-            if (px >= W) continue;
-            const float v = act_fwd(acc[i * 4 + j] + bv, act, slope);
-            const size_t o = ((size_t(n) * H + yy) * W + px) * Cout + co;
-            if (y) y[o] = bf16(v);
-            if (yf) yf[o] = v;
+        for (int r = 0; r < R; r++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int px = x0 + wv * 32 * R + r * 32 + i * 8 + kh * 4 + j;
+                    if (px >= W) continue;
+                    const float v = act_fwd(acc[r][c][i * 4 + j] + bv, act, slope);
+                    const size_t o = ((size_t(n) * H + yy) * W + px) * Cout + co;
+                    if (y) y[o] = bf16(v);
+                    if (yf) yf[o] = v;
+                }
+            }
         }
     }
 }
@@ -413,12 +454,33 @@ unsigned nb(size_t n, unsigned b = 256) { return unsigned((n + b - 1) / b); }
 
 }  // namespace
 
+template <int R, int C, int KCH>
+static void launch_conv_fwd(const void* x1, int c1, int mode1, const void* x2, int c2, int N, int H, int W,
+                            const void* wgt, const float* bias, int Cout, int act, float slope, void* y, float* yf,
+                            hipStream_t s) {
+    dim3 grid((W + 128 * R - 1) / (128 * R), H, N * ((Cout + 32 * C - 1) / (32 * C)));
+    conv3x3_fwd_kernel<R, C, KCH><<<grid, 256, 0, s>>>((const bf16*)x1, c1, mode1, (const bf16*)x2, c2, N, H, W,
+                                                       (const bf16*)wgt, bias, Cout, act, slope, (bf16*)y, yf);
+}
+
 hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N, int H, int W, const void* wgt,
-                           const float* bias, int Cout, int act, float slope, void* y, float* yf, hipStream_t s) {
+                           const float* bias, int Cout, int act, float slope, void* y, float* yf, hipStream_t s,
+                           int x1_mode) {
     if ((c1 % 8) || (c2 % 8) || c1 <= 0 || c2 < 0 || Cout <= 0) return hipErrorInvalidValue;
-    dim3 grid((W + TPX - 1) / TPX, H, N * ((Cout + TCO - 1) / TCO));
-    conv3x3_fwd_kernel<<<grid, 256, 0, s>>>((const bf16*)x1, c1, (const bf16*)x2, c2, N, H, W, (const bf16*)wgt, bias,
-                                            Cout, act, slope, (bf16*)y, yf);
+    if (x1_mode < 0 || x1_mode > 2 || (x1_mode == 1 && ((H | W) & 1))) return hipErrorInvalidValue;
+    const int Cin = c1 + c2;
+    if (Cout <= 32) {
+        if (Cin <= 16)
+            launch_conv_fwd<2, 1, 16>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
+        else
+            launch_conv_fwd<2, 1, 32>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
+    } else if (Cout <= 64) {
+        launch_conv_fwd<1, 2, 32>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
+    } else if (Cout <= 96) {
+        launch_conv_fwd<1, 3, 16>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
+    } else {
+        launch_conv_fwd<1, 4, 16>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
+    }
     return hipGetLastError();
 }
 

@@ -9,10 +9,13 @@
 
 namespace rfk::nn {
 
-// y = act(conv3x3(concat(x1, x2)) + bias); c1, c2 multiples of 8 (c2 may be 0).
+// y = act(conv3x3(concat(x1', x2)) + bias); c1, c2 multiples of 8 (c2 may be 0).
 // act: 0 none, 1 ReLU, 2 leaky ReLU(slope). y (BF16) and/or yf (FP32) may be null.
+// x1_mode: 0 x1' = x1 (H x W), 1 x1' = nearest 2x upsample of x1 (H/2 x W/2),
+// 2 x1' = 2x2 average pool of x1 (2H x 2W).
 hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N, int H, int W, const void* wgt,
-                           const float* bias, int Cout, int act, float slope, void* y, float* yf, hipStream_t s);
+                           const float* bias, int Cout, int act, float slope, void* y, float* yf, hipStream_t s,
+                           int x1_mode = 0);
 // bytes of FP32 workspace conv3x3_wgrad needs
 size_t conv3x3_wgrad_workspace(int c1, int c2, int Cout, int N, int H, int W, int* groups, int* tiles_per_group);
 // dw = beta * dw + dL/dW (FP32, [Cout][9][Cin])

@@ -23,12 +23,14 @@ def _bf16_dev(a: np.ndarray) -> render.DeviceBuffer:
     return _dev(nn.bf16_bits(a.astype(np.float32)))
 
 
-CASES = [  # (N, H, W, c1, c2, Cout)
+CASES = [  # (N, H, W, c1, c2, Cout): every tile shape of the forward kernel
     (1, 5, 70, 8, 0, 32),
     (2, 9, 33, 32, 0, 64),
     (1, 6, 130, 64, 32, 96),
     (1, 4, 64, 128, 96, 96),
     (1, 8, 20, 32, 0, 8),
+    (1, 3, 300, 96, 0, 128),
+    (1, 2, 260, 64, 32, 32),
 ]
 
 
@@ -49,6 +51,47 @@ def test_conv_forward_matches_oracle(gpu, case):
     x = np.concatenate([x1, x2], axis=-1) if c2 else x1
     want = np.maximum(nn.conv3x3(x, w, b), 0)
     np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_conv_fused_upsample_and_pool(gpu, mode):
+    """x1 read through a nearest 2x upsample (1) or a 2x2 average pool (2)"""
+    rng = np.random.default_rng(40 + mode)
+    N, H, W, c1, c2, Co = 1, 6, 136, 32, 16, 64
+    if mode == 1:
+        x1 = nn.to_bf16(rng.normal(size=(N, H // 2, W // 2, c1)).astype(np.float32))
+        x1_eff = nn.up(x1)
+    else:
+        x1 = nn.to_bf16(rng.normal(size=(N, 2 * H, 2 * W, c1)).astype(np.float32))
+        x1_eff = nn.to_bf16(nn.pool(x1))
+    x2 = nn.to_bf16(rng.normal(size=(N, H, W, c2)).astype(np.float32))
+    w = nn.to_bf16((rng.normal(size=(Co, 9, c1 + c2)) * 0.1).astype(np.float32))
+    b = rng.normal(size=Co).astype(np.float32)
+    yf = render.DeviceBuffer(N * H * W * Co * 4)
+    d1, d2, dw, db_ = _bf16_dev(x1), _bf16_dev(x2), _bf16_dev(w), _dev(b)
+    _lib.check(_lib.lib().rfk_nn_conv3x3_forward_ex(d1.ptr, c1, mode, d2.ptr, c2, N, H, W, dw.ptr, db_.ptr, Co, 0,
+                                                   None, yf.ptr, None))
+    want = nn.conv3x3(np.concatenate([x1_eff, x2], axis=-1), w, b)
+    np.testing.assert_allclose(yf.download(np.float32, (N, H, W, Co)), want, rtol=1e-4, atol=1e-4)
+
+
+def test_unet_inference_matches_oracle(gpu):
+    """the fused inference forward of the whole network vs the float64 network"""
+    rng = np.random.default_rng(12)
+    N, H, W = 1, 24, 40
+    den = denoise.Denoiser(seed=5)
+    w, b = den.params()
+    b = (rng.normal(size=b.shape) * 0.01).astype(np.float32)
+    den.set_params(w, b)
+    feat = nn.to_bf16(rng.uniform(size=(N, H, W, nn.FEAT)).astype(np.float32))
+    base = rng.uniform(size=(N, H, W, 3)).astype(np.float32)
+    out = render.DeviceBuffer(N * H * W * 12)
+    den.forward(_bf16_dev(feat), _dev(base), N, H, W, out)
+    got = out.download(np.float32, (N, H, W, 3))
+    want = nn.unet_forward(feat.astype(np.float64), base, nn.to_bf16(w).astype(np.float64), b.astype(np.float64))
+    err = np.abs(got - want)
+    # BF16 activations: a few values sit on rounding boundaries
+    assert np.quantile(err, 0.99) < 2e-3 and err.max() < 2e-2, (np.quantile(err, 0.99), err.max())
 
 
 @pytest.mark.parametrize("case", CASES[:4])
