@@ -43,7 +43,7 @@ __device__ __forceinline__ bf16x8 zero8() {
 // Input = virtual concat of x1 (c1 channels) and x2 (c2). x1 may be read
 // through a 2x upsample (x1 stored at H/2 x W/2, nearest) or a 2x2 average
 // pool (x1 stored at 2H x 2W), so the decoder and encoder skip the
-// intermediate tensors. Shapes: (R, C, KCH) = (2,1,32) and (2,1,16) for
+// intermediate tensors. Shapes: (R, C, KCH) = (1,1,32) and (1,1,16) for
 // Cout <= 32, (1,2,32) for <= 64, (1,3,16) for <= 96, (1,4,16) above.
 __device__ __forceinline__ bf16x8 pool8(const bf16* p00, const bf16* p01, const bf16* p10, const bf16* p11) {
     const bf16x8 a = load8(p00), b = load8(p01), c = load8(p10), d = load8(p11);
@@ -81,13 +81,18 @@ __global__ __launch_bounds__(256) void conv3x3_fwd_kernel(const bf16* __restrict
 #pragma unroll
             for (int i = 0; i < 16; i++) acc[r][c][i] = 0.f;
 
+    // staging: every thread first issues all of its global loads, then writes
+    // them to LDS, so the loads of a chunk are in flight together
+    constexpr int NIN = (3 * (BPX + 2) * G + 255) / 256, NW = (9 * TC * G + 255) / 256;
     for (int c0 = 0; c0 < Cin; c0 += KCH) {
-        // stage the input halo tile: 3 rows x (BPX + 2) columns x KCH channels
-        for (int v = t; v < 3 * (BPX + 2) * G; v += 256) {
+        bf16x8 vin[NIN], vw[NW];
+#pragma unroll
+        for (int q = 0; q < NIN; q++) {
+            const int v = t + q * 256;
             const int g = v % G, col = (v / G) % (BPX + 2), rr = v / (G * (BPX + 2));
             const int gy = yy + rr - 1, gx = x0 + col - 1, c = c0 + g * 8;
             bf16x8 val = zero8();
-            if (gy >= 0 && gy < H && gx >= 0 && gx < W && c < Cin) {
+            if (rr < 3 && gy >= 0 && gy < H && gx >= 0 && gx < W && c < Cin) {
                 if (c < c1) {
                     if (mode1 == 1) {
                         val = load8(x1 + ((size_t(n) * H1 + gy / 2) * W1 + gx / 2) * c1 + c);
@@ -101,15 +106,28 @@ __global__ __launch_bounds__(256) void conv3x3_fwd_kernel(const bf16* __restrict
                     val = load8(x2 + ((size_t(n) * H + gy) * W + gx) * c2 + (c - c1));
                 }
             }
-            *reinterpret_cast<bf16x8*>(&s_in[rr][col][g * 8]) = val;
+            vin[q] = val;
         }
-        // stage weights [tap][co][ci]
-        for (int v = t; v < 9 * TC * G; v += 256) {
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            const int v = t + q * 256;
             const int g = v % G, co = (v / G) % TC, tap = v / (G * TC);
             const int c = c0 + g * 8, o = co0 + co;
             bf16x8 val = zero8();
-            if (o < Cout && c < Cin) val = load8(wgt + (size_t(o) * 9 + tap) * Cin + c);
-            *reinterpret_cast<bf16x8*>(&s_w[tap][co][g * 8]) = val;
+            if (tap < 9 && o < Cout && c < Cin) val = load8(wgt + (size_t(o) * 9 + tap) * Cin + c);
+            vw[q] = val;
+        }
+#pragma unroll
+        for (int q = 0; q < NIN; q++) {
+            const int v = t + q * 256;
+            const int g = v % G, col = (v / G) % (BPX + 2), rr = v / (G * (BPX + 2));
+            if (rr < 3) *reinterpret_cast<bf16x8*>(&s_in[rr][col][g * 8]) = vin[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            const int v = t + q * 256;
+            const int g = v % G, co = (v / G) % TC, tap = v / (G * TC);
+            if (tap < 9) *reinterpret_cast<bf16x8*>(&s_w[tap][co][g * 8]) = vw[q];
         }
         __syncthreads();
 #pragma unroll
@@ -471,9 +489,9 @@ hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
     const int Cin = c1 + c2;
     if (Cout <= 32) {
         if (Cin <= 16)
-            launch_conv_fwd<2, 1, 16>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
+            launch_conv_fwd<1, 1, 16>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
         else
-            launch_conv_fwd<2, 1, 32>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
+            launch_conv_fwd<1, 1, 32>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
     } else if (Cout <= 64) {
         launch_conv_fwd<1, 2, 32>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
     } else if (Cout <= 96) {
