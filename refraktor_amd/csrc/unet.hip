@@ -472,6 +472,122 @@ unsigned nb(size_t n, unsigned b = 256) { return unsigned((n + b - 1) / b); }
 
 }  // namespace
 
+// Narrow full-resolution layers (Cin = c1 + c2 <= KCH, Cout <= 32): a
+// workgroup walks T rows of a 128-pixel column strip with a ring of three
+// staged input rows, so every input row is loaded once per strip (not three
+// times) and the weights once per T rows; the next row's loads are in flight
+// while the current row's MFMAs run. x1 may be read through a 2x upsample.
+template <int KCH, int T>
+__global__ __launch_bounds__(256) void conv3x3_fwd_rows_kernel(const bf16* __restrict__ x1, int c1, int mode1,
+                                                               const bf16* __restrict__ x2, int c2, int N, int H,
+                                                               int W, const bf16* __restrict__ wgt,
+                                                               const float* __restrict__ bias, int Cout, int act,
+                                                               float slope, bf16* __restrict__ y,
+                                                               float* __restrict__ yf) {
+    const int Cin = c1 + c2;
+    constexpr int BPX = 128, KP = KCH + PAD, G = KCH / 8;
+    constexpr int NROW = ((BPX + 2) * G + 255) / 256, NW = (9 * 32 * G + 255) / 256;
+    __shared__ __attribute__((aligned(16))) bf16 s_in[3][BPX + 2][KP];
+    __shared__ __attribute__((aligned(16))) bf16 s_w[9][32][KP];
+    const int x0 = int(blockIdx.x) * BPX;
+    const int y0 = int(blockIdx.y) * T;
+    const int n = int(blockIdx.z);
+    const int t = int(threadIdx.x);
+    const int wv = t >> 6, l = t & 63, m = l & 31, kh = l >> 5;
+
+    auto load_row = [&](int gy, bf16x8* v) {
+#pragma unroll
+        for (int q = 0; q < NROW; q++) {
+            const int e = t + q * 256;
+            const int g = e % G, col = e / G;
+            const int gx = x0 + col - 1, c = g * 8;
+            bf16x8 val = zero8();
+            if (col < BPX + 2 && gy >= 0 && gy < H && gx >= 0 && gx < W && c < Cin) {
+                if (c >= c1)
+                    val = load8(x2 + ((size_t(n) * H + gy) * W + gx) * c2 + (c - c1));
+                else if (mode1 == 1)
+                    val = load8(x1 + ((size_t(n) * (H / 2) + gy / 2) * (W / 2) + gx / 2) * c1 + c);
+                else
+                    val = load8(x1 + ((size_t(n) * H + gy) * W + gx) * c1 + c);
+            }
+            v[q] = val;
+        }
+    };
+    auto store_row = [&](int slot, const bf16x8* v) {
+#pragma unroll
+        for (int q = 0; q < NROW; q++) {
+            const int e = t + q * 256;
+            const int g = e % G, col = e / G;
+            if (col < BPX + 2) *reinterpret_cast<bf16x8*>(&s_in[slot][col][g * 8]) = v[q];
+        }
+    };
+    {
+        bf16x8 vw[NW];
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            const int e = t + q * 256;
+            const int g = e % G, co = (e / G) % 32, tap = e / (G * 32);
+            bf16x8 val = zero8();
+            if (tap < 9 && co < Cout && g * 8 < Cin) val = load8(wgt + (size_t(co) * 9 + tap) * Cin + g * 8);
+            vw[q] = val;
+        }
+        bf16x8 r0[NROW], r1[NROW], r2[NROW];
+        load_row(y0 - 1, r0);
+        load_row(y0, r1);
+        load_row(y0 + 1, r2);
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            const int e = t + q * 256;
+            const int g = e % G, co = (e / G) % 32, tap = e / (G * 32);
+            if (tap < 9) *reinterpret_cast<bf16x8*>(&s_w[tap][co][g * 8]) = vw[q];
+        }
+        store_row((y0 + 2) % 3, r0);  // slot of row r = (r + 3) % 3
+        store_row(y0 % 3, r1);
+        store_row((y0 + 1) % 3, r2);
+    }
+    __syncthreads();
+    const int co = m;
+    const float bv = (bias && co < Cout) ? bias[co] : 0.f;
+    for (int r = 0; r < T; r++) {
+        const int yy = y0 + r;
+        if (yy >= H) break;  // uniform over the workgroup
+        bf16x8 nxt[NROW];
+        const bool more = r + 1 < T && yy + 1 < H;
+        if (more) load_row(yy + 2, nxt);
+        f32x16 acc;
+#pragma unroll
+        for (int i = 0; i < 16; i++) acc[i] = 0.f;
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap % 3;
+            const int slot = (yy + dy + 2) % 3;  // row yy + dy - 1
+#pragma unroll
+            for (int ks = 0; ks < KCH / 16; ks++) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&s_w[tap][m][ks * 16 + kh * 8]);
+                const bf16x8 av = *reinterpret_cast<const bf16x8*>(&s_in[slot][wv * 32 + m + dx][ks * 16 + kh * 8]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b, acc, 0, 0, 0);
+            }
+        }
+        if (co < Cout) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int px = x0 + wv * 32 + i * 8 + kh * 4 + j;
+                    if (px >= W) continue;
+                    const float v = act_fwd(acc[i * 4 + j] + bv, act, slope);
+                    const size_t o = ((size_t(n) * H + yy) * W + px) * Cout + co;
+                    if (y) y[o] = bf16(v);
+                    if (yf) yf[o] = v;
+                }
+            }
+        }
+        __syncthreads();  // every wave is done with row yy - 1's slot
+        if (more) store_row((yy + 2) % 3, nxt);  // row yy + 2 takes row yy - 1's slot
+        __syncthreads();
+    }
+}
+
 template <int R, int C, int KCH>
 static void launch_conv_fwd(const void* x1, int c1, int mode1, const void* x2, int c2, int N, int H, int W,
                             const void* wgt, const float* bias, int Cout, int act, float slope, void* y, float* yf,
@@ -487,6 +603,19 @@ hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
     if ((c1 % 8) || (c2 % 8) || c1 <= 0 || c2 < 0 || Cout <= 0) return hipErrorInvalidValue;
     if (x1_mode < 0 || x1_mode > 2 || (x1_mode == 1 && ((H | W) & 1))) return hipErrorInvalidValue;
     const int Cin = c1 + c2;
+    // (measured: the ring pays off while one staged row holds every channel
+    // of a 32-channel input; wider inputs run faster on the chunked kernel)
+    if (Cout <= 32 && x1_mode != 2 && Cin <= 32) {
+        constexpr int T = 8;
+        dim3 grid((W + 127) / 128, (H + T - 1) / T, N);
+#define RFK_ROWS(K)                                                                                            \
+    conv3x3_fwd_rows_kernel<K, T><<<grid, 256, 0, s>>>((const bf16*)x1, c1, x1_mode, (const bf16*)x2, c2, N, H, W, \
+                                                       (const bf16*)wgt, bias, Cout, act, slope, (bf16*)y, yf)
+        if (Cin <= 16) RFK_ROWS(16);
+        else RFK_ROWS(32);
+#undef RFK_ROWS
+        return hipGetLastError();
+    }
     if (Cout <= 32) {
         if (Cin <= 16)
             launch_conv_fwd<1, 1, 16>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);

@@ -31,6 +31,10 @@ CASES = [  # (N, H, W, c1, c2, Cout): every tile shape of the forward kernel
     (1, 8, 20, 32, 0, 8),
     (1, 3, 300, 96, 0, 128),
     (1, 2, 260, 64, 32, 32),
+    (1, 19, 300, 32, 0, 32),  # row-strip kernel: several strips and row groups
+    (2, 11, 140, 16, 0, 24),
+    (1, 10, 150, 64, 32, 32),
+    (1, 9, 131, 48, 0, 16),
 ]
 
 
@@ -53,11 +57,11 @@ def test_conv_forward_matches_oracle(gpu, case):
     np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-4)
 
 
-@pytest.mark.parametrize("mode", [1, 2])
-def test_conv_fused_upsample_and_pool(gpu, mode):
+@pytest.mark.parametrize("mode,Co", [(1, 64), (2, 64), (1, 32), (2, 32)])
+def test_conv_fused_upsample_and_pool(gpu, mode, Co):
     """x1 read through a nearest 2x upsample (1) or a 2x2 average pool (2)"""
-    rng = np.random.default_rng(40 + mode)
-    N, H, W, c1, c2, Co = 1, 6, 136, 32, 16, 64
+    rng = np.random.default_rng(40 + mode + Co)
+    N, H, W, c1, c2 = 1, 10, 136, 32, 16
     if mode == 1:
         x1 = nn.to_bf16(rng.normal(size=(N, H // 2, W // 2, c1)).astype(np.float32))
         x1_eff = nn.up(x1)
