@@ -182,9 +182,10 @@ struct level_shape {
 };
 
 // Forward pass; activations are kept under their names for the backward pass.
-// fused (inference): pools and upsamples are read through by the next
-// convolution instead of being stored; training keeps every activation for
-// the backward pass.
+// fused (inference): upsamples are read through by the next convolution
+// instead of being stored (the average pools stay separate: reading four
+// pixels per tap in the convolution's staging costs more than the pool);
+// training keeps every activation for the backward pass.
 int forward(rfk_denoiser* d, int N, int H, int W, const void* feat, float* rout, hipStream_t s, bool fused = false) {
     int rc = 0;
     const level_shape L[4] = {{H, W}, {H / 2, W / 2}, {H / 4, W / 4}, {H / 8, W / 8}};
@@ -201,9 +202,9 @@ int forward(rfk_denoiser* d, int N, int H, int W, const void* feat, float* rout,
     };
     const int* ch = d->ch;
     if (fused) {
-        void *a0 = A("a0", 0, ch[0]), *e0 = A("e0", 0, ch[0]);
-        void *a1 = A("a1", 1, ch[1]), *e1 = A("e1", 1, ch[1]);
-        void *a2 = A("a2", 2, ch[2]), *e2 = A("e2", 2, ch[2]);
+        void *a0 = A("a0", 0, ch[0]), *e0 = A("e0", 0, ch[0]), *p0 = A("p0", 1, ch[0]);
+        void *a1 = A("a1", 1, ch[1]), *e1 = A("e1", 1, ch[1]), *p1 = A("p1", 2, ch[1]);
+        void *a2 = A("a2", 2, ch[2]), *e2 = A("e2", 2, ch[2]), *p2 = A("p2", 3, ch[2]);
         void *a3 = A("a3", 3, ch[3]), *bb = A("b", 3, ch[3]);
         void *c2 = A("c2", 2, ch[2]), *d2 = A("d2", 2, ch[2]);
         void *c1 = A("c1", 1, ch[1]), *d1 = A("d1", 1, ch[1]);
@@ -211,11 +212,14 @@ int forward(rfk_denoiser* d, int N, int H, int W, const void* feat, float* rout,
         if (rc) return rc;
         if ((rc = conv(0, feat, nullptr, 0, a0, nullptr))) return rc;
         if ((rc = conv(1, a0, nullptr, 0, e0, nullptr))) return rc;
-        if ((rc = conv(2, e0, nullptr, 1, a1, nullptr, 2))) return rc;
+        NN_HIP(rfk::nn::avgpool2_forward(e0, N, H, W, ch[0], p0, s));
+        if ((rc = conv(2, p0, nullptr, 1, a1, nullptr))) return rc;
         if ((rc = conv(3, a1, nullptr, 1, e1, nullptr))) return rc;
-        if ((rc = conv(4, e1, nullptr, 2, a2, nullptr, 2))) return rc;
+        NN_HIP(rfk::nn::avgpool2_forward(e1, N, L[1].H, L[1].W, ch[1], p1, s));
+        if ((rc = conv(4, p1, nullptr, 2, a2, nullptr))) return rc;
         if ((rc = conv(5, a2, nullptr, 2, e2, nullptr))) return rc;
-        if ((rc = conv(6, e2, nullptr, 3, a3, nullptr, 2))) return rc;
+        NN_HIP(rfk::nn::avgpool2_forward(e2, N, L[2].H, L[2].W, ch[2], p2, s));
+        if ((rc = conv(6, p2, nullptr, 3, a3, nullptr))) return rc;
         if ((rc = conv(7, a3, nullptr, 3, bb, nullptr))) return rc;
         if ((rc = conv(8, bb, e2, 2, c2, nullptr, 1))) return rc;
         if ((rc = conv(9, c2, nullptr, 2, d2, nullptr))) return rc;

@@ -308,14 +308,15 @@ __global__ void act_bwd_kernel(const bf16* __restrict__ y, const bf16* __restric
     dx[g] = bf16(r);
 }
 
+// 8 channels of one output pixel per thread (C is a multiple of 8)
 __global__ void avgpool2_fwd_kernel(const bf16* __restrict__ x, int N, int H, int W, int C, bf16* __restrict__ y) {
-    const int Ho = H / 2, Wo = W / 2;
+    const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
     const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g >= size_t(N) * Ho * Wo * C) return;
-    const int c = int(g % C), xo = int((g / C) % Wo), yo = int((g / (size_t(C) * Wo)) % Ho),
-              n = int(g / (size_t(C) * Wo * Ho));
-    auto at = [&](int yy, int xx) { return float(x[((size_t(n) * H + yy) * W + xx) * C + c]); };
-    y[g] = bf16(0.25f * (at(2 * yo, 2 * xo) + at(2 * yo, 2 * xo + 1) + at(2 * yo + 1, 2 * xo) + at(2 * yo + 1, 2 * xo + 1)));
+    if (g >= size_t(N) * Ho * Wo * C8) return;
+    const int c = int(g % C8) * 8, xo = int((g / C8) % Wo), yo = int((g / (size_t(C8) * Wo)) % Ho),
+              n = int(g / (size_t(C8) * Wo * Ho));
+    const size_t r0 = (size_t(n) * H + 2 * yo) * W + 2 * xo, r1 = r0 + W;
+    *reinterpret_cast<bf16x8*>(y + g * 8) = pool8(x + r0 * C + c, x + (r0 + 1) * C + c, x + r1 * C + c, x + (r1 + 1) * C + c);
 }
 
 __global__ void avgpool2_bwd_kernel(const bf16* __restrict__ dy, int N, int H, int W, int C, bf16* __restrict__ dx) {
@@ -673,7 +674,8 @@ hipError_t act_backward(const void* y, const void* dy, size_t n, int act, float 
 
 hipError_t avgpool2_forward(const void* x, int N, int H, int W, int C, void* y, hipStream_t s) {
     size_t n = size_t(N) * (H / 2) * (W / 2) * C;
-    avgpool2_fwd_kernel<<<nb(n), 256, 0, s>>>((const bf16*)x, N, H, W, C, (bf16*)y);
+    if (C % 8) return hipErrorInvalidValue;
+    avgpool2_fwd_kernel<<<nb(n / 8), 256, 0, s>>>((const bf16*)x, N, H, W, C, (bf16*)y);
     return hipGetLastError();
 }
 
