@@ -1,13 +1,21 @@
-"""Benchmark: 1080p plain fractal-flame frames on one MI355X.
+"""Benchmark: 1080p fractal-flame frames on one MI355X.
 
-One step = one full frame of the 1920x1080 workload (8 xforms of 2-4
-variations + final xform, seeded synthetic flame): warmup (sample
-evaluation, precalc, walker seeding, 20 fuse passes), 2^30 binned iterations,
-density estimation (radius 9) and a log-density tonemap to half4.
+Workload: the 1920x1080 flame (8 xforms of 2-4 variations + final xform,
+seeded synthetic flame). Two paths, each timed over --steps frames:
+  plain     warmup (sample evaluation, precalc, walker seeding, 20 fuse
+            passes), 2^30 binned iterations, density estimation (radius 9),
+            log-density tonemap to half4;
+  denoised  the same pipeline at 2^26 iterations, then the U-Net (features,
+            forward) to RGB.
+Both are scored by PSNR against a converged 2^32-iteration render of the same
+flame. `value` is the frames/s of the denoised path when it reaches 40 dB
+(else of the plain path), the metric "1080p frames/sec at PSNR>=40dB".
 
-Prints one JSON line: frames/s (value), iterations/s, PSNR of the frame vs a
-converged 2^32-iteration render, the iterate kernel's roofline and the host
-baseline (the oracle's OpenMP render of the same flame, on a bounded sample).
+Prints one JSON line with both paths, iterations/s, the binning roofline
+(16 B of histogram increment per draw over the bin call's device time; the
+HBM traffic of the same call from the committed rocprofv3 counter summary)
+and the host baseline (the oracle's OpenMP render of the same flame on a
+bounded sample).
 """
 from __future__ import annotations
 
@@ -43,9 +51,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tpb", type=int, default=1024)
-    ap.add_argument("--tile", type=int, default=12, help="LDS bin cache slots (log2), 0 = off")
+    ap.add_argument("--binning", choices=("queue", "atomic"), default="queue",
+                    help="queue: records sorted by screen tile and summed in LDS; atomic: LDS bin cache + atomics")
+    ap.add_argument("--tile", type=int, default=12, help="LDS bin cache slots (log2) of atomic binning, 0 = off")
     ap.add_argument("--atomic-width", type=int, default=64)
-    ap.add_argument("--queue", action="store_true", help="queue binning (records sorted by screen tile)")
     ap.add_argument("--hist-bits", type=int, default=32)
     ap.add_argument("--walkers", type=int, default=1 << 20)
     ap.add_argument("--low-iters", type=int, default=1 << 26, help="budget of the denoised path")
@@ -73,11 +82,12 @@ def main() -> int:
     DE_R = 9.0
     f = fl.flame_1080(args.seed)
     db = render.VariationDB()
-    if args.queue:
+    queue = args.binning == "queue"
+    if queue:
         args.tile = 0
     k = db.compile(f, threads_per_block=args.tpb, atomic_width=args.atomic_width, bin_tile_log2=args.tile,
-                   hist_bits=args.hist_bits, bin_queue=args.queue)
-    k64 = db.compile(f, threads_per_block=args.tpb, bin_tile_log2=args.tile, hist_bits=64, bin_queue=args.queue)
+                   hist_bits=args.hist_bits, bin_queue=queue)
+    k64 = db.compile(f, threads_per_block=args.tpb, bin_tile_log2=args.tile, hist_bits=64, bin_queue=queue)
     samples = k.pack_samples(W, H)
     gamma, bright, vib = f["gamma"], f["brightness"], f["vibrancy"]
     lib = _lib.lib()
@@ -149,7 +159,9 @@ def main() -> int:
         if den_img is not None:
             psnr_den = psnr(den_img, ref)
 
-    # iterate kernel: algorithmic bytes = 16 B per draw (one RGBA u32 increment)
+    # binning: algorithmic bytes = 16 B per draw (one RGBA u32 increment) over
+    # the bin call's device time (HIP events on the state's stream: the
+    # iterate launches and, with queue binning, the sort and tile-sum kernels)
     achieved_gbs = dr * 16.0 / (kms * 1e-3) / 1e9
     it_per_s = ps / (kms * 1e-3)
     peak_inc = 0.0
@@ -159,6 +171,7 @@ def main() -> int:
         peak_inc = v.value
     except Exception:
         pass
+    traffic = measured_traffic(args, queue)
 
     cpu = None
     if not args.no_cpu:
@@ -185,7 +198,7 @@ def main() -> int:
                                "DE radius 9, log-density tonemap, vibrancy 1",
                    "width": W, "height": H, "iterations": ITERS, "walkers": args.walkers,
                    "threads_per_block": args.tpb, "atomic_width": args.atomic_width, "bin_tile_log2": args.tile,
-                   "hist_bits": args.hist_bits, "binning": "queue" if args.queue else "atomic",
+                   "hist_bits": args.hist_bits, "binning": args.binning,
                    "path": "denoised" if use_den else "plain"},
         "paths": {"plain": {"fps": fps_plain, "ms": ms_plain, "iterations": ITERS, "psnr_db": psnr_plain},
                   "denoised": None if fps_den is None else
@@ -195,14 +208,33 @@ def main() -> int:
         "iterate_kernel_ms": kms,
         "psnr_db_vs_converged": psnr_den if use_den else psnr_plain,
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "rfk_iterate", "bytes_per_draw": 16, "draws_per_launch": dr,
-                     "atomic_peak_increments_per_s": peak_inc,
-                     "atomic_frac": (dr * 4.0 / (kms * 1e-3) / peak_inc) if peak_inc else None},
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "rfk_iterate" + (" + queue sort/bin kernels" if queue else ""),
+                     "unit_of_work": "one bin call of 2^30 iterations", "bytes_per_draw": 16,
+                     "draws_per_launch": dr, "launch_ms": kms,
+                     "atomic_peak_increments_per_s": peak_inc},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def measured_traffic(args, queue: bool):
+    """HBM bytes of one 2^30-iteration bin call from the committed rocprofv3
+    counter summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated against
+    a plain device copy), when it was measured with this launch shape"""
+    path = ROOT / "profiles" / "traffic_1080p_bin.json"
+    try:
+        t = json.loads(path.read_text())
+    except (OSError, ValueError):
+        return None
+    shape = {"binning": args.binning, "threads_per_block": args.tpb, "walkers": args.walkers,
+             "hist_bits": args.hist_bits, "atomic_width": args.atomic_width}
+    if not queue:
+        shape["bin_tile_log2"] = args.tile
+    if any(t.get("shape", {}).get(k) != v for k, v in shape.items()):
+        return None
+    return t.get("bytes_per_bin_call")
 
 
 def cpu_baseline(f: dict, W: int, H: int, seed: int, it_per_frame: int) -> dict:
