@@ -254,7 +254,9 @@ int rfk_density_histogram(rfk_state* s, uint64_t* bins97 /* host, 97 */, double*
 /* Density estimation (no reference counterpart: flam3-style variable-width
  * filter). Writes a float4 RGBA image in draw units (device, w*h*4 floats).
  * Radius h = max_radius / n^curve clamped to >= min_radius, n = bin density;
- * kernel exp(-2 (r/h)^2), r <= h, normalised per source bin.                */
+ * kernel exp(-2 (r/h)^2), r <= h, normalised per source bin. Enqueued on the
+ * state's stream: the image is complete once that stream has drained
+ * (rfk_synchronize, or any later work on the stream).                       */
 int rfk_density_estimate(rfk_state* s, float max_radius, float min_radius, float curve, float* out_rgba);
 
 /* rfkt::tonemapper::run (src/librefrakt/include/librefrakt/image/tonemapper.h:18-22,
@@ -271,14 +273,16 @@ typedef struct rfk_tonemap_args {
     double vibrancy;
     double max_density;  /* informational, as in the reference                */
 } rfk_tonemap_args;
-/* Tonemap the state's fixed-point histogram. */
+/* Tonemap the state's fixed-point histogram, on the state's stream. Like
+ * tonemapper::run the call only enqueues the work.                          */
 int rfk_tonemap(rfk_state* s, void* out /* device */, int pixel_format, const rfk_tonemap_args* a);
-/* Tonemap a float4 RGBA image in draw units (e.g. the density-estimated one). */
 /* density_hdr (assets/kernels/tonemap.cu:159-233): remap a
  * tonemapped float4 / half4 image (device) by the CDF of the render's
  * 97-bin density histogram (host, from rfk_density_histogram) into float4. */
 int rfk_density_hdr(const void* tonemapped, int pixel_format, size_t npix, const uint64_t* bins97 /* host */,
                     float* out_rgba, void* stream);
+/* Tonemap a float4 RGBA image in draw units (e.g. the density-estimated
+ * one), enqueued on `stream`.                                               */
 int rfk_tonemap_image(const float* in_rgba, uint32_t width, uint32_t height, void* out, int pixel_format,
                       const rfk_tonemap_args* a, void* stream);
 

@@ -83,6 +83,11 @@ struct rfk_state {
     uint32_t *qcounts = nullptr, *qnseg = nullptr, *qtiles = nullptr;
     void* qsegments = nullptr;
     uint32_t chunk_passes = 0, max_segments = 0;
+    // density estimation: per-level 1/Z table for the last (max, min) radius
+    // pair, and the per-pixel level scratch
+    float* de_inv_z = nullptr;
+    int* de_levels = nullptr;
+    float de_max_radius = -1.0f, de_min_radius = -1.0f;
 };
 
 struct rfk_batch {
@@ -147,7 +152,7 @@ void free_state_buffers(rfk_state* s) {
     auto f = [](void* p) { if (p) (void)hipFree(p); };
     f(s->params); f(s->palettes); f(s->wx); f(s->wy); f(s->wc); f(s->wlast); f(s->block_pass); f(s->hist);
     f(s->queue); f(s->sorted); f(s->qcounts); f(s->qnseg); f(s->qsegments); f(s->qtiles);
-    f(s->d_samples); f(s->d_segments); f(s->d_kind);
+    f(s->d_samples); f(s->d_segments); f(s->d_kind); f(s->de_inv_z); f(s->de_levels);
     if (s->ev_staging) (void)hipEventSynchronize(s->ev_staging);
     if (s->staging) (void)hipHostFree(s->staging);
     if (s->ev_staging) (void)hipEventDestroy(s->ev_staging);
@@ -1131,35 +1136,32 @@ int rfk_density_estimate(rfk_state* s, float max_radius, float min_radius, float
     const float hmax = std::max(max_radius, min_radius);
     const int nlevels = int(hmax * 16.0f + 0.5f) + 1;
     const int radius = (nlevels - 1 + 15) / 16;
-    // per-level normalisation 1/Z(h), Z(h) = sum over |v| <= h of exp(-2|v|^2/h^2)
-    std::vector<float> inv_z(static_cast<size_t>(nlevels), 0.0f);
-    for (int l = 0; l < nlevels; l++) {
-        double h = l / 16.0, z = 0.0;
-        for (int dy = -radius; dy <= radius; dy++)
-            for (int dx = -radius; dx <= radius; dx++) {
-                double d2 = double(dx * dx + dy * dy);
-                if (d2 > h * h) continue;
-                z += d2 == 0.0 ? 1.0 : std::exp(-2.0 * d2 / (h * h));
-            }
-        inv_z[size_t(l)] = float(1.0 / z);
-    }
+    constexpr int kMaxLevels = 20 * 16 + 1;
     const size_t npix = size_t(s->width) * s->height;
-    float* d_iz = nullptr;
-    int* d_lvl = nullptr;
-    RFK_HIP(hipMalloc(&d_iz, inv_z.size() * sizeof(float)));
-    hipError_t e = hipMalloc(&d_lvl, npix * sizeof(int));
-    if (e != hipSuccess) {
-        (void)hipFree(d_iz);
-        return fail(RFK_ERR_HIP, hipGetErrorString(e));
+    if (!s->de_levels) RFK_HIP(dalloc(&s->de_levels, npix));
+    if (!s->de_inv_z) RFK_HIP(dalloc(&s->de_inv_z, size_t(kMaxLevels)));
+    if (max_radius != s->de_max_radius || min_radius != s->de_min_radius) {
+        // per-level normalisation 1/Z(h), Z(h) = sum over |v| <= h of exp(-2|v|^2/h^2)
+        std::vector<float> inv_z(static_cast<size_t>(nlevels), 0.0f);
+        for (int l = 0; l < nlevels; l++) {
+            double h = l / 16.0, z = 0.0;
+            for (int dy = -radius; dy <= radius; dy++)
+                for (int dx = -radius; dx <= radius; dx++) {
+                    double d2 = double(dx * dx + dy * dy);
+                    if (d2 > h * h) continue;
+                    z += d2 == 0.0 ? 1.0 : std::exp(-2.0 * d2 / (h * h));
+                }
+            inv_z[size_t(l)] = float(1.0 / z);
+        }
+        // the table may still be read by an earlier estimate on this stream
+        RFK_HIP(hipStreamSynchronize(s->stream));
+        RFK_HIP(hipMemcpy(s->de_inv_z, inv_z.data(), inv_z.size() * sizeof(float), hipMemcpyHostToDevice));
+        s->de_max_radius = max_radius;
+        s->de_min_radius = min_radius;
     }
-    e = hipMemcpyAsync(d_iz, inv_z.data(), inv_z.size() * sizeof(float), hipMemcpyHostToDevice, s->stream);
-    if (e == hipSuccess)
-        e = rfk::sk::density_estimate(s->hist, s->hist64, s->width, s->height, max_radius, min_radius, curve, d_iz, nlevels,
-                                      radius, d_lvl, out_rgba, s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    (void)hipFree(d_iz);
-    (void)hipFree(d_lvl);
-    if (e != hipSuccess) return fail(RFK_ERR_HIP, hipGetErrorString(e));
+    // enqueued on the state's stream; the call returns before it runs
+    RFK_HIP(rfk::sk::density_estimate(s->hist, s->hist64, s->width, s->height, max_radius, min_radius, curve,
+                                      s->de_inv_z, nlevels, radius, s->de_levels, out_rgba, s->stream));
     return RFK_OK;
 }
 
@@ -1173,7 +1175,6 @@ static int tonemap_common(const void* in, int in_kind, size_t npix, void* out, i
     float k = std::pow(10.0f, -std::log10(float(q)) - 0.5f);
     RFK_HIP(rfk::sk::tonemap(in, in_kind, npix, out, fmt, k, float(a->brightness), float(a->gamma), float(a->vibrancy),
                              stream));
-    RFK_HIP(hipStreamSynchronize(stream));
     return RFK_OK;
 }
 

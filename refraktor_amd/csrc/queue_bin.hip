@@ -148,15 +148,27 @@ __global__ void __launch_bounds__(kThreads)
     constexpr uint32_t kPer = kPiece / kThreads;
     const uint32_t per = (ntiles + kThreads - 1) / kThreads;
     const uint32_t lo = threadIdx.x * per, hi = std::min(ntiles, lo + per);
-    for (size_t base = b0; base < b1; base += kPiece) {
-        for (uint32_t t = threadIdx.x; t < ntiles; t += kThreads) lcount[t] = 0;
-        __syncthreads();
-        u64 r[kPer];
-        uint32_t rank[kPer];
+    // all loads of a piece are issued before any of them is used, and the
+    // next piece's loads are in flight while this one is ranked and written
+    u64 nxt[kPer];
+    auto load_piece = [&](size_t base) {
 #pragma unroll
         for (uint32_t j = 0; j < kPer; j++) {
             const size_t i = base + size_t(j) * kThreads + threadIdx.x;
-            r[j] = i < b1 ? rec[i] : ~0ull;
+            nxt[j] = i < b1 ? rec[i] : ~0ull;
+        }
+    };
+    load_piece(b0);
+    for (size_t base = b0; base < b1; base += kPiece) {
+        for (uint32_t t = threadIdx.x; t < ntiles; t += kThreads) lcount[t] = 0;
+        u64 r[kPer];
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; j++) r[j] = nxt[j];
+        if (base + kPiece < b1) load_piece(base + kPiece);
+        __syncthreads();
+        uint32_t rank[kPer];
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; j++) {
             const uint32_t tile = rec_tile(r[j]);
             rank[j] = wave_count(lcount, tile, tile < ntiles);
         }
@@ -184,7 +196,6 @@ __global__ void __launch_bounds__(kThreads)
         }
         __syncthreads();
         for (uint32_t t = threadIdx.x; t < ntiles; t += kThreads) cursor[t] += lcount[t];
-        __syncthreads();
     }
 }
 
@@ -214,35 +225,47 @@ __device__ __forceinline__ void add_bin(HistT* __restrict__ hist, size_t p, uint
     if (w) atomicAdd(h + 3, HistT(w));
 }
 
+// LDS sums of a work item: (r | g << 32) and (b | w << 32) per pixel. A work
+// item holds at most 2^18 records of 8-bit channels, so each 32-bit half
+// stays below 2^26 and never carries into its neighbour.
 template <class HistT>
 __global__ void __launch_bounds__(kThreads)
     qbin_kernel(const u64* __restrict__ sorted, const uint4* __restrict__ segments, const uint32_t* __restrict__ nseg,
                 uint32_t width, uint32_t height, uint32_t tiles_x, HistT* __restrict__ hist, bool packed64) {
-    __shared__ uint32_t acc[4][kTilePix];
+    static_assert(kSegment <= (1u << 24), "32-bit halves of the LDS sums must not carry");
+    __shared__ u64 acc[2][kTilePix];
     if (blockIdx.x >= nseg[0]) return;
     const uint4 sg = segments[blockIdx.x];
     for (uint32_t p = threadIdx.x; p < kTilePix; p += kThreads) {
-        acc[0][p] = 0u;
-        acc[1][p] = 0u;
-        acc[2][p] = 0u;
-        acc[3][p] = 0u;
+        acc[0][p] = 0ull;
+        acc[1][p] = 0ull;
     }
     __syncthreads();
-    for (uint32_t i = sg.y + threadIdx.x; i < sg.z; i += kThreads) {
-        const u64 v = sorted[i];
-        const uint32_t local = uint32_t(v >> 32) & (kTilePix - 1u);
-        const uint32_t c = uint32_t(v);
-        atomicAdd(&acc[0][local], c & 0xFFu);
-        atomicAdd(&acc[1][local], (c >> 8) & 0xFFu);
-        atomicAdd(&acc[2][local], (c >> 16) & 0xFFu);
-        atomicAdd(&acc[3][local], c >> 24);
+    // U records per thread in flight before the first LDS add
+    constexpr uint32_t U = 8;
+    for (uint32_t i0 = sg.y + threadIdx.x; i0 < sg.z; i0 += kThreads * U) {
+        u64 v[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t i = i0 + u * kThreads;
+            v[u] = i < sg.z ? sorted[i] : 0ull;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            if (i0 + u * kThreads >= sg.z) break;
+            const uint32_t local = uint32_t(v[u] >> 32) & (kTilePix - 1u);
+            const uint32_t c = uint32_t(v[u]);
+            atomicAdd(&acc[0][local], u64(c & 0xFFu) | (u64((c >> 8) & 0xFFu) << 32));
+            atomicAdd(&acc[1][local], u64((c >> 16) & 0xFFu) | (u64(c >> 24) << 32));
+        }
     }
     __syncthreads();
     const uint32_t tx = sg.x % tiles_x, ty = sg.x / tiles_x;
     const bool exclusive = sg.w != 0u;
     for (uint32_t p = threadIdx.x; p < kTilePix; p += kThreads) {
-        const uint32_t r = acc[0][p], g = acc[1][p], b = acc[2][p], w = acc[3][p];
-        if ((r | g | b | w) == 0u) continue;
+        const u64 rg = acc[0][p], bw = acc[1][p];
+        if ((rg | bw) == 0ull) continue;
+        const uint32_t r = uint32_t(rg), g = uint32_t(rg >> 32), b = uint32_t(bw), w = uint32_t(bw >> 32);
         const uint32_t x = (tx << kTileLog2) + (p & 63u), y = (ty << kTileLog2) + (p >> kTileLog2);
         if (x < width && y < height) add_bin(hist, size_t(y) * width + x, r, g, b, w, packed64, exclusive);
     }
