@@ -50,13 +50,15 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--tpb", type=int, default=1024)
+    ap.add_argument("--tpb", type=int, default=512)
     ap.add_argument("--binning", choices=("queue", "atomic"), default="queue",
                     help="queue: records sorted by screen tile and summed in LDS; atomic: LDS bin cache + atomics")
     ap.add_argument("--tile", type=int, default=12, help="LDS bin cache slots (log2) of atomic binning, 0 = off")
     ap.add_argument("--atomic-width", type=int, default=64)
     ap.add_argument("--hist-bits", type=int, default=32)
-    ap.add_argument("--walkers", type=int, default=1 << 20)
+    ap.add_argument("--walkers", type=int, default=1 << 20, help="walkers of the plain path")
+    ap.add_argument("--low-walkers", type=int, default=1 << 19,
+                    help="walkers of the denoised path (its small budget pays less warmup with fewer)")
     ap.add_argument("--low-iters", type=int, default=1 << 26, help="budget of the denoised path")
     ap.add_argument("--weights", type=str, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
@@ -97,10 +99,10 @@ def main() -> int:
     states, images = {}, {}
     de = render.DeviceBuffer(npx * 16)
 
-    def plain(seed: int, kern=k, iters=ITERS, fmt=_lib.PIXEL_HALF4):
-        st = states.get(id(kern))
+    def plain(seed: int, kern=k, iters=ITERS, fmt=_lib.PIXEL_HALF4, walkers=args.walkers):
+        st = states.get((id(kern), walkers))
         if st is None:
-            st = states[id(kern)] = kern.warmup(samples, W, H, seed=seed, fuse=20, walkers=args.walkers)
+            st = states[(id(kern), walkers)] = kern.warmup(samples, W, H, seed=seed, fuse=20, walkers=walkers)
         else:
             st.rewarm(samples, seed=seed, fuse=20)
         res = st.bin(iters=iters)
@@ -121,7 +123,7 @@ def main() -> int:
     rgb = render.DeviceBuffer(npx * 12)
 
     def denoised(seed: int):
-        st, res, tm = plain(seed, k, args.low_iters, _lib.PIXEL_FLOAT4)
+        st, res, tm = plain(seed, k, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers)
         inf = st.info()
         check(lib.rfk_denoiser_features(inf.histogram, inf.hist_bits, tm.ptr, npx, feat.ptr, base.ptr, inf.stream))
         den.forward(feat, base, 1, H, W, rgb, stream=inf.stream)
@@ -202,7 +204,8 @@ def main() -> int:
                    "path": "denoised" if use_den else "plain"},
         "paths": {"plain": {"fps": fps_plain, "ms": ms_plain, "iterations": ITERS, "psnr_db": psnr_plain},
                   "denoised": None if fps_den is None else
-                  {"fps": fps_den, "ms": ms_den, "iterations": args.low_iters, "psnr_db": psnr_den,
+                  {"fps": fps_den, "ms": ms_den, "iterations": args.low_iters, "walkers": args.low_walkers,
+                   "psnr_db": psnr_den,
                    "speedup_vs_plain": fps_den / fps_plain, "weights": str(weights.name)}},
         "iterations_per_s": it_per_s,
         "iterate_kernel_ms": kms,

@@ -6,7 +6,9 @@
 // counterpart and is checked against the oracle's float64 convolution.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <unordered_map>
 
 #include "unet_kernels.hpp"
 
@@ -43,7 +45,9 @@ __device__ __forceinline__ bf16x8 zero8() {
 // Input = virtual concat of x1 (c1 channels) and x2 (c2). x1 may be read
 // through a 2x upsample (x1 stored at H/2 x W/2, nearest) or a 2x2 average
 // pool (x1 stored at 2H x 2W), so the decoder and encoder skip the
-// intermediate tensors. Shapes: (R, C, KCH) = (1,1,32) and (1,1,16) for
+// intermediate tensors. It serves the layers the strip kernel below does not
+// take (more than 96 output channels, or inputs too wide for the strip
+// kernel's weights in LDS). Shapes: (R, C, KCH) = (1,1,32) and (1,1,16) for
 // Cout <= 32, (1,2,32) for <= 64, (1,3,16) for <= 96, (1,4,16) above.
 __device__ __forceinline__ bf16x8 pool8(const bf16* p00, const bf16* p01, const bf16* p10, const bf16* p11) {
     const bf16x8 a = load8(p00), b = load8(p01), c = load8(p10), d = load8(p11);
@@ -473,120 +477,211 @@ unsigned nb(size_t n, unsigned b = 256) { return unsigned((n + b - 1) / b); }
 
 }  // namespace
 
-// Narrow full-resolution layers (Cin = c1 + c2 <= KCH, Cout <= 32): a
-// workgroup walks T rows of a 128-pixel column strip with a ring of three
-// staged input rows, so every input row is loaded once per strip (not three
-// times) and the weights once per T rows; the next row's loads are in flight
-// while the current row's MFMAs run. x1 may be read through a 2x upsample.
-template <int KCH, int T>
-__global__ __launch_bounds__(256) void conv3x3_fwd_rows_kernel(const bf16* __restrict__ x1, int c1, int mode1,
-                                                               const bf16* __restrict__ x2, int c2, int N, int H,
-                                                               int W, const bf16* __restrict__ wgt,
-                                                               const float* __restrict__ bias, int Cout, int act,
-                                                               float slope, bf16* __restrict__ y,
-                                                               float* __restrict__ yf) {
+// Every wave walks its own 32-pixel column strip down `rows` output rows for
+// 32 output channels (grid.z = images x 32-channel tiles), input-stationary:
+// each staged input
+// row (one KC-channel chunk at a time, from registers loaded one step ahead)
+// feeds the three output rows it touches, held in three rotating
+// accumulators, so an input row is read from LDS once per (dx, k-step)
+// instead of three times, and waves never wait for each other after the
+// weights are staged. D = W (32 out channels x K) * X (K x 32 pixels): a lane
+// ends up with 16 channels of one pixel and stores them as 4-channel vectors.
+template <int KC>
+__global__ __launch_bounds__(256) void conv3x3_strip_kernel(const bf16* __restrict__ x1, int c1, int mode1,
+                                                            const bf16* __restrict__ x2, int c2, int N, int H,
+                                                            int W, const bf16* __restrict__ wgt,
+                                                            const float* __restrict__ bias, int Cout, int act,
+                                                            float slope, int rows, bf16* __restrict__ y,
+                                                            float* __restrict__ yf) {
+    constexpr int SPX = 34, XP = KC + PAD, G = KC / 8, NQ = (SPX * G + 63) / 64;
+    extern __shared__ __attribute__((aligned(16))) unsigned char conv_smem[];
     const int Cin = c1 + c2;
-    constexpr int BPX = 128, KP = KCH + PAD, G = KCH / 8;
-    constexpr int NROW = ((BPX + 2) * G + 255) / 256, NW = (9 * 32 * G + 255) / 256;
-    __shared__ __attribute__((aligned(16))) bf16 s_in[3][BPX + 2][KP];
-    __shared__ __attribute__((aligned(16))) bf16 s_w[9][32][KP];
-    const int x0 = int(blockIdx.x) * BPX;
-    const int y0 = int(blockIdx.y) * T;
-    const int n = int(blockIdx.z);
-    const int t = int(threadIdx.x);
-    const int wv = t >> 6, l = t & 63, m = l & 31, kh = l >> 5;
+    const int nch = (Cin + KC - 1) / KC, CinP = nch * KC, WP = CinP + PAD, GW = CinP / 8;
+    const int t = int(threadIdx.x), wv = t >> 6, l = t & 63, m = l & 31, kh = l >> 5;
+    bf16* s_w = reinterpret_cast<bf16*>(conv_smem);  // [9 taps][32 out][WP]
+    bf16* s_x = s_w + 9 * 32 * WP + wv * SPX * XP;   // this wave's staged chunk [SPX][XP]
+    const int x0 = int(blockIdx.x) * 128 + wv * 32;  // the wave's first output pixel
+    const int y0 = int(blockIdx.y) * rows;
+    const int ntile = (Cout + 31) / 32;
+    const int n = int(blockIdx.z) / ntile, co0 = (int(blockIdx.z) % ntile) * 32;
+    const int H1 = mode1 == 1 ? H / 2 : (mode1 == 2 ? 2 * H : H), W1 = mode1 == 1 ? W / 2 : (mode1 == 2 ? 2 * W : W);
 
-    auto load_row = [&](int gy, bf16x8* v) {
+    for (int e = t; e < 9 * 32 * GW; e += 256) {
+        const int g = e % GW, co = (e / GW) % 32, tap = e / (32 * GW);
+        bf16x8 v = zero8();
+        if (co0 + co < Cout && g * 8 < Cin) v = load8(wgt + (size_t(co0 + co) * 9 + tap) * Cin + g * 8);
+        *reinterpret_cast<bf16x8*>(s_w + (tap * 32 + co) * WP + g * 8) = v;
+    }
+    // (input row jj of the strip = image row y0 - 1 + jj, chunk ch) -> registers
+    auto load_chunk = [&](int jj, int ch, bf16x8* v) {
+        const int gy = y0 - 1 + jj;
 #pragma unroll
-        for (int q = 0; q < NROW; q++) {
-            const int e = t + q * 256;
+        for (int q = 0; q < NQ; q++) {
+            const int e = l + q * 64;
             const int g = e % G, col = e / G;
-            const int gx = x0 + col - 1, c = g * 8;
+            const int gx = x0 - 1 + col, c = ch * KC + g * 8;
             bf16x8 val = zero8();
-            if (col < BPX + 2 && gy >= 0 && gy < H && gx >= 0 && gx < W && c < Cin) {
-                if (c >= c1)
+            if (col < SPX && gy >= 0 && gy < H && gx >= 0 && gx < W && c < Cin) {
+                if (c >= c1) {
                     val = load8(x2 + ((size_t(n) * H + gy) * W + gx) * c2 + (c - c1));
-                else if (mode1 == 1)
-                    val = load8(x1 + ((size_t(n) * (H / 2) + gy / 2) * (W / 2) + gx / 2) * c1 + c);
-                else
+                } else if (mode1 == 1) {
+                    val = load8(x1 + ((size_t(n) * H1 + gy / 2) * W1 + gx / 2) * c1 + c);
+                } else if (mode1 == 2) {
+                    const size_t r0 = (size_t(n) * H1 + 2 * gy) * W1 + 2 * gx, r1 = r0 + W1;
+                    val = pool8(x1 + r0 * c1 + c, x1 + (r0 + 1) * c1 + c, x1 + r1 * c1 + c, x1 + (r1 + 1) * c1 + c);
+                } else {
                     val = load8(x1 + ((size_t(n) * H + gy) * W + gx) * c1 + c);
+                }
             }
             v[q] = val;
         }
     };
-    auto store_row = [&](int slot, const bf16x8* v) {
+    float bv[16];
 #pragma unroll
-        for (int q = 0; q < NROW; q++) {
-            const int e = t + q * 256;
-            const int g = e % G, col = e / G;
-            if (col < BPX + 2) *reinterpret_cast<bf16x8*>(&s_in[slot][col][g * 8]) = v[q];
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int co = co0 + 8 * i + 4 * kh + j;
+            bv[i * 4 + j] = (bias && co < Cout) ? bias[co] : 0.f;
         }
-    };
-    {
-        bf16x8 vw[NW];
+    const int px = x0 + m;
+    auto store_row = [&](int yy, const f32x16& acc) {
+        if (px >= W) return;
+        const size_t o = ((size_t(n) * H + yy) * W + px) * Cout;
 #pragma unroll
-        for (int q = 0; q < NW; q++) {
-            const int e = t + q * 256;
-            const int g = e % G, co = (e / G) % 32, tap = e / (G * 32);
-            bf16x8 val = zero8();
-            if (tap < 9 && co < Cout && g * 8 < Cin) val = load8(wgt + (size_t(co) * 9 + tap) * Cin + g * 8);
-            vw[q] = val;
-        }
-        bf16x8 r0[NROW], r1[NROW], r2[NROW];
-        load_row(y0 - 1, r0);
-        load_row(y0, r1);
-        load_row(y0 + 1, r2);
+        for (int i = 0; i < 4; i++) {
+            const int co = co0 + 8 * i + 4 * kh;
+            if (co >= Cout) continue;
+            float v[4];
 #pragma unroll
-        for (int q = 0; q < NW; q++) {
-            const int e = t + q * 256;
-            const int g = e % G, co = (e / G) % 32, tap = e / (G * 32);
-            if (tap < 9) *reinterpret_cast<bf16x8*>(&s_w[tap][co][g * 8]) = vw[q];
-        }
-        store_row((y0 + 2) % 3, r0);  // slot of row r = (r + 3) % 3
-        store_row(y0 % 3, r1);
-        store_row((y0 + 1) % 3, r2);
-    }
-    __syncthreads();
-    const int co = m;
-    const float bv = (bias && co < Cout) ? bias[co] : 0.f;
-    for (int r = 0; r < T; r++) {
-        const int yy = y0 + r;
-        if (yy >= H) break;  // uniform over the workgroup
-        bf16x8 nxt[NROW];
-        const bool more = r + 1 < T && yy + 1 < H;
-        if (more) load_row(yy + 2, nxt);
-        f32x16 acc;
+            for (int j = 0; j < 4; j++) v[j] = act_fwd(acc[i * 4 + j] + bv[i * 4 + j], act, slope);
+            if ((Cout & 3) == 0) {
+                if (y) {
+                    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                    bf16x4 b;
 #pragma unroll
-        for (int i = 0; i < 16; i++) acc[i] = 0.f;
-#pragma unroll
-        for (int tap = 0; tap < 9; tap++) {
-            const int dy = tap / 3, dx = tap % 3;
-            const int slot = (yy + dy + 2) % 3;  // row yy + dy - 1
-#pragma unroll
-            for (int ks = 0; ks < KCH / 16; ks++) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&s_w[tap][m][ks * 16 + kh * 8]);
-                const bf16x8 av = *reinterpret_cast<const bf16x8*>(&s_in[slot][wv * 32 + m + dx][ks * 16 + kh * 8]);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b, acc, 0, 0, 0);
-            }
-        }
-        if (co < Cout) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
+                    for (int j = 0; j < 4; j++) b[j] = bf16(v[j]);
+                    *reinterpret_cast<bf16x4*>(y + o + co) = b;
+                }
+                if (yf) *reinterpret_cast<float4*>(yf + o + co) = make_float4(v[0], v[1], v[2], v[3]);
+            } else {
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    const int px = x0 + wv * 32 + i * 8 + kh * 4 + j;
-                    if (px >= W) continue;
-                    const float v = act_fwd(acc[i * 4 + j] + bv, act, slope);
-                    const size_t o = ((size_t(n) * H + yy) * W + px) * Cout + co;
-                    if (y) y[o] = bf16(v);
-                    if (yf) yf[o] = v;
+                    if (co + j >= Cout) break;
+                    if (y) y[o + co + j] = bf16(v[j]);
+                    if (yf) yf[o + co + j] = v[j];
                 }
             }
         }
-        __syncthreads();  // every wave is done with row yy - 1's slot
-        if (more) store_row((yy + 2) % 3, nxt);  // row yy + 2 takes row yy - 1's slot
-        __syncthreads();
+    };
+
+    bf16x8 nxt[NQ];
+    load_chunk(0, 0, nxt);
+    __syncthreads();  // weights staged; from here on every wave runs on its own
+
+    // one input row: aA/aB/aC accumulate output rows jj, jj - 1, jj - 2 (taps
+    // dy = 0, 1, 2); afterwards row jj - 2 is complete
+    auto step = [&](int jj, f32x16& aA, f32x16& aB, f32x16& aC) {
+        if (jj >= rows + 2) return;
+        const int gy = y0 - 1 + jj;
+        const bool in_img = gy >= 0 && gy < H;
+        const bool vA = jj < rows && y0 + jj < H;
+        const bool vB = jj >= 1 && jj - 1 < rows && y0 + jj - 1 < H;
+        const bool vC = jj >= 2 && y0 + jj - 2 < H;
+        for (int ch = 0; ch < nch; ch++) {
+            // the wave's previous reads of s_x are issued before these writes
+            // (LDS executes one wave's instructions in order)
+            __asm__ volatile("" ::: "memory");
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int e = l + q * 64;
+                const int g = e % G, col = e / G;
+                if (col < SPX) *reinterpret_cast<bf16x8*>(s_x + col * XP + g * 8) = nxt[q];
+            }
+            __asm__ volatile("" ::: "memory");
+            {
+                int nj = jj, nc = ch + 1;
+                if (nc == nch) {
+                    nc = 0;
+                    nj++;
+                }
+                if (nj < rows + 2) load_chunk(nj, nc, nxt);
+            }
+            if (!in_img) continue;
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) {
+#pragma unroll
+                for (int ks = 0; ks < KC / 16; ks++) {
+                    const bf16x8 xb = *reinterpret_cast<const bf16x8*>(s_x + (m + dx) * XP + ks * 16 + kh * 8);
+                    const int k = ch * KC + ks * 16 + kh * 8;
+                    if (vA) {
+                        const bf16x8 w = *reinterpret_cast<const bf16x8*>(s_w + ((0 + dx) * 32 + m) * WP + k);
+                        aA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, xb, aA, 0, 0, 0);
+                    }
+                    if (vB) {
+                        const bf16x8 w = *reinterpret_cast<const bf16x8*>(s_w + ((3 + dx) * 32 + m) * WP + k);
+                        aB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, xb, aB, 0, 0, 0);
+                    }
+                    if (vC) {
+                        const bf16x8 w = *reinterpret_cast<const bf16x8*>(s_w + ((6 + dx) * 32 + m) * WP + k);
+                        aC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, xb, aC, 0, 0, 0);
+                    }
+                }
+            }
+        }
+        if (vC) store_row(y0 + jj - 2, aC);
+#pragma unroll
+        for (int i = 0; i < 16; i++) aC[i] = 0.f;
+    };
+    f32x16 a0, a1, a2;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        a0[i] = 0.f;
+        a1[i] = 0.f;
+        a2[i] = 0.f;
     }
+    for (int jj = 0; jj < rows + 2; jj += 3) {
+        step(jj, a0, a2, a1);
+        step(jj + 1, a1, a0, a2);
+        step(jj + 2, a2, a1, a0);
+    }
+}
+
+static size_t strip_lds_bytes(int Cin, int KC) {
+    const int CinP = (Cin + KC - 1) / KC * KC;
+    return (size_t(9) * 32 * (CinP + PAD) + size_t(4) * 34 * (KC + PAD)) * sizeof(bf16);
+}
+
+// Rows per strip: a workgroup's time grows with rows + 2 (the halo rows), and
+// a launch takes ceil(workgroups / resident workgroups) rounds, so pick the
+// row count that minimises rounds x (rows + 2).
+static int strip_rows(const void* fn, size_t lds, int zdim, int H, int W) {
+    thread_local std::unordered_map<size_t, int> resident_cache;
+    int resident = 0;
+    const size_t key = lds * 2 + (fn == reinterpret_cast<const void*>(&conv3x3_strip_kernel<16>) ? 1 : 0);
+    auto it = resident_cache.find(key);
+    if (it != resident_cache.end()) {
+        resident = it->second;
+    } else {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds) != hipSuccess)
+            return 16;
+        resident = resident_cache[key] = std::max(1, cus * per_cu);
+    }
+    const long cols = (W + 127) / 128;
+    int best = 16;
+    long best_cost = -1;
+    for (int r = 4; r <= 64; r++) {
+        const long blocks = cols * ((H + r - 1) / r) * zdim;
+        const long cost = (blocks + resident - 1) / resident * (r + 2);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = r;
+        }
+    }
+    return best;
 }
 
 template <int R, int C, int KCH>
@@ -604,17 +699,26 @@ hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
     if ((c1 % 8) || (c2 % 8) || c1 <= 0 || c2 < 0 || Cout <= 0) return hipErrorInvalidValue;
     if (x1_mode < 0 || x1_mode > 2 || (x1_mode == 1 && ((H | W) & 1))) return hipErrorInvalidValue;
     const int Cin = c1 + c2;
-    // (measured: the ring pays off while one staged row holds every channel
-    // of a 32-channel input; wider inputs run faster on the chunked kernel)
-    if (Cout <= 32 && x1_mode != 2 && Cin <= 32) {
-        constexpr int T = 8;
-        dim3 grid((W + 127) / 128, (H + T - 1) / T, N);
-#define RFK_ROWS(K)                                                                                            \
-    conv3x3_fwd_rows_kernel<K, T><<<grid, 256, 0, s>>>((const bf16*)x1, c1, x1_mode, (const bf16*)x2, c2, N, H, W, \
-                                                       (const bf16*)wgt, bias, Cout, act, slope, (bf16*)y, yf)
-        if (Cin <= 16) RFK_ROWS(16);
-        else RFK_ROWS(32);
-#undef RFK_ROWS
+    // the strip kernel wherever its weights leave room for two workgroups per
+    // CU and the output has at most three 32-channel tiles (measured: the
+    // 128-channel and the widest-input layers run faster on the tiled kernel)
+    const int KC = Cin <= 16 ? 16 : 32;
+    const size_t lds = strip_lds_bytes(Cin, KC);
+    if (Cout <= 96 && lds <= 80 * 1024) {
+        const void* fn = KC == 16 ? reinterpret_cast<const void*>(&conv3x3_strip_kernel<16>)
+                                  : reinterpret_cast<const void*>(&conv3x3_strip_kernel<32>);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess) return e;
+        const int rows = strip_rows(fn, lds, N * ((Cout + 31) / 32), H, W);
+        dim3 grid((W + 127) / 128, (H + rows - 1) / rows, N * ((Cout + 31) / 32));
+        if (KC == 16)
+            conv3x3_strip_kernel<16><<<grid, 256, lds, s>>>((const bf16*)x1, c1, x1_mode, (const bf16*)x2, c2, N, H, W,
+                                                            (const bf16*)wgt, bias, Cout, act, slope, rows, (bf16*)y,
+                                                            yf);
+        else
+            conv3x3_strip_kernel<32><<<grid, 256, lds, s>>>((const bf16*)x1, c1, x1_mode, (const bf16*)x2, c2, N, H, W,
+                                                            (const bf16*)wgt, bias, Cout, act, slope, rows, (bf16*)y,
+                                                            yf);
         return hipGetLastError();
     }
     if (Cout <= 32) {

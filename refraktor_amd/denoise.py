@@ -112,7 +112,10 @@ def render_frame(db: render.VariationDB, f: dict, width: int, height: int, iters
     check(lib.rfk_density_estimate(st.handle, de_radius, 0.0, 0.4, de.ptr))
     tm = render.DeviceBuffer(npx * 16)
     args = _lib.TonemapArgs(inf.quality, f["gamma"], f["brightness"], f["vibrancy"], 0.0)
-    check(lib.rfk_tonemap_image(de.ptr, width, height, tm.ptr, _lib.PIXEL_FLOAT4, C.byref(args), None))
+    # density estimation runs on the state's stream: tonemap behind it there,
+    # and hand over the frame only once both are done
+    check(lib.rfk_tonemap_image(de.ptr, width, height, tm.ptr, _lib.PIXEL_FLOAT4, C.byref(args), inf.stream))
+    check(lib.rfk_synchronize())
     return Frame(hist, inf.hist_bits, tm, width, height, inf.quality)
 
 

@@ -35,6 +35,7 @@ CASES = [  # (N, H, W, c1, c2, Cout): every tile shape of the forward kernel
     (2, 11, 140, 16, 0, 24),
     (1, 10, 150, 64, 32, 32),
     (1, 9, 131, 48, 0, 16),
+    (1, 37, 45, 16, 8, 3),  # strip kernel: channel count not a multiple of 4, three row groups
 ]
 
 
