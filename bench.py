@@ -245,7 +245,7 @@ def cpu_baseline(f: dict, W: int, H: int, seed: int, it_per_frame: int) -> dict:
     from oracle import oracle
     from refraktor_amd import render
     blocks, tpb = 1024, 256
-    passes = 256  # 1024*256*256 = 2^26 iterations
+    passes = 1024  # 1024*256*1024 = 2^28 iterations, a few seconds on 16 threads
     L, P32, pals = oracle.prepare(f, render.CONFIG_DIR, W, H, blocks, seed)
     threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or min(os.cpu_count() or 1, 16)
     t0 = time.perf_counter()
