@@ -73,7 +73,7 @@ std::string emit_vlink(const variation_db& db, const vlink& vl, const vlink_slot
 
 std::string emit_xform_fn(const variation_db& db, const xform& xf, const xform_slot& xs, const std::string& name) {
     std::ostringstream os;
-    os << "RFK_INLINE void " << name << "(const Real* __restrict__ P, v2& q, rng_t& rs) {\n";
+    os << "RFK_INLINE void " << name << "(param_ptr P, v2& q, rng_t& rs) {\n";
     os << "    (void)rs;\n";
     for (size_t j = 0; j < xs.vlinks.size(); j++) {
         os << "    // vlink " << j << "\n";
@@ -111,7 +111,7 @@ std::string make_flame_functions(const variation_db& db, const flame& f, const f
     if (f.final_xform) os << emit_xform_fn(db, *f.final_xform, *L.final_xform, "rfk_xform_final");
 
     // dispatch (flame.tpl:146-160): xform, then colour blend, returns opacity
-    os << "RFK_INLINE Real rfk_dispatch(u32 xid, const Real* __restrict__ P, v3& pt, rng_t& rs) {\n";
+    os << "RFK_INLINE Real rfk_dispatch(u32 xid, param_ptr P, v3& pt, rng_t& rs) {\n";
     os << "    v2 q = {pt.x, pt.y};\n";
     os << "    switch (xid) {\n";
     auto emit_case = [&](size_t idx, const xform_slot& xs, const std::string& fn) {
@@ -128,7 +128,7 @@ std::string make_flame_functions(const variation_db& db, const flame& f, const f
     os << "        default: return Real(0);\n    }\n}\n\n";
 
     // cumulative-weight selection (flame.tpl:110-123)
-    os << "RFK_INLINE u32 rfk_select_xform(const Real* __restrict__ P, Real ratio) {\n";
+    os << "RFK_INLINE u32 rfk_select_xform(param_ptr P, Real ratio) {\n";
     os << "    RFK_EXACT_FP\n";
     // IEEE-exact products and sums so the choice is reproducible off the GPU
     os << "    ratio = RFK_MUL_RN(ratio, " << P(L.weight_sum) << ");\n";
@@ -147,10 +147,10 @@ std::string make_flame_functions(const variation_db& db, const flame& f, const f
 
     if (L.chaos_base >= 0) {
         // xaos selection (flame.tpl:127-142): row `last` of the chaos table
-        os << "RFK_INLINE u32 rfk_select_xform_chaos(const Real* __restrict__ P, u32 last, Real ratio) {\n";
+        os << "RFK_INLINE u32 rfk_select_xform_chaos(param_ptr P, u32 last, Real ratio) {\n";
         os << "    RFK_EXACT_FP\n";
         os << "    if (last >= " << n << "u) return rfk_select_xform(P, ratio);\n";
-        os << "    const Real* row = P + " << L.chaos_base << " + last * " << (n + 1) << "u;\n";
+        os << "    param_ptr row = P + " << L.chaos_base << " + last * " << (n + 1) << "u;\n";
         os << "    ratio = RFK_MUL_RN(ratio, row[0]);\n    Real rsum = Real(0);\n    u32 last_nonzero = 0;\n";
         for (size_t i = 0; i < n; i++) {
             const std::string w = "row[" + std::to_string(i + 1) + "]";

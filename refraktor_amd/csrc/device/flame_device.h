@@ -25,6 +25,11 @@ typedef double Real;
 typedef float Real;
 #endif
 
+// A sample's parameter row as the iterate kernels read it: constant address
+// space (nothing writes it while they run), so loads at workgroup-uniform
+// addresses become scalar loads into SGPRs instead of vector loads per pass.
+typedef const __attribute__((address_space(4))) Real* param_ptr;
+
 #define RFK_INLINE __device__ __forceinline__
 
 // Start of a block whose arithmetic must come out as written: no FMA
@@ -139,7 +144,8 @@ RFK_INLINE v3 vec3(Real x, Real y, Real z) { return {x, y, z}; }
 
 // affine in librefrakt field order (a, d, b, e, c, f): x' = a x + b y + c,
 // y' = d x + e y + f, as two nested FMAs (assets/templates/flame.tpl:10-14)
-RFK_INLINE v2 affine(const Real* __restrict__ A, v2 p) {
+template <class Ptr>
+RFK_INLINE v2 affine(Ptr A, v2 p) {
     return {fma(A[0], p.x, fma(A[2], p.y, A[4])), fma(A[1], p.x, fma(A[3], p.y, A[5]))};
 }
 

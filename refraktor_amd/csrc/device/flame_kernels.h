@@ -161,7 +161,7 @@ RFK_INLINE u32 rfk_wave_pick(u32 pass, u32 block, u32 wave, u32 k0, u32 k1, u32x
 //   wx..wl:   walker state arrays (null: the block keeps its walkers only
 //             for this call)
 // Returns the passes done (bailouts may stop early); counters land in sm.
-RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, const Real* __restrict__ P, const u32* pal,
+RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, param_ptr P, const u32* pal,
                          u32 block, u32 key1, hist_t* hist, bool init, u32 total, u32 pass0, u32 nfuse, u32 npasses,
                          u32 flags, float* wx, float* wy, float* wc, u32* wl) {
     const u32 t = threadIdx.x;
@@ -413,7 +413,7 @@ extern "C" __global__ void rfk_precalc(rfk_iter_args a) {
 extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate(rfk_iter_args a) {
     __shared__ rfk_shared sm;
     const u32 block = blockIdx.x;
-    const Real* __restrict__ P = (const Real*)a.params + (size_t)block * a.param_stride;
+    const param_ptr P = (param_ptr)((const Real*)a.params + (size_t)block * a.param_stride);
     const bool init = (a.flags & RFK_FLAG_INIT) != 0u;
     const u32 pass0 = init ? 0u : a.block_pass[block];
     const u32 done = rfk_block(a, sm, P, a.palettes + (size_t)block * 256u, block, a.stream, (hist_t*)a.hist, init,
@@ -444,7 +444,7 @@ extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate_batch(rfk_iter
         if (item >= b.nitems) break;  // uniform: every wave of the group leaves together
         const u32 f = b.item_frame[item], blk = b.item_block[item];
         const u32 row = b.frame_row0[f] + blk;
-        const Real* __restrict__ P = (const Real*)a.params + (size_t)row * a.param_stride;
+        const param_ptr P = (param_ptr)((const Real*)a.params + (size_t)row * a.param_stride);
         const u32* pal = a.palettes + (size_t)row * 256u;
         hist_t* hist = (hist_t*)a.hist + (size_t)f * b.frame_pixels * 4u;
         const u32 total = b.frame_blocks[f] * RFK_TPB;

@@ -102,32 +102,38 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // One workgroup: tile_start = exclusive scan of tile_total; work items
-// (tile, begin, end, exclusive) in tile order; nseg[0] = their count.
+// (tile, begin, end, exclusive), nseg[0] = their count. The full 2^18-record
+// items come first and the tiles' shorter remainders after them, so the
+// workgroups that start last are the short ones (longest first).
 __global__ void __launch_bounds__(kThreads)
     qseg_kernel(const uint32_t* __restrict__ tile_total, uint32_t ntiles, uint32_t* __restrict__ tile_start,
                 uint4* __restrict__ segments, uint32_t max_segments, uint32_t* __restrict__ nseg) {
     __shared__ uint32_t s_wave[kThreads / 64];
     const uint32_t per = (ntiles + kThreads - 1) / kThreads;
     const uint32_t lo = threadIdx.x * per, hi = std::min(ntiles, lo + per);
-    uint32_t sum = 0, segs = 0;
+    uint32_t sum = 0, nfull = 0, nrem = 0;
     for (uint32_t t = lo; t < hi; t++) {
         sum += tile_total[t];
-        segs += (tile_total[t] + kSegment - 1) / kSegment;
+        nfull += tile_total[t] / kSegment;
+        nrem += (tile_total[t] % kSegment) != 0u;
     }
     uint32_t run = block_exclusive_scan(sum, s_wave, nullptr);
-    uint32_t nall = 0;
-    uint32_t k = block_exclusive_scan(segs, s_wave, &nall);
+    uint32_t full_all = 0, rem_all = 0;
+    uint32_t kf = block_exclusive_scan(nfull, s_wave, &full_all);
+    uint32_t kr = full_all + block_exclusive_scan(nrem, s_wave, &rem_all);
     for (uint32_t t = lo; t < hi; t++) {
         const uint32_t n = tile_total[t];
         tile_start[t] = run;
-        const uint32_t nt = (n + kSegment - 1) / kSegment;
-        for (uint32_t s = 0; s < nt; s++, k++) {
-            if (k < max_segments)
-                segments[k] = make_uint4(t, run + s * kSegment, run + std::min(n, (s + 1) * kSegment), nt == 1 ? 1u : 0u);
+        const uint32_t nf = n / kSegment, exclusive = (n <= kSegment) ? 1u : 0u;
+        for (uint32_t s = 0; s < nf; s++, kf++)
+            if (kf < max_segments) segments[kf] = make_uint4(t, run + s * kSegment, run + (s + 1) * kSegment, exclusive);
+        if (n % kSegment) {
+            if (kr < max_segments) segments[kr] = make_uint4(t, run + nf * kSegment, run + n, exclusive);
+            kr++;
         }
         run += n;
     }
-    if (threadIdx.x == 0) nseg[0] = std::min(nall, max_segments);
+    if (threadIdx.x == 0) nseg[0] = std::min(full_all + rem_all, max_segments);
 }
 
 // grid = nparts: partition p = records [p * per_part, (p + 1) * per_part).
