@@ -200,9 +200,13 @@ __global__ void de_gather_kernel(const BinT* __restrict__ hist, const int* __res
     float4* s_col = reinterpret_cast<float4*>(smem);
     int* s_lvl = reinterpret_cast<int*>(s_col + span * span);
     float* s_iz = reinterpret_cast<float*>(s_lvl + span * span);
+    __shared__ int s_maxlv;
     const int x0 = int(blockIdx.x) * TILE - radius, y0 = int(blockIdx.y) * TILE - radius;
     const int tid = int(threadIdx.y) * TILE + int(threadIdx.x);
+    if (tid == 0) s_maxlv = -1;
     for (int i = tid; i < nlevels; i += TILE * TILE) s_iz[i] = inv_z[i];
+    __syncthreads();
+    int maxlv = -1;
     for (int i = tid; i < span * span; i += TILE * TILE) {
         const int xx = x0 + i % span, yy = y0 + i / span;
         int lv = -1;
@@ -214,14 +218,18 @@ __global__ void de_gather_kernel(const BinT* __restrict__ hist, const int* __res
         }
         s_lvl[i] = lv;
         s_col[i] = c;
+        maxlv = max(maxlv, lv);
     }
+    atomicMax(&s_maxlv, maxlv);
     __syncthreads();
     const int ox = int(blockIdx.x) * TILE + int(threadIdx.x), oy = int(blockIdx.y) * TILE + int(threadIdx.y);
     if (ox >= int(width) || oy >= int(height)) return;
+    // no source of this span reaches further than its widest kernel
+    const int reach = min(radius, (s_maxlv + 15) / 16);
     float4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int dy = -radius; dy <= radius; dy++) {
+    for (int dy = -reach; dy <= reach; dy++) {
         const int row = (int(threadIdx.y) + radius + dy) * span + int(threadIdx.x) + radius;
-        for (int dx = -radius; dx <= radius; dx++) {
+        for (int dx = -reach; dx <= reach; dx++) {
             const int lv = s_lvl[row + dx];
             if (lv < 0) continue;
             const float h = float(lv) * (1.0f / 16.0f);
