@@ -266,6 +266,7 @@ int rfk_density_estimate(rfk_state* s, float max_radius, float min_radius, float
 #define RFK_PIXEL_FLOAT3 2
 #define RFK_PIXEL_HALF3 3
 #define RFK_PIXEL_RGBA8 4
+#define RFK_PIXEL_RGB8 5 /* conversion output only */
 typedef struct rfk_tonemap_args {
     double quality;      /* mean density (state quality); <= 0 uses the state's */
     double gamma;
@@ -285,6 +286,19 @@ int rfk_density_hdr(const void* tonemapped, int pixel_format, size_t npix, const
  * one), enqueued on `stream`.                                               */
 int rfk_tonemap_image(const float* in_rgba, uint32_t width, uint32_t height, void* out, int pixel_format,
                       const rfk_tonemap_args* a, void* stream);
+
+/* rfkt::converter (src/librefrakt/include/librefrakt/image/converter.h:14-109,
+ * assets/kernels/convert.cu:72-98): convert a tonemapped HALF3 / HALF4 image
+ * (device) into FLOAT4, HALF4, FLOAT3, HALF3, RGBA8 or RGB8 (device),
+ * enqueued on `stream`. 8-bit channels are min(v * 255, 255) truncated (0
+ * for negative v); a 3-channel input gets alpha 1. flags: RFK_CONVERT_FLIP
+ * writes row y to row height-1-y, RFK_CONVERT_SWAP exchanges R and B. (The
+ * reference's to_float3 for half4 inputs launches its half3 kernel; here
+ * every pair converts as its formats say.)                                  */
+#define RFK_CONVERT_FLIP 1
+#define RFK_CONVERT_SWAP 2
+int rfk_convert_image(const void* in, int in_format, void* out, int out_format, uint32_t width, uint32_t height,
+                      int flags, void* stream);
 
 /* Catmull-Rom coefficients: generate_sample_coefficients
  * (assets/kernels/catmull.cu:33-42): device samples [nseg+3][sample_size]

@@ -558,3 +558,27 @@ def density_hdr(img: np.ndarray, bins97) -> np.ndarray:
         out[..., 2] = np.where(m, cb, out[..., 2])
     out[..., 3] = 1.0
     return out
+
+
+def convert_image(img: np.ndarray, fmt: int, flip: bool = False, swap: bool = False) -> np.ndarray:
+    """convert_to (assets/kernels/convert.cu:72-98) of a half3 / half4 image:
+    element by element to float / half / 8-bit, where 8-bit is
+    min(v * 255, 255) truncated toward zero (convert.cu:46-48; negative v,
+    undefined there, give 0); a 3-channel input gets alpha 1; `swap`
+    exchanges R and B; `flip` writes row y to row h-1-y. fmt: 0 float4,
+    1 half4, 2 float3, 3 half3, 4 RGBA8, 5 RGB8."""
+    h = np.asarray(img, dtype=np.float16)
+    if h.shape[-1] == 3:
+        h = np.concatenate([h, np.ones(h.shape[:-1] + (1,), np.float16)], axis=-1)
+    if swap:
+        h = h[..., [2, 1, 0, 3]]
+    if flip:
+        h = h[::-1]
+    ch = 4 if fmt in (0, 1, 4) else 3
+    h = h[..., :ch]
+    if fmt in (0, 2):
+        return h.astype(np.float32)
+    if fmt in (1, 3):
+        return h.copy()
+    v = np.minimum(h.astype(np.float64) * 255.0, 255.0)
+    return np.trunc(np.maximum(v, 0.0)).astype(np.uint8)

@@ -19,8 +19,9 @@ RFK_ERR_HIP = -4
 RFK_ERR_NODEVICE = -5
 RFK_ERR_OVERFLOW = -6
 
-PIXEL_FLOAT4, PIXEL_HALF4, PIXEL_FLOAT3, PIXEL_HALF3, PIXEL_RGBA8 = 0, 1, 2, 3, 4
-PIXEL_BYTES = {0: 16, 1: 8, 2: 12, 3: 6, 4: 4}
+PIXEL_FLOAT4, PIXEL_HALF4, PIXEL_FLOAT3, PIXEL_HALF3, PIXEL_RGBA8, PIXEL_RGB8 = 0, 1, 2, 3, 4, 5
+PIXEL_BYTES = {0: 16, 1: 8, 2: 12, 3: 6, 4: 4, 5: 3}
+CONVERT_FLIP, CONVERT_SWAP = 1, 2
 
 
 class ExecConfig(C.Structure):
@@ -127,6 +128,7 @@ _SIGS = [
     ("rfk_tonemap", C.c_int, [_P, _P, C.c_int, C.POINTER(TonemapArgs)]),
     ("rfk_tonemap_image", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_int, C.POINTER(TonemapArgs), _P]),
     ("rfk_density_hdr", C.c_int, [_P, C.c_int, C.c_size_t, C.POINTER(C.c_uint64), _P, _P]),
+    ("rfk_convert_image", C.c_int, [_P, C.c_int, _P, C.c_int, C.c_uint32, C.c_uint32, C.c_int, _P]),
     ("rfk_catmull_coefficients", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     ("rfk_fix_rotation", C.c_int, [C.POINTER(C.c_double), C.c_size_t, C.c_size_t, C.POINTER(C.c_size_t),
                                    C.c_size_t, C.c_int]),

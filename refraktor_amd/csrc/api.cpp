@@ -1190,6 +1190,19 @@ int rfk_tonemap_image(const float* in_rgba, uint32_t width, uint32_t height, voi
                           static_cast<hipStream_t>(stream));
 }
 
+int rfk_convert_image(const void* in, int in_format, void* out, int out_format, uint32_t width, uint32_t height,
+                      int flags, void* stream) {
+    if (!in || !out) return fail(RFK_ERR_INVALID, "null argument");
+    if (in_format != RFK_PIXEL_HALF3 && in_format != RFK_PIXEL_HALF4)
+        return fail(RFK_ERR_INVALID, "conversion reads half3 or half4 images");
+    if (out_format < RFK_PIXEL_FLOAT4 || out_format > RFK_PIXEL_RGB8) return fail(RFK_ERR_INVALID, "bad pixel format");
+    if (flags & ~(RFK_CONVERT_FLIP | RFK_CONVERT_SWAP)) return fail(RFK_ERR_INVALID, "unknown conversion flags");
+    RFK_HIP(rfk::sk::convert_image(in, in_format == RFK_PIXEL_HALF4 ? 4 : 3, out, out_format, width, height,
+                                   (flags & RFK_CONVERT_FLIP) != 0, (flags & RFK_CONVERT_SWAP) != 0,
+                                   static_cast<hipStream_t>(stream)));
+    return RFK_OK;
+}
+
 int rfk_density_hdr(const void* tonemapped, int pixel_format, size_t npix, const uint64_t* bins97, float* out_rgba,
                     void* stream) {
     if (!tonemapped || !bins97 || !out_rgba) return fail(RFK_ERR_INVALID, "null argument");

@@ -494,6 +494,89 @@ hipError_t tonemap(const void* in, int in_kind, size_t npix, void* out, int form
     return hipGetLastError();
 }
 
+// ----------------------------------------------------------- conversion
+// convert_to (assets/kernels/convert.cu:72-98): half3/half4 in; out element
+// by element as float, half or 8-bit (min(v * 255, 255), truncated: exact in
+// float for half inputs); a 3-channel input gets alpha 1; `swap` exchanges
+// R and B; `flip` writes row y to row height - 1 - y.
+template <int InC, int OutFmt>
+__global__ void convert_kernel(const __half* __restrict__ in, void* __restrict__ out, uint32_t width, uint32_t height,
+                               bool flip, bool swap) {
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t npix = size_t(width) * height;
+    if (g >= npix) return;
+    __half h[4];
+    if constexpr (InC == 4) {
+        const half4_t v = reinterpret_cast<const half4_t*>(in)[g];
+        h[0] = v.x;
+        h[1] = v.y;
+        h[2] = v.z;
+        h[3] = v.w;
+    } else {
+        const half3_t v = reinterpret_cast<const half3_t*>(in)[g];
+        h[0] = v.x;
+        h[1] = v.y;
+        h[2] = v.z;
+        h[3] = __float2half(1.0f);
+    }
+    if (swap) {
+        const __half t = h[0];
+        h[0] = h[2];
+        h[2] = t;
+    }
+    size_t o = g;
+    if (flip) {
+        const size_t y = g / width, x = g % width;
+        o = (size_t(height) - 1 - y) * width + x;
+    }
+    // (negative inputs, undefined in the reference's cast, give 0)
+    auto u8 = [](__half v) { return uint32_t(fmaxf(fminf(__half2float(v) * 255.0f, 255.0f), 0.0f)); };
+    if constexpr (OutFmt == 0) {
+        reinterpret_cast<float4*>(out)[o] = {__half2float(h[0]), __half2float(h[1]), __half2float(h[2]), __half2float(h[3])};
+    } else if constexpr (OutFmt == 1) {
+        reinterpret_cast<half4_t*>(out)[o] = {h[0], h[1], h[2], h[3]};
+    } else if constexpr (OutFmt == 2) {
+        float* f = reinterpret_cast<float*>(out) + 3 * o;
+        f[0] = __half2float(h[0]);
+        f[1] = __half2float(h[1]);
+        f[2] = __half2float(h[2]);
+    } else if constexpr (OutFmt == 3) {
+        reinterpret_cast<half3_t*>(out)[o] = {h[0], h[1], h[2]};
+    } else if constexpr (OutFmt == 4) {
+        reinterpret_cast<uint32_t*>(out)[o] = u8(h[0]) | (u8(h[1]) << 8) | (u8(h[2]) << 16) | (u8(h[3]) << 24);
+    } else {
+        uint8_t* b = reinterpret_cast<uint8_t*>(out) + 3 * o;
+        b[0] = uint8_t(u8(h[0]));
+        b[1] = uint8_t(u8(h[1]));
+        b[2] = uint8_t(u8(h[2]));
+    }
+}
+
+hipError_t convert_image(const void* in, int in_channels, void* out, int out_format, uint32_t width, uint32_t height,
+                         bool flip, bool swap, hipStream_t s) {
+    const size_t npix = size_t(width) * height;
+    if (!npix) return hipSuccess;
+    if ((in_channels != 3 && in_channels != 4) || out_format < 0 || out_format > 5) return hipErrorInvalidValue;
+    const unsigned g = nblocks(npix);
+    const __half* src = reinterpret_cast<const __half*>(in);
+#define RFK_CV(C)                                                                                        \
+    switch (out_format) {                                                                               \
+        case 0: convert_kernel<C, 0><<<g, kBlock, 0, s>>>(src, out, width, height, flip, swap); break; \
+        case 1: convert_kernel<C, 1><<<g, kBlock, 0, s>>>(src, out, width, height, flip, swap); break; \
+        case 2: convert_kernel<C, 2><<<g, kBlock, 0, s>>>(src, out, width, height, flip, swap); break; \
+        case 3: convert_kernel<C, 3><<<g, kBlock, 0, s>>>(src, out, width, height, flip, swap); break; \
+        case 4: convert_kernel<C, 4><<<g, kBlock, 0, s>>>(src, out, width, height, flip, swap); break; \
+        default: convert_kernel<C, 5><<<g, kBlock, 0, s>>>(src, out, width, height, flip, swap); break; \
+    }
+    if (in_channels == 4) {
+        RFK_CV(4)
+    } else {
+        RFK_CV(3)
+    }
+#undef RFK_CV
+    return hipGetLastError();
+}
+
 hipError_t density_hdr(const void* in, bool in_half, size_t npix, const unsigned long long* bins97, float* out,
                        hipStream_t s) {
     if (!npix) return hipSuccess;

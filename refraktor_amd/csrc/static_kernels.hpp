@@ -60,6 +60,11 @@ hipError_t queue_bin(const unsigned long long* records, size_t per_part, uint32_
 hipError_t density_hdr(const void* in, bool in_half, size_t npix, const unsigned long long* bins97, float* out,
                        hipStream_t s);
 
+// convert_to (convert.cu:72-98): a half3 / half4 image (in_channels 3 / 4)
+// to out_format 0 float4, 1 half4, 2 float3, 3 half3, 4 RGBA8, 5 RGB8.
+hipError_t convert_image(const void* in, int in_channels, void* out, int out_format, uint32_t width, uint32_t height,
+                         bool flip, bool swap, hipStream_t s);
+
 // Random RGBA atomic increments for the binning roofline.
 hipError_t atomic_peak(uint32_t* buf, uint64_t nbins, uint32_t grid, uint32_t block, uint32_t iters, uint32_t seed,
                        hipStream_t s);

@@ -384,6 +384,8 @@ def _host_pixels(buf: DeviceBuffer, fmt: int, h: int, w: int) -> np.ndarray:
         return buf.download(np.float16, (h, w, 4))
     if fmt == _lib.PIXEL_HALF3:
         return buf.download(np.float16, (h, w, 3))
+    if fmt == _lib.PIXEL_RGB8:
+        return buf.download(np.uint8, (h, w, 3))
     return buf.download(np.uint8, (h, w, 4))
 
 
@@ -411,3 +413,19 @@ def density_hdr(img: np.ndarray, bins97: np.ndarray) -> np.ndarray:
     check(_lib.lib().rfk_density_hdr(src.ptr, _lib.PIXEL_FLOAT4, h * w, b.ctypes.data_as(C.POINTER(C.c_uint64)),
                                      out.ptr, None))
     return out.download(np.float32, (h, w, 4))
+
+
+def convert_image(img: np.ndarray, fmt: int, flip: bool = False, swap: bool = False) -> np.ndarray:
+    """rfkt::converter on the device: a host half3 / half4 image (float16,
+    h x w x 3|4) to `fmt` (PIXEL_FLOAT4 ... PIXEL_RGB8)"""
+    img = np.ascontiguousarray(img, dtype=np.float16)
+    h, w, c = img.shape
+    if c not in (3, 4):
+        raise ValueError("conversion reads half3 or half4 images")
+    src = DeviceBuffer(img.nbytes)
+    src.upload(img)
+    out = DeviceBuffer(h * w * _lib.PIXEL_BYTES[fmt])
+    flags = (_lib.CONVERT_FLIP if flip else 0) | (_lib.CONVERT_SWAP if swap else 0)
+    check(_lib.lib().rfk_convert_image(src.ptr, _lib.PIXEL_HALF4 if c == 4 else _lib.PIXEL_HALF3, out.ptr, fmt, w, h,
+                                       flags, None))
+    return _host_pixels(out, fmt, h, w)

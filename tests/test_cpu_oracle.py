@@ -220,3 +220,26 @@ def test_density_hdr_closed_forms():
     bins[0] = bins[96] = 1  # cdf: 0.5 up to bin 95, 1 at bin 96
     px = np.array([[[0.3, 0.3, 0.3, 10 / 96.0], [0.2, 0.5, 0.9, 20 / 96.0]]])
     np.testing.assert_allclose(oracle.density_hdr(px, bins)[0, :, :3], px[0, :, :3], atol=1e-7)
+
+
+def test_convert_image_closed_forms():
+    """8-bit channels truncate min(v * 255, 255) (convert.cu:46-48): 0.5 ->
+    127, not 128; half(1/255) sits just below 1/255 and gives 0; values above
+    1 clamp to 255; a half3 input gets alpha 1; swap exchanges R and B; flip
+    reverses the rows"""
+    x = np.array([[[1.0, 0.5, 0.0], [2.0, 0.25, 1 / 255]],
+                  [[0.75, 0.125, 0.999], [0.0, 1.0, 0.5]]], np.float16)
+    rgba = oracle.convert_image(x, 4)
+    assert rgba.dtype == np.uint8 and rgba.shape == (2, 2, 4)
+    assert rgba[0, 0].tolist() == [255, 127, 0, 255]
+    assert rgba[0, 1].tolist() == [255, 63, 0, 255]
+    assert float(np.float16(1 / 255)) * 255 < 1.0
+    assert rgba[1, 0].tolist() == [191, 31, int(float(np.float16(0.999)) * 255), 255]
+    rgb_sw = oracle.convert_image(x, 5, swap=True)
+    assert rgb_sw.shape == (2, 2, 3) and rgb_sw[0, 0].tolist() == [0, 127, 255]
+    fl4 = oracle.convert_image(x, 0, flip=True)
+    assert fl4.dtype == np.float32 and fl4[0, 1].tolist() == [0.0, 1.0, 0.5, 1.0]
+    np.testing.assert_array_equal(oracle.convert_image(x, 2), x.astype(np.float32))
+    h4 = np.concatenate([x, np.full((2, 2, 1), 0.5, np.float16)], axis=-1)
+    assert oracle.convert_image(h4, 4)[..., 3].tolist() == [[127, 127], [127, 127]]
+    np.testing.assert_array_equal(oracle.convert_image(h4, 3, swap=True, flip=True), h4[::-1, :, [2, 1, 0]])
