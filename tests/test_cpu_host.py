@@ -19,7 +19,8 @@ from refraktor_amd import render
 
 ROOT = Path(__file__).resolve().parents[1]
 REFERENCE_CONFIG = Path("/root/reference/config")
-REFERENCE_FLAM3 = Path("/root/reference/assets/flames_test/electricsheep.247.50049.flam3")
+# the reference's flam3 test flame (assets/flames_test/), kept as data
+REFERENCE_FLAM3 = Path(__file__).resolve().parent / "golden" / "electricsheep.247.50049.flam3"
 
 
 def test_header_symbols_exported():
@@ -196,7 +197,6 @@ def test_flam3_import(db):
     assert (h, s, v) == (0.0, 1.0, 1.0)
 
 
-@pytest.mark.skipif(not REFERENCE_FLAM3.exists(), reason="reference tree not mounted")
 def test_reference_flam3_fixture_imports(db):
     f = db.flame_from_flam3(REFERENCE_FLAM3.read_text())
     js = f.to_json()
