@@ -58,11 +58,13 @@ def test_conv_forward_matches_oracle(gpu, case):
     np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-4)
 
 
-@pytest.mark.parametrize("mode,Co", [(1, 64), (2, 64), (1, 32), (2, 32)])
-def test_conv_fused_upsample_and_pool(gpu, mode, Co):
-    """x1 read through a nearest 2x upsample (1) or a 2x2 average pool (2)"""
-    rng = np.random.default_rng(40 + mode + Co)
-    N, H, W, c1, c2 = 1, 10, 136, 32, 16
+@pytest.mark.parametrize("mode,Co,c1,c2", [(1, 64, 32, 16), (2, 64, 32, 16), (1, 32, 32, 16), (2, 32, 32, 16),
+                                           (1, 64, 96, 64), (1, 32, 64, 32), (2, 96, 96, 8), (1, 96, 128, 96)])
+def test_conv_fused_upsample_and_pool(gpu, mode, Co, c1, c2):
+    """x1 read through a nearest 2x upsample (1) or a 2x2 average pool (2);
+    the wide inputs take the weight-stationary kernel"""
+    rng = np.random.default_rng(40 + mode + Co + c1)
+    N, H, W = 1, 10, 136
     if mode == 1:
         x1 = nn.to_bf16(rng.normal(size=(N, H // 2, W // 2, c1)).astype(np.float32))
         x1_eff = nn.up(x1)
