@@ -708,8 +708,12 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wstat_kernel(const bf16* __res
     for (int r = 0; r < R; r++)
 #pragma unroll
         for (int i = 0; i < 16; i++) acc[r][i] = 0.f;
-    bf16x8 nxt[NQ];
-    load_chunk(0, 0, nxt);
+    // input rows are loaded two steps ahead (R + 2 is even, so the two
+    // register sets keep their parity across chunks)
+    static_assert((R & 1) == 0, "R must be even");
+    bf16x8 pf[2][NQ];
+    load_chunk(0, 0, pf[0]);
+    load_chunk(1, 0, pf[1]);
     const int co_l = co0 + m;
     for (int ch = 0; ch < nch; ch++) {
         bf16x8 wf[9][2];
@@ -729,11 +733,11 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wstat_kernel(const bf16* __res
             for (int q = 0; q < NQ; q++) {
                 const int e = l + q * 64;
                 const int g = e % G, col = e / G;
-                if (col < SPX) *reinterpret_cast<bf16x8*>(s_x + col * XP + g * 8) = nxt[q];
+                if (col < SPX) *reinterpret_cast<bf16x8*>(s_x + col * XP + g * 8) = pf[jj & 1][q];
             }
             __asm__ volatile("" ::: "memory");
-            if (jj + 1 < R + 2) load_chunk(jj + 1, ch, nxt);
-            else if (ch + 1 < nch) load_chunk(0, ch + 1, nxt);
+            if (jj + 2 < R + 2) load_chunk(jj + 2, ch, pf[jj & 1]);
+            else if (ch + 1 < nch) load_chunk(jj + 2 - (R + 2), ch + 1, pf[jj & 1]);
             const int gy = y0 - 1 + jj;
             if (gy < 0 || gy >= H) continue;  // zero rows add nothing
 #pragma unroll
