@@ -486,10 +486,12 @@ unsigned nb(size_t n, unsigned b = 256) { return unsigned((n + b - 1) / b); }
 // instead of three times, and waves never wait for each other after the
 // weights are staged. D = W (32 out channels x K) * X (K x 32 pixels): a lane
 // ends up with 16 channels of one pixel and stores them as 4-channel vectors.
-template <int KC>
-__global__ __launch_bounds__(256) void conv3x3_strip_kernel(const bf16* __restrict__ x1, int c1, int mode1,
-                                                            const bf16* __restrict__ x2, int c2, int N, int H,
-                                                            int W, const bf16* __restrict__ wgt,
+// MODE is x1's read (0 direct, 1 2x upsample, 2 2x2 average pool); element
+// offsets inside one image are 32-bit (the host checks they fit).
+template <int KC, int MODE>
+__global__ __launch_bounds__(256) void conv3x3_strip_kernel(const bf16* __restrict__ x1, int c1,
+                                                            const bf16* __restrict__ x2, int c2, int H, int W,
+                                                            const bf16* __restrict__ wgt,
                                                             const float* __restrict__ bias, int Cout, int act,
                                                             float slope, int rows, bf16* __restrict__ y,
                                                             float* __restrict__ yf) {
@@ -504,33 +506,36 @@ __global__ __launch_bounds__(256) void conv3x3_strip_kernel(const bf16* __restri
     const int y0 = int(blockIdx.y) * rows;
     const int ntile = (Cout + 31) / 32;
     const int n = int(blockIdx.z) / ntile, co0 = (int(blockIdx.z) % ntile) * 32;
-    const int H1 = mode1 == 1 ? H / 2 : (mode1 == 2 ? 2 * H : H), W1 = mode1 == 1 ? W / 2 : (mode1 == 2 ? 2 * W : W);
+    const int H1 = MODE == 1 ? H / 2 : (MODE == 2 ? 2 * H : H), W1 = MODE == 1 ? W / 2 : (MODE == 2 ? 2 * W : W);
+    const bf16* x1n = x1 + size_t(n) * H1 * W1 * c1;
+    const bf16* x2n = x2 ? x2 + size_t(n) * H * W * c2 : nullptr;
 
     for (int e = t; e < 9 * 32 * GW; e += 256) {
         const int g = e % GW, co = (e / GW) % 32, tap = e / (32 * GW);
         bf16x8 v = zero8();
-        if (co0 + co < Cout && g * 8 < Cin) v = load8(wgt + (size_t(co0 + co) * 9 + tap) * Cin + g * 8);
+        if (co0 + co < Cout && g * 8 < Cin) v = load8(wgt + unsigned(((co0 + co) * 9 + tap) * Cin + g * 8));
         *reinterpret_cast<bf16x8*>(s_w + (tap * 32 + co) * WP + g * 8) = v;
     }
     // (input row jj of the strip = image row y0 - 1 + jj, chunk ch) -> registers
     auto load_chunk = [&](int jj, int ch, bf16x8* v) {
         const int gy = y0 - 1 + jj;
+        const bool row_ok = gy >= 0 && gy < H;
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             const int e = l + q * 64;
             const int g = e % G, col = e / G;
             const int gx = x0 - 1 + col, c = ch * KC + g * 8;
             bf16x8 val = zero8();
-            if (col < SPX && gy >= 0 && gy < H && gx >= 0 && gx < W && c < Cin) {
+            if (row_ok && col < SPX && gx >= 0 && gx < W && c < Cin) {
                 if (c >= c1) {
-                    val = load8(x2 + ((size_t(n) * H + gy) * W + gx) * c2 + (c - c1));
-                } else if (mode1 == 1) {
-                    val = load8(x1 + ((size_t(n) * H1 + gy / 2) * W1 + gx / 2) * c1 + c);
-                } else if (mode1 == 2) {
-                    const size_t r0 = (size_t(n) * H1 + 2 * gy) * W1 + 2 * gx, r1 = r0 + W1;
-                    val = pool8(x1 + r0 * c1 + c, x1 + (r0 + 1) * c1 + c, x1 + r1 * c1 + c, x1 + (r1 + 1) * c1 + c);
+                    val = load8(x2n + unsigned((gy * W + gx) * c2 + (c - c1)));
+                } else if (MODE == 1) {
+                    val = load8(x1n + unsigned(((gy >> 1) * W1 + (gx >> 1)) * c1 + c));
+                } else if (MODE == 2) {
+                    const unsigned r0 = unsigned(((2 * gy) * W1 + 2 * gx) * c1 + c), r1 = r0 + unsigned(W1 * c1);
+                    val = pool8(x1n + r0, x1n + r0 + c1, x1n + r1, x1n + r1 + c1);
                 } else {
-                    val = load8(x1 + ((size_t(n) * H + gy) * W + gx) * c1 + c);
+                    val = load8(x1n + unsigned((gy * W + gx) * c1 + c));
                 }
             }
             v[q] = val;
@@ -822,7 +827,7 @@ static size_t strip_lds_bytes(int Cin, int KC) {
 static int strip_rows(const void* fn, size_t lds, int zdim, int H, int W) {
     thread_local std::unordered_map<size_t, int> resident_cache;
     int resident = 0;
-    const size_t key = lds * 2 + (fn == reinterpret_cast<const void*>(&conv3x3_strip_kernel<16>) ? 1 : 0);
+    const size_t key = reinterpret_cast<size_t>(fn) * 1000003u + lds;
     auto it = resident_cache.find(key);
     if (it != resident_cache.end()) {
         resident = it->second;
@@ -848,6 +853,27 @@ static int strip_rows(const void* fn, size_t lds, int zdim, int H, int W) {
     return best;
 }
 
+template <int KC>
+static hipError_t launch_strip(const void* x1, int c1, int mode, const void* x2, int c2, int N, int H, int W,
+                               const void* wgt, const float* bias, int Cout, int act, float slope, void* y, float* yf,
+                               size_t lds, hipStream_t s) {
+    const void* fn = mode == 1   ? reinterpret_cast<const void*>(&conv3x3_strip_kernel<KC, 1>)
+                     : mode == 2 ? reinterpret_cast<const void*>(&conv3x3_strip_kernel<KC, 2>)
+                                 : reinterpret_cast<const void*>(&conv3x3_strip_kernel<KC, 0>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+    const int rows = strip_rows(fn, lds, N * ((Cout + 31) / 32), H, W);
+    dim3 grid((W + 127) / 128, (H + rows - 1) / rows, N * ((Cout + 31) / 32));
+#define RFK_ST(M)                                                                                               \
+    conv3x3_strip_kernel<KC, M><<<grid, 256, lds, s>>>((const bf16*)x1, c1, (const bf16*)x2, c2, H, W,          \
+                                                       (const bf16*)wgt, bias, Cout, act, slope, rows, (bf16*)y, yf)
+    if (mode == 1) RFK_ST(1);
+    else if (mode == 2) RFK_ST(2);
+    else RFK_ST(0);
+#undef RFK_ST
+    return hipGetLastError();
+}
+
 template <int R, int C, int KCH>
 static void launch_conv_fwd(const void* x1, int c1, int mode1, const void* x2, int c2, int N, int H, int W,
                             const void* wgt, const float* bias, int Cout, int act, float slope, void* y, float* yf,
@@ -867,8 +893,11 @@ hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
     // faster there: 96->32 at 1080p 209 -> 186 us, 160->64 at 540x960
     // 192 -> 149 us; narrower inputs pay more for reloading weights per strip
     // than they save in LDS reads)
-    if (Cin >= 96 && Cout <= 96 && size_t(N) * H * W * std::max({c1, c2, Cout}) * 4 < (size_t(1) << 31) &&
-        size_t(Cout) * 9 * Cin < (size_t(1) << 31)) {
+    // 32-bit element offsets inside one image (the strip kernels' indexing)
+    const int H1 = x1_mode == 2 ? 2 * H : H, W1 = x1_mode == 2 ? 2 * W : W;
+    const bool offsets32 = size_t(H1) * W1 * c1 < (size_t(1) << 31) && size_t(H) * W * std::max(c2, Cout) < (size_t(1) << 31) &&
+                           size_t(Cout) * 9 * Cin < (size_t(1) << 31);
+    if (Cin >= 96 && Cout <= 96 && offsets32) {
         launch_wstat<4>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
         return hipGetLastError();
     }
@@ -877,22 +906,9 @@ hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
     // 128-channel and the widest-input layers run faster on the tiled kernel)
     const int KC = Cin <= 16 ? 16 : 32;
     const size_t lds = strip_lds_bytes(Cin, KC);
-    if (Cout <= 96 && lds <= 80 * 1024) {
-        const void* fn = KC == 16 ? reinterpret_cast<const void*>(&conv3x3_strip_kernel<16>)
-                                  : reinterpret_cast<const void*>(&conv3x3_strip_kernel<32>);
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-        if (e != hipSuccess) return e;
-        const int rows = strip_rows(fn, lds, N * ((Cout + 31) / 32), H, W);
-        dim3 grid((W + 127) / 128, (H + rows - 1) / rows, N * ((Cout + 31) / 32));
-        if (KC == 16)
-            conv3x3_strip_kernel<16><<<grid, 256, lds, s>>>((const bf16*)x1, c1, x1_mode, (const bf16*)x2, c2, N, H, W,
-                                                            (const bf16*)wgt, bias, Cout, act, slope, rows, (bf16*)y,
-                                                            yf);
-        else
-            conv3x3_strip_kernel<32><<<grid, 256, lds, s>>>((const bf16*)x1, c1, x1_mode, (const bf16*)x2, c2, N, H, W,
-                                                            (const bf16*)wgt, bias, Cout, act, slope, rows, (bf16*)y,
-                                                            yf);
-        return hipGetLastError();
+    if (Cout <= 96 && lds <= 80 * 1024 && offsets32) {
+        if (KC == 16) return launch_strip<16>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s);
+        return launch_strip<32>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s);
     }
     if (Cout <= 32) {
         if (Cin <= 16)
