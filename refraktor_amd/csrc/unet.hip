@@ -477,6 +477,40 @@ unsigned nb(size_t n, unsigned b = 256) { return unsigned((n + b - 1) / b); }
 
 }  // namespace
 
+// Eight channels [c, c + 8) of input pixel (gy, gx) of the virtual concat
+// (x1 through MODE: 0 direct, 1 nearest 2x upsample, 2 2x2 average pool;
+// then x2), zero where `ok` is false. Branch-free: the coordinates and the
+// channel are clamped into the tensors first, so the load is always in
+// bounds and a select applies `ok` afterwards. 32-bit offsets inside one
+// image (x1n, x2n point at the image).
+template <int MODE>
+__device__ __forceinline__ bf16x8 load_px8(const bf16* __restrict__ x1n, int c1, const bf16* __restrict__ x2n, int c2,
+                                           int H, int W, int gy, int gx, int c, bool ok) {
+    gy = min(max(gy, 0), H - 1);
+    gx = min(max(gx, 0), W - 1);
+    const bool use2 = c >= c1 && c2 > 0;
+    bf16x8 v;
+    if (MODE == 0) {
+        const int cx = use2 ? min(c - c1, c2 - 8) : min(c, c1 - 8);
+        const bf16* p = use2 ? x2n + unsigned((gy * W + gx) * c2 + cx) : x1n + unsigned((gy * W + gx) * c1 + cx);
+        v = load8(p);
+    } else if (MODE == 1) {
+        const int cx = use2 ? min(c - c1, c2 - 8) : min(c, c1 - 8);
+        const bf16* p = use2 ? x2n + unsigned((gy * W + gx) * c2 + cx)
+                             : x1n + unsigned(((gy >> 1) * (W >> 1) + (gx >> 1)) * c1 + cx);
+        v = load8(p);
+    } else {
+        if (use2) {
+            v = load8(x2n + unsigned((gy * W + gx) * c2 + min(c - c1, c2 - 8)));
+        } else {
+            const int cx = min(c, c1 - 8), W1 = 2 * W;
+            const unsigned r0 = unsigned(((2 * gy) * W1 + 2 * gx) * c1 + cx), r1 = r0 + unsigned(W1 * c1);
+            v = pool8(x1n + r0, x1n + r0 + c1, x1n + r1, x1n + r1 + c1);
+        }
+    }
+    return ok ? v : zero8();
+}
+
 // Every wave walks its own 32-pixel column strip down `rows` output rows for
 // 32 output channels (grid.z = images x 32-channel tiles), input-stationary:
 // each staged input
@@ -525,20 +559,8 @@ __global__ __launch_bounds__(256) void conv3x3_strip_kernel(const bf16* __restri
             const int e = l + q * 64;
             const int g = e % G, col = e / G;
             const int gx = x0 - 1 + col, c = ch * KC + g * 8;
-            bf16x8 val = zero8();
-            if (row_ok && col < SPX && gx >= 0 && gx < W && c < Cin) {
-                if (c >= c1) {
-                    val = load8(x2n + unsigned((gy * W + gx) * c2 + (c - c1)));
-                } else if (MODE == 1) {
-                    val = load8(x1n + unsigned(((gy >> 1) * W1 + (gx >> 1)) * c1 + c));
-                } else if (MODE == 2) {
-                    const unsigned r0 = unsigned(((2 * gy) * W1 + 2 * gx) * c1 + c), r1 = r0 + unsigned(W1 * c1);
-                    val = pool8(x1n + r0, x1n + r0 + c1, x1n + r1, x1n + r1 + c1);
-                } else {
-                    val = load8(x1n + unsigned((gy * W + gx) * c1 + c));
-                }
-            }
-            v[q] = val;
+            v[q] = load_px8<MODE>(x1n, c1, x2n, c2, H, W, gy, gx, c,
+                                  row_ok && col < SPX && gx >= 0 && gx < W && c < Cin);
         }
     };
     float bv[16];
@@ -590,8 +612,9 @@ __global__ __launch_bounds__(256) void conv3x3_strip_kernel(const bf16* __restri
         if (jj >= rows + 2) return;
         const int gy = y0 - 1 + jj;
         const bool in_img = gy >= 0 && gy < H;
-        const bool vA = jj < rows && y0 + jj < H;
-        const bool vB = jj >= 1 && jj - 1 < rows && y0 + jj - 1 < H;
+        // all three MFMAs run unconditionally (rows outside the strip or the
+        // image collect junk that is never stored and is cleared on reuse):
+        // no branch between an operand read and its MFMA
         const bool vC = jj >= 2 && y0 + jj - 2 < H;
         for (int ch = 0; ch < nch; ch++) {
             // the wave's previous reads of s_x are issued before these writes
@@ -619,18 +642,12 @@ __global__ __launch_bounds__(256) void conv3x3_strip_kernel(const bf16* __restri
                 for (int ks = 0; ks < KC / 16; ks++) {
                     const bf16x8 xb = *reinterpret_cast<const bf16x8*>(s_x + (m + dx) * XP + ks * 16 + kh * 8);
                     const int k = ch * KC + ks * 16 + kh * 8;
-                    if (vA) {
-                        const bf16x8 w = *reinterpret_cast<const bf16x8*>(s_w + ((0 + dx) * 32 + m) * WP + k);
-                        aA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, xb, aA, 0, 0, 0);
-                    }
-                    if (vB) {
-                        const bf16x8 w = *reinterpret_cast<const bf16x8*>(s_w + ((3 + dx) * 32 + m) * WP + k);
-                        aB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, xb, aB, 0, 0, 0);
-                    }
-                    if (vC) {
-                        const bf16x8 w = *reinterpret_cast<const bf16x8*>(s_w + ((6 + dx) * 32 + m) * WP + k);
-                        aC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, xb, aC, 0, 0, 0);
-                    }
+                    const bf16x8 w0 = *reinterpret_cast<const bf16x8*>(s_w + ((0 + dx) * 32 + m) * WP + k);
+                    const bf16x8 w1 = *reinterpret_cast<const bf16x8*>(s_w + ((3 + dx) * 32 + m) * WP + k);
+                    const bf16x8 w2 = *reinterpret_cast<const bf16x8*>(s_w + ((6 + dx) * 32 + m) * WP + k);
+                    aA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, xb, aA, 0, 0, 0);
+                    aB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, xb, aB, 0, 0, 0);
+                    aC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2, xb, aC, 0, 0, 0);
                 }
             }
         }
@@ -691,6 +708,8 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wstat_kernel(const bf16* __res
             const int e = l + q * 64;
             const int g = e % G, col = e / G;
             const int gx = x0 - 1 + col, c = ch * KC + g * 8;
+            // (branches, not the clamped branch-free load: measured faster
+            // here, where whole rows and chunks fall outside more often)
             bf16x8 val = zero8();
             if (row_ok && col < SPX && gx >= 0 && gx < W && c < Cin) {
                 if (c >= c1) {
