@@ -222,11 +222,62 @@ __global__ void de_gather_kernel(const BinT* __restrict__ hist, const int* __res
     }
     atomicMax(&s_maxlv, maxlv);
     __syncthreads();
+    // the span's non-empty sources, compacted in span (row-major) order: a
+    // contiguous range of entries per thread, counted, scanned, written
+    unsigned short* s_list = reinterpret_cast<unsigned short*>(s_iz + nlevels);
+    __shared__ int s_wave_tot[TILE * TILE / 64], s_count;
+    {
+        const int total = span * span, per = (total + TILE * TILE - 1) / (TILE * TILE);
+        const int i0 = min(tid * per, total), i1 = min(i0 + per, total);
+        int cnt = 0;
+        for (int i = i0; i < i1; i++) cnt += s_lvl[i] >= 0;
+        int incl = cnt;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(incl, d);
+            if (__lane_id() >= unsigned(d)) incl += u;
+        }
+        if (__lane_id() == 63) s_wave_tot[tid >> 6] = incl;
+        __syncthreads();
+        int base = 0, all = 0;
+        for (int w = 0; w < TILE * TILE / 64; w++) {
+            if (w < (tid >> 6)) base += s_wave_tot[w];
+            all += s_wave_tot[w];
+        }
+        int k = base + incl - cnt;
+        for (int i = i0; i < i1; i++)
+            if (s_lvl[i] >= 0) s_list[k++] = (unsigned short)((i % span) | ((i / span) << 8));
+        if (tid == 0) s_count = all;
+        __syncthreads();
+    }
     const int ox = int(blockIdx.x) * TILE + int(threadIdx.x), oy = int(blockIdx.y) * TILE + int(threadIdx.y);
     if (ox >= int(width) || oy >= int(height)) return;
     // no source of this span reaches further than its widest kernel
     const int reach = min(radius, (s_maxlv + 15) / 16);
     float4 acc = {0.f, 0.f, 0.f, 0.f};
+    if (s_count <= (2 * reach + 1) * (2 * reach + 1)) {
+        // fewer sources than window offsets: visit the sources instead, in
+        // the same (row-major) order as the window loop below, with the same
+        // arithmetic, so both paths give the same bits
+        const int cx = int(threadIdx.x) + radius, cy = int(threadIdx.y) + radius;
+        for (int i = 0; i < s_count; i++) {
+            const int e = s_list[i];
+            const int sx = e & 0xFF, sy = e >> 8;
+            const int dx = sx - cx, dy = sy - cy;
+            const int si = sy * span + sx;
+            const int lv = s_lvl[si];
+            const float h = float(lv) * (1.0f / 16.0f);
+            const float d2 = float(dx * dx + dy * dy);
+            if (d2 > h * h) continue;
+            const float wgt = (d2 == 0.0f ? 1.0f : __expf(-2.0f * d2 / (h * h))) * s_iz[lv];
+            const float4 c = s_col[si];
+            acc.x = fmaf(c.x, wgt, acc.x);
+            acc.y = fmaf(c.y, wgt, acc.y);
+            acc.z = fmaf(c.z, wgt, acc.z);
+            acc.w = fmaf(c.w, wgt, acc.w);
+        }
+        out[size_t(oy) * width + size_t(ox)] = acc;
+        return;
+    }
     for (int dy = -reach; dy <= reach; dy++) {
         const int row = (int(threadIdx.y) + radius + dy) * span + int(threadIdx.x) + radius;
         for (int dx = -reach; dx <= reach; dx++) {
@@ -456,7 +507,8 @@ static hipError_t density_estimate_t(const BinT* hist, uint32_t width, uint32_t 
                                                       level_scratch);
     constexpr int T = 16;
     const int span = T + 2 * radius;
-    size_t shmem = size_t(span) * span * (sizeof(float4) + sizeof(int)) + size_t(nlevels) * sizeof(float);
+    size_t shmem = size_t(span) * span * (sizeof(float4) + sizeof(int) + sizeof(unsigned short)) +
+                   size_t(nlevels) * sizeof(float);
     dim3 grid((width + T - 1) / T, (height + T - 1) / T);
     de_gather_kernel<T><<<grid, dim3(T, T), shmem, s>>>(hist, level_scratch, width, height, radius, inv_z, nlevels,
                                                         reinterpret_cast<float4*>(out_rgba));
