@@ -317,3 +317,41 @@ def test_density_hdr_matches_oracle(small):
     got = render.density_hdr(img, bins)
     want = oracle.density_hdr(img, bins)
     np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-5)
+
+
+def test_density_estimate_paths_agree(small):
+    """a tile with fewer non-empty sources than window offsets visits its
+    source list, a crowded tile its window: both give the same bits. Dense
+    sources (kernel radius below one pixel: they reach only their own pixel)
+    fill the free pixels of the left half in B; every other pixel must come
+    out as in A"""
+    import ctypes as C
+    from refraktor_amd import _lib
+    f, k = small
+    w, h = 64, 48
+    st = k.warmup(k.pack_samples(w, h), w, h, seed=3, fuse=4, walkers=1024)
+    inf = st.info()
+    assert inf.hist_bits == 32
+    rng = np.random.default_rng(5)
+    yy, xx = np.mgrid[0:h, 0:w]
+    sparse = rng.random((h, w)) < 0.05
+    A = np.zeros((h, w, 4), np.uint32)
+    A[sparse, :3] = rng.integers(0, 256, size=(int(sparse.sum()), 3))
+    A[sparse, 3] = 255  # one full-opacity draw: the widest kernel
+    dense = ~sparse & (xx < 32)
+    B = A.copy()
+    B[dense, :3] = rng.integers(0, 256, size=(int(dense.sum()), 3)) * 4096
+    B[dense, 3] = 255 * 4096  # 4096 draws: kernel radius 9 / 4096^0.4 < 1
+    outs = []
+    for hist in (A, B):
+        hist = np.ascontiguousarray(hist)
+        _lib.check(_lib.lib().rfk_memcpy_htod(C.c_void_p(inf.histogram), hist.ctypes.data_as(C.c_void_p), hist.nbytes))
+        outs.append(st.density_estimate(9.0))
+    # the span of the first 16x16 tile (radius 9 around it) holds fewer
+    # sources than the 19x19 window in A and more in B
+    span_a = int(sparse[:25, :25].sum())
+    span_b = span_a + int(dense[:25, :25].sum())
+    assert span_a < 19 * 19 < span_b, (span_a, span_b)
+    keep = ~dense
+    np.testing.assert_array_equal(outs[0][keep], outs[1][keep])
+    assert (outs[0][keep] != 0).any()
