@@ -105,8 +105,11 @@ def main() -> int:
             st = states[(id(kern), walkers)] = kern.warmup(samples, W, H, seed=seed, fuse=20, walkers=walkers)
         else:
             st.rewarm(samples, seed=seed, fuse=20)
-        res = st.bin(iters=iters)
+        # density estimation is enqueued behind the bin call; the host only
+        # waits for the call's counters (the tonemap needs its quality)
+        st.bin_async(iters=iters)
         check(lib.rfk_density_estimate(st.handle, DE_R, 0.0, 0.4, de.ptr))
+        res = st.bin_wait()
         out = images.get(fmt)
         if out is None:
             out = images[fmt] = render.DeviceBuffer(npx * _lib.PIXEL_BYTES[fmt])

@@ -63,6 +63,7 @@ struct rfk_state {
     uint32_t* stop_host = nullptr;                 // pinned, mapped
     uint32_t* stop_dev = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_result = nullptr;  // a bin call's counters have reached the host
     double quality = 0.0;
     uint64_t passes_done = 0;
     bool pending = false;
@@ -161,6 +162,7 @@ void free_state_buffers(rfk_state* s) {
     if (s->stop_host) (void)hipHostFree(s->stop_host);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->ev_result) (void)hipEventDestroy(s->ev_result);
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -643,6 +645,7 @@ int rfk_warmup_ex(rfk_kernel* k, void* stream, const double* samples, size_t n_r
     RFK_HIPC(hipHostGetDevicePointer(reinterpret_cast<void**>(&s->stop_dev), s->stop_host, 0));
     RFK_HIPC(hipEventCreate(&s->ev0));
     RFK_HIPC(hipEventCreate(&s->ev1));
+    RFK_HIPC(hipEventCreateWithFlags(&s->ev_result, hipEventDisableTiming));
     RFK_HIPC(hipEventCreateWithFlags(&s->ev_staging, hipEventDisableTiming));
     std::vector<int8_t> kind(size_t(k->L.size_reals), 0);
     for (size_t ai : k->L.affine_indices)
@@ -699,13 +702,14 @@ int rfk_bin_async(rfk_kernel* k, rfk_state* s, const rfk_bailout* bo) {
     }
     RFK_HIP(hipEventRecord(s->ev1, s->stream));
     // density statistics after every bin, as the reference does (flame_compiler.cpp:928-937)
-    RFK_HIP(hipMemsetAsync(s->dstats, 0, 98 * sizeof(unsigned long long), s->stream));
+    RFK_HIP(rfk::sk::fill_u32(reinterpret_cast<uint32_t*>(s->dstats), 0u, 2 * 98, s->stream));
     RFK_HIP(rfk::sk::density_histogram(s->hist, s->hist64, size_t(s->width) * s->height, s->dstats, s->dstats + 97,
                                        s->stream));
     RFK_HIP(hipMemcpyAsync(s->counters_host, s->counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            s->stream));
     RFK_HIP(hipMemcpyAsync(s->counters_host + 8, s->dstats, 98 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            s->stream));
+    RFK_HIP(hipEventRecord(s->ev_result, s->stream));
     s->pending = true;
     s->pending_passes = passes;
     return RFK_OK;
@@ -714,7 +718,9 @@ int rfk_bin_async(rfk_kernel* k, rfk_state* s, const rfk_bailout* bo) {
 int rfk_bin_wait(rfk_state* s, rfk_bin_result* out) {
     if (!s || !out) return fail(RFK_ERR_INVALID, "null argument");
     if (!s->pending) return fail(RFK_ERR_INVALID, "no bin call pending");
-    RFK_HIP(hipStreamSynchronize(s->stream));
+    // the call's own work only: what the caller enqueued behind it (density
+    // estimation, tonemap) keeps running
+    RFK_HIP(hipEventSynchronize(s->ev_result));
     s->pending = false;
     const unsigned long long* c = s->counters_host;
     const double bins = double(s->width) * s->height;
