@@ -11,6 +11,11 @@ Both are scored by PSNR against a converged 2^32-iteration render of the same
 flame. `value` is the frames/s of the denoised path when it reaches 40 dB
 (else of the plain path), the metric "1080p frames/sec at PSNR>=40dB".
 
+Frames are independent renders (seed per frame); --pipeline N keeps N frames
+in flight, each on its own streams and buffers, for both paths alike, so one
+frame's U-Net and density estimation run beside the next frame's binning.
+Each path is also timed one frame at a time (`sequential_ms`, the latency).
+
 Prints one JSON line with both paths, iterations/s, the binning roofline
 (16 B of histogram increment per draw over the bin call's device time; the
 HBM traffic of the same call from the committed rocprofv3 counter summary)
@@ -64,6 +69,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
     ap.add_argument("--no-psnr", action="store_true", help="skip the converged reference render")
     ap.add_argument("--no-denoise", action="store_true", help="plain path only")
+    ap.add_argument("--pipeline", type=int, default=3,
+                    help="frames in flight, each on its own streams (both paths alike); 1 = one frame at a time")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
@@ -95,51 +102,61 @@ def main() -> int:
     lib = _lib.lib()
     npx = W * H
 
-    # one state and one set of images per kernel, re-warmed for every frame
-    states, images = {}, {}
-    de = render.DeviceBuffer(npx * 16)
+    weights = Path(args.weights) if args.weights else denoise.DEFAULT_WEIGHTS
+    use_den = not args.no_denoise and weights.exists()
 
-    def plain(seed: int, kern=k, iters=ITERS, fmt=_lib.PIXEL_HALF4, walkers=args.walkers):
-        st = states.get((id(kern), walkers))
+    class Lane:
+        """one frame in flight: its states (re-warmed for every frame), images,
+        feature buffers and denoiser; all of a frame's work runs on its
+        state's stream, so frames of different lanes overlap on the device"""
+
+        def __init__(self):
+            self.states, self.images = {}, {}
+            self.de = render.DeviceBuffer(npx * 16)
+            self.den = denoise.Denoiser(weights=weights) if use_den else None
+            self.feat = render.DeviceBuffer(npx * denoise.FEAT_C * 2)
+            self.base = render.DeviceBuffer(npx * 12)
+            self.rgb = render.DeviceBuffer(npx * 12)
+
+    lanes = [Lane() for _ in range(max(1, args.pipeline))]
+
+    def plain(lane: Lane, seed: int, kern=k, iters=ITERS, fmt=_lib.PIXEL_HALF4, walkers=args.walkers):
+        st = lane.states.get((id(kern), walkers))
         if st is None:
-            st = states[(id(kern), walkers)] = kern.warmup(samples, W, H, seed=seed, fuse=20, walkers=walkers)
+            st = lane.states[(id(kern), walkers)] = kern.warmup(samples, W, H, seed=seed, fuse=20, walkers=walkers)
         else:
             st.rewarm(samples, seed=seed, fuse=20)
         # density estimation is enqueued behind the bin call; the host only
         # waits for the call's counters (the tonemap needs its quality)
         st.bin_async(iters=iters)
-        check(lib.rfk_density_estimate(st.handle, DE_R, 0.0, 0.4, de.ptr))
+        check(lib.rfk_density_estimate(st.handle, DE_R, 0.0, 0.4, lane.de.ptr))
         res = st.bin_wait()
-        out = images.get(fmt)
+        out = lane.images.get(fmt)
         if out is None:
-            out = images[fmt] = render.DeviceBuffer(npx * _lib.PIXEL_BYTES[fmt])
+            out = lane.images[fmt] = render.DeviceBuffer(npx * _lib.PIXEL_BYTES[fmt])
         inf = st.info()
         ta = _lib.TonemapArgs(inf.quality, gamma, bright, vib, 0.0)
         # everything of a frame on the state's stream: the next re-warm waits for it
-        check(lib.rfk_tonemap_image(de.ptr, W, H, out.ptr, fmt, C.byref(ta), inf.stream))
+        check(lib.rfk_tonemap_image(lane.de.ptr, W, H, out.ptr, fmt, C.byref(ta), inf.stream))
         return st, res, out
 
-    weights = Path(args.weights) if args.weights else denoise.DEFAULT_WEIGHTS
-    den = denoise.Denoiser(weights=weights) if (not args.no_denoise and weights.exists()) else None
-    feat = render.DeviceBuffer(npx * denoise.FEAT_C * 2)
-    base = render.DeviceBuffer(npx * 12)
-    rgb = render.DeviceBuffer(npx * 12)
-
-    def denoised(seed: int):
-        st, res, tm = plain(seed, k, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers)
+    def denoised(lane: Lane, seed: int):
+        st, res, tm = plain(lane, seed, k, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers)
         inf = st.info()
-        check(lib.rfk_denoiser_features(inf.histogram, inf.hist_bits, tm.ptr, npx, feat.ptr, base.ptr, inf.stream))
-        den.forward(feat, base, 1, H, W, rgb, stream=inf.stream)
-        return st, res, rgb
+        check(lib.rfk_denoiser_features(inf.histogram, inf.hist_bits, tm.ptr, npx, lane.feat.ptr, lane.base.ptr,
+                                        inf.stream))
+        lane.den.forward(lane.feat, lane.base, 1, H, W, lane.rgb, stream=inf.stream)
+        return st, res, lane.rgb
 
-    def timed(fn, steps: int, warm: int):
-        for i in range(warm):
-            fn(args.seed + 1000 + i)
+    def timed(fn, steps: int, warm: int, nl: int = len(lanes)):
+        """`steps` frames round-robin over the first `nl` lanes"""
+        for i in range(max(warm, nl)):  # every lane warm
+            fn(lanes[i % nl], args.seed + 1000 + i)
         lib.rfk_synchronize()
         t0 = time.perf_counter()
         kms, draws, passes, last = [], [], [], None
         for i in range(steps):
-            st, res, img = fn(args.seed + i)
+            st, res, img = fn(lanes[i % nl], args.seed + i)
             kms.append(res["kernel_ms"])
             draws.append(res["total_draws"])
             passes.append(res["total_passes"])
@@ -150,7 +167,7 @@ def main() -> int:
 
     ms_plain, kms, dr, ps, img_plain = timed(plain, args.steps, args.warmup)
     ms_den = img_den = None
-    if den is not None:
+    if use_den:
         ms_den, _, _, _, img_den = timed(denoised, args.steps, args.warmup)
 
     psnr_plain = psnr_den = None
@@ -158,11 +175,21 @@ def main() -> int:
         # converged reference: 2^32 iterations into u64 bins (no wrap in hot pixels)
         plain_img = img_plain.download(np.float16, (H, W, 4)).astype(np.float32)
         den_img = img_den.download(np.float32, (H, W, 3)) if img_den is not None else None
-        _, _, ref_buf = plain(args.seed + 5000, k64, 1 << 32, _lib.PIXEL_FLOAT4)
+        _, _, ref_buf = plain(lanes[0], args.seed + 5000, k64, 1 << 32, _lib.PIXEL_FLOAT4)
         ref = ref_buf.download(np.float32, (H, W, 4))
         psnr_plain = psnr(plain_img, ref)
         if den_img is not None:
             psnr_den = psnr(den_img, ref)
+
+    # one frame at a time: the per-frame latency of each path, and the bin
+    # call's device time without other frames' kernels beside it
+    if len(lanes) > 1:
+        seq_plain, kms, dr, ps, _ = timed(plain, args.steps, args.warmup, 1)
+    else:
+        seq_plain = ms_plain
+    seq_den = None
+    if use_den:
+        seq_den = timed(denoised, args.steps, args.warmup, 1)[0] if len(lanes) > 1 else ms_den
 
     # binning: algorithmic bytes = 16 B per draw (one RGBA u32 increment) over
     # the bin call's device time (HIP events on the state's stream: the
@@ -184,8 +211,8 @@ def main() -> int:
 
     fps_plain = 1e3 / ms_plain
     fps_den = 1e3 / ms_den if ms_den else None
-    use_den = fps_den is not None and psnr_den is not None and psnr_den >= 40.0
-    value = fps_den if use_den else fps_plain
+    den_ok = fps_den is not None and psnr_den is not None and psnr_den >= 40.0
+    value = fps_den if den_ok else fps_plain
     line = {
         "metric": "1080p frames/sec at PSNR>=40dB vs converged plain render; iterations/sec",
         "value": value,
@@ -193,7 +220,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_den if use_den else ms_plain,
+        "ms_per_step": ms_den if den_ok else ms_plain,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -204,15 +231,17 @@ def main() -> int:
                    "width": W, "height": H, "iterations": ITERS, "walkers": args.walkers,
                    "threads_per_block": args.tpb, "atomic_width": args.atomic_width, "bin_tile_log2": args.tile,
                    "hist_bits": args.hist_bits, "binning": args.binning,
-                   "path": "denoised" if use_den else "plain"},
-        "paths": {"plain": {"fps": fps_plain, "ms": ms_plain, "iterations": ITERS, "psnr_db": psnr_plain},
+                   "path": "denoised" if den_ok else "plain", "frames_in_flight": args.pipeline},
+        "paths": {"plain": {"fps": fps_plain, "ms": ms_plain, "sequential_ms": seq_plain, "iterations": ITERS,
+                            "psnr_db": psnr_plain},
                   "denoised": None if fps_den is None else
-                  {"fps": fps_den, "ms": ms_den, "iterations": args.low_iters, "walkers": args.low_walkers,
+                  {"fps": fps_den, "ms": ms_den, "sequential_ms": seq_den, "iterations": args.low_iters,
+                   "walkers": args.low_walkers,
                    "psnr_db": psnr_den,
                    "speedup_vs_plain": fps_den / fps_plain, "weights": str(weights.name)}},
         "iterations_per_s": it_per_s,
         "iterate_kernel_ms": kms,
-        "psnr_db_vs_converged": psnr_den if use_den else psnr_plain,
+        "psnr_db_vs_converged": psnr_den if den_ok else psnr_plain,
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "rfk_iterate" + (" + queue sort/bin kernels" if queue else ""),
