@@ -219,10 +219,7 @@ def test_every_variation_compiles(db):
     xfs = []
     for i in range(0, len(names), 4):
         group = names[i:i + 4]
-        vars_ = {}
-        for n in group:
-            _, _, _ = None, None, None
-            vars_[n] = 0.25
+        vars_ = {n: 0.25 for n in group}
         xfs.append(fl.xform(1.0, 0.5, 0.5, [fl.vlink([1, 0, 0, 1, 0, 0], vars_)]))
     fd = fl.make_flame(xfs, [[0, 0, 1]] * 256)
     # parameterised variations need the object form
