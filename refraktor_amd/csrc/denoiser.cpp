@@ -186,7 +186,11 @@ struct level_shape {
 // instead of being stored (the average pools stay separate: reading four
 // pixels per tap in the convolution's staging costs more than the pool);
 // training keeps every activation for the backward pass.
-int forward(rfk_denoiser* d, int N, int H, int W, const void* feat, float* rout, hipStream_t s, bool fused = false) {
+// With out_rgb (inference only), the output layer writes clamp(base + r)
+// directly (rout untouched) where the strip kernel takes its shape; the
+// return value 1 tells the caller that out_rgb is done.
+int forward(rfk_denoiser* d, int N, int H, int W, const void* feat, float* rout, hipStream_t s, bool fused = false,
+            const float* base = nullptr, float* out_rgb = nullptr, bool* rgb_done = nullptr) {
     int rc = 0;
     const level_shape L[4] = {{H, W}, {H / 2, W / 2}, {H / 4, W / 4}, {H / 8, W / 8}};
     auto A = [&](const std::string& name, int lvl, int C) -> void* {
@@ -227,6 +231,15 @@ int forward(rfk_denoiser* d, int N, int H, int W, const void* feat, float* rout,
         if ((rc = conv(11, c1, nullptr, 1, d1, nullptr))) return rc;
         if ((rc = conv(12, d1, e0, 0, c0, nullptr, 1))) return rc;
         if ((rc = conv(13, c0, nullptr, 0, d0, nullptr))) return rc;
+        if (out_rgb) {
+            const conv_def& c = d->convs[14];
+            if (rfk::nn::conv3x3_forward_rgb(d0, c.c1, N, H, W, static_cast<char*>(d->wbf) + c.w_off * 2,
+                                             d->b + c.b_off, c.cout, c.act, base, out_rgb, s) == hipSuccess) {
+                if (rgb_done) *rgb_done = true;
+                return RFK_OK;
+            }
+            (void)hipGetLastError();
+        }
         return conv(14, d0, nullptr, 0, nullptr, rout);
     }
     void *a0 = A("a0", 0, ch[0]), *e0 = A("e0", 0, ch[0]), *p0 = A("p0", 1, ch[0]);
@@ -474,7 +487,9 @@ int rfk_denoiser_forward(rfk_denoiser* d, const void* feat, const float* base, i
     const size_t npix = size_t(N) * H * W;
     float* r = static_cast<float*>(act_buf(d, N, H, W, "r", npix * kOut * 4, rc).p);
     if (rc) return rc;
-    if ((rc = forward(d, N, H, W, feat, r, s, true))) return rc;
+    bool done = false;
+    if ((rc = forward(d, N, H, W, feat, r, s, true, base, out_rgb, &done))) return rc;
+    if (done) return RFK_OK;
     NN_HIP(rfk::nn::residual_out(base, 3, r, kOut, npix, out_rgb, s));
     NN_HIP(rfk::nn::clamp_rgb(out_rgb, npix * 3, out_rgb, s));
     return RFK_OK;

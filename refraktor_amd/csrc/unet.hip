@@ -528,7 +528,8 @@ __global__ __launch_bounds__(256) void conv3x3_strip_kernel(const bf16* __restri
                                                             const bf16* __restrict__ wgt,
                                                             const float* __restrict__ bias, int Cout, int act,
                                                             float slope, int rows, bf16* __restrict__ y,
-                                                            float* __restrict__ yf) {
+                                                            float* __restrict__ yf, const float* __restrict__ rbase,
+                                                            float* __restrict__ rgb) {
     constexpr int SPX = 34, XP = KC + PAD, G = KC / 8, NQ = (SPX * G + 63) / 64;
     extern __shared__ __attribute__((aligned(16))) unsigned char conv_smem[];
     const int Cin = c1 + c2;
@@ -574,6 +575,16 @@ __global__ __launch_bounds__(256) void conv3x3_strip_kernel(const bf16* __restri
     const int px = x0 + m;
     auto store_row = [&](int yy, const f32x16& acc) {
         if (px >= W) return;
+        if (rgb) {
+            // the network's output layer: rgb = clamp(base + first three channels)
+            if (co0 == 0 && kh == 0) {
+                const size_t q = ((size_t(n) * H + yy) * W + px) * 3;
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    rgb[q + j] = fminf(fmaxf(rbase[q + j] + act_fwd(acc[j] + bv[j], act, slope), 0.f), 1.f);
+            }
+            return;
+        }
         const size_t o = ((size_t(n) * H + yy) * W + px) * Cout;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -739,15 +750,27 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wstat_kernel(const bf16* __res
     load_chunk(0, 0, pf[0]);
     load_chunk(1, 0, pf[1]);
     const int co_l = co0 + m;
+    // whole 32-channel tiles on both sides (every layer of the network): the
+    // weight loads need no per-lane bounds
+    const bool full = (Cin & 31) == 0 && (Cout & 31) == 0;
     for (int ch = 0; ch < nch; ch++) {
         bf16x8 wf[9][2];
+        if (full) {
+            const bf16* wrow = wgt + unsigned(co_l * 9 * Cin + ch * KC + kh * 8);
 #pragma unroll
-        for (int tap = 0; tap < 9; tap++)
+            for (int tap = 0; tap < 9; tap++)
 #pragma unroll
-            for (int ks = 0; ks < 2; ks++) {
-                const int k = ch * KC + ks * 16 + kh * 8;
-                wf[tap][ks] = (co_l < Cout && k < Cin) ? load8(wgt + unsigned((co_l * 9 + tap) * Cin + k)) : zero8();
-            }
+                for (int ks = 0; ks < 2; ks++) wf[tap][ks] = load8(wrow + tap * Cin + ks * 16);
+        } else {
+#pragma unroll
+            for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+                for (int ks = 0; ks < 2; ks++) {
+                    const int k = ch * KC + ks * 16 + kh * 8;
+                    wf[tap][ks] =
+                        (co_l < Cout && k < Cin) ? load8(wgt + unsigned((co_l * 9 + tap) * Cin + k)) : zero8();
+                }
+        }
 #pragma unroll
         for (int jj = 0; jj < R + 2; jj++) {
             // the wave's reads of the previous row are issued before these
@@ -875,7 +898,7 @@ static int strip_rows(const void* fn, size_t lds, int zdim, int H, int W) {
 template <int KC>
 static hipError_t launch_strip(const void* x1, int c1, int mode, const void* x2, int c2, int N, int H, int W,
                                const void* wgt, const float* bias, int Cout, int act, float slope, void* y, float* yf,
-                               size_t lds, hipStream_t s) {
+                               size_t lds, hipStream_t s, const float* rbase = nullptr, float* rgb = nullptr) {
     const void* fn = mode == 1   ? reinterpret_cast<const void*>(&conv3x3_strip_kernel<KC, 1>)
                      : mode == 2 ? reinterpret_cast<const void*>(&conv3x3_strip_kernel<KC, 2>)
                                  : reinterpret_cast<const void*>(&conv3x3_strip_kernel<KC, 0>);
@@ -885,7 +908,8 @@ static hipError_t launch_strip(const void* x1, int c1, int mode, const void* x2,
     dim3 grid((W + 127) / 128, (H + rows - 1) / rows, N * ((Cout + 31) / 32));
 #define RFK_ST(M)                                                                                               \
     conv3x3_strip_kernel<KC, M><<<grid, 256, lds, s>>>((const bf16*)x1, c1, (const bf16*)x2, c2, H, W,          \
-                                                       (const bf16*)wgt, bias, Cout, act, slope, rows, (bf16*)y, yf)
+                                                       (const bf16*)wgt, bias, Cout, act, slope, rows, (bf16*)y, yf, \
+                                                       rbase, rgb)
     if (mode == 1) RFK_ST(1);
     else if (mode == 2) RFK_ST(2);
     else RFK_ST(0);
@@ -942,6 +966,20 @@ hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
         launch_conv_fwd<1, 4, 16>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
     }
     return hipGetLastError();
+}
+
+hipError_t conv3x3_forward_rgb(const void* x, int cin, int N, int H, int W, const void* wgt, const float* bias,
+                               int Cout, int act, const float* base, float* out_rgb, hipStream_t s) {
+    if ((cin % 8) || cin <= 0 || Cout < 3 || Cout > 32 || !base || !out_rgb) return hipErrorInvalidValue;
+    const int KC = cin <= 16 ? 16 : 32;
+    const size_t lds = strip_lds_bytes(cin, KC);
+    if (cin >= 96 || lds > 80 * 1024 || size_t(H) * W * std::max(cin, 3) >= (size_t(1) << 31))
+        return hipErrorInvalidValue;
+    if (KC == 16)
+        return launch_strip<16>(x, cin, 0, nullptr, 0, N, H, W, wgt, bias, Cout, act, 0.f, nullptr, nullptr, lds, s, base,
+                                out_rgb);
+    return launch_strip<32>(x, cin, 0, nullptr, 0, N, H, W, wgt, bias, Cout, act, 0.f, nullptr, nullptr, lds, s, base,
+                            out_rgb);
 }
 
 size_t conv3x3_wgrad_workspace(int c1, int c2, int Cout, int N, int H, int W, int* groups, int* tiles_per_group) {

@@ -16,6 +16,11 @@ namespace rfk::nn {
 hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N, int H, int W, const void* wgt,
                            const float* bias, int Cout, int act, float slope, void* y, float* yf, hipStream_t s,
                            int x1_mode = 0);
+// The inference network's output layer with the residual fused:
+// out_rgb[p][c] = clamp(base[p][c] + conv(x)[p][c], 0, 1) for c < 3 (FP32,
+// N*H*W*3). hipErrorInvalidValue when the strip kernel does not take the shape.
+hipError_t conv3x3_forward_rgb(const void* x, int cin, int N, int H, int W, const void* wgt, const float* bias,
+                               int Cout, int act, const float* base, float* out_rgb, hipStream_t s);
 // bytes of FP32 workspace conv3x3_wgrad needs
 size_t conv3x3_wgrad_workspace(int c1, int c2, int Cout, int N, int H, int W, int* groups, int* tiles_per_group);
 // dw = beta * dw + dL/dW (FP32, [Cout][9][Cin])
