@@ -85,9 +85,17 @@ typedef u32 hist_t;
 #define RFK_TILE_EMPTY 0xFFFFFFFFu
 #define RFK_TILE_FLUSH 64u
 
+// Per-wave tables of the wave-uniform Philox draws: rows [0, 16) hold the
+// xform picks of 16 groups of 4 passes, rows [16, 48) the permutation keys of
+// 32 groups of 2 passes. A wave fills its own rows on the vector unit, one
+// group per lane, when its pass leaves the table, and then reads one row per
+// pass instead of running Philox on the scalar unit.
+#define RFK_PICK_ROWS 16u
+#define RFK_PERM_ROWS 32u
 struct rfk_shared {
     u32 pal[256];
     Real pts[2][3][RFK_TPB];
+    u32x4 rng[RFK_TPB / 64][RFK_PICK_ROWS + RFK_PERM_ROWS];
     u64 hits, draws, bad, start;
     u64 published;  // hits already added to the call's counter (queue binning)
     u32 stop, item;
@@ -146,11 +154,6 @@ RFK_INLINE void rfk_tile_draw(rfk_shared& sm, hist_t* __restrict__ hist, const r
 }
 #endif
 
-RFK_INLINE u32 rfk_wave_pick(u32 pass, u32 block, u32 wave, u32 k0, u32 k1, u32x4& cache, bool refresh) {
-    if (refresh) cache = philox4x32_10({pass >> 2, block, wave, RFK_TAG_PICK}, k0, k1);
-    const u32 j = pass & 3u;
-    return j == 0 ? cache.x : j == 1 ? cache.y : j == 2 ? cache.z : cache.w;
-}
 
 // One block of walkers for `nfuse` un-binned passes followed by `npasses`
 // passes (binned when flags has RFK_FLAG_DRAW), starting at pass `pass0`.
@@ -221,7 +224,8 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, param_ptr P, co
     rs.init(a.seed, key1, gid, RFK_TAG_LANE);
     u64 hits = 0, draws = 0, bad = 0;
     u32 done = 0, buf = 0;
-    u32x4 pick_cache = {0u, 0u, 0u, 0u}, perm_cache = {0u, 0u, 0u, 0u};
+    u32 pick_base = 0u, perm_base = 0u;  // first group held by the wave's tables
+    const u32 lane = __lane_id();
 
     const u32 ntotal = nfuse + npasses;
 #if RFK_BIN_QUEUE
@@ -255,8 +259,19 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, param_ptr P, co
         pt = res;
         last = xid;
 #else
-        xid = rfk_select_xform(P, rfk_u01(rfk_wave_pick(pass, block, wave, a.seed, key1, pick_cache,
-                                                        i == 0 || (pass & 3u) == 0u)));
+        if (i == 0 || (pass >> 2) - pick_base >= RFK_PICK_ROWS) {
+            pick_base = pass >> 2;
+            // the wave's earlier reads of these rows are issued before the
+            // writes (LDS executes one wave's instructions in order)
+            __asm__ volatile("" ::: "memory");
+            if (lane < RFK_PICK_ROWS)
+                sm.rng[wave][lane] = philox4x32_10({pick_base + lane, block, wave, RFK_TAG_PICK}, a.seed, key1);
+            __asm__ volatile("" ::: "memory");
+        }
+        const u32x4 prow = sm.rng[wave][(pass >> 2) - pick_base];
+        const u32 pj = pass & 3u;
+        const u32 pick = __builtin_amdgcn_readfirstlane(pj == 0 ? prow.x : pj == 1 ? prow.y : pj == 2 ? prow.z : prow.w);
+        xid = rfk_select_xform(P, rfk_u01(pick));
         xid = __builtin_amdgcn_readfirstlane(xid);
         op = rfk_dispatch(xid, P, pt, rs);
 #endif
@@ -321,10 +336,17 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, param_ptr P, co
         y = pt.y;
         c = pt.z;
 #else
-        if (i == 0 || (pass & 1u) == 0u)
-            perm_cache = philox4x32_10({pass >> 1, block, 0u, RFK_TAG_PERM}, a.seed, key1);
-        const u32 wa = (pass & 1u) ? perm_cache.z : perm_cache.x;
-        const u32 wb = (pass & 1u) ? perm_cache.w : perm_cache.y;
+        if (i == 0 || (pass >> 1) - perm_base >= RFK_PERM_ROWS) {
+            perm_base = pass >> 1;
+            __asm__ volatile("" ::: "memory");
+            if (lane < RFK_PERM_ROWS)
+                sm.rng[wave][RFK_PICK_ROWS + lane] =
+                    philox4x32_10({perm_base + lane, block, 0u, RFK_TAG_PERM}, a.seed, key1);
+            __asm__ volatile("" ::: "memory");
+        }
+        const u32x4 krow = sm.rng[wave][RFK_PICK_ROWS + (pass >> 1) - perm_base];
+        const u32 wa = __builtin_amdgcn_readfirstlane((pass & 1u) ? krow.z : krow.x);
+        const u32 wb = __builtin_amdgcn_readfirstlane((pass & 1u) ? krow.w : krow.y);
         const u32 slot = rfk_permute(t, wa, wb, RFK_TPB_BITS);
         sm.pts[buf][0][slot] = pt.x;
         sm.pts[buf][1][slot] = pt.y;
