@@ -73,7 +73,7 @@ def main() -> int:
                     help="frames in flight, each on its own streams (both paths alike); 1 = one frame at a time")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
+        print(json.dumps({"error": "bench.py runs on one GPU; --gpus above 1 is not supported",
                           "n_gpus": args.gpus}))
         return 2
 
