@@ -9,7 +9,8 @@
 //   2. qseg       exclusive scan of the tile totals -> tile starts, and the
 //                 (tile, segment of <= 2^18 records) work items of step 4
 //   3. qscatter   counting sort by tile: each block's records, 8192 at a time,
-//                 ranked in LDS and written as tile runs at their offsets
+//                 ranked in LDS and written as tile runs at their offsets,
+//                 colour (u32) and pixel in tile (u16) in separate arrays
 //   4. qbin       one workgroup per work item sums its records in a 64x64
 //                 LDS tile, then adds the non-zero pixels into the histogram:
 //                 plain read-add-write when the work item is the tile's only
@@ -141,7 +142,7 @@ __global__ void __launch_bounds__(kThreads)
 __global__ void __launch_bounds__(kThreads)
     qscatter_kernel(const u64* __restrict__ rec, size_t per_part, uint32_t ntiles, uint32_t nparts,
                     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ tile_start,
-                    u64* __restrict__ sorted) {
+                    uint32_t* __restrict__ sorted_col, uint16_t* __restrict__ sorted_loc) {
     extern __shared__ uint32_t smem[];
     uint32_t* cursor = smem;
     uint32_t* lcount = cursor + ntiles;
@@ -198,7 +199,9 @@ __global__ void __launch_bounds__(kThreads)
         for (uint32_t k = threadIdx.x; k < used; k += kThreads) {
             const u64 v = piece[k];
             const uint32_t tile = rec_tile(v);
-            sorted[size_t(cursor[tile]) + (k - lstart[tile])] = v;
+            const size_t pos = size_t(cursor[tile]) + (k - lstart[tile]);
+            sorted_col[pos] = uint32_t(v);
+            sorted_loc[pos] = uint16_t((v >> 32) & (kTilePix - 1u));
         }
         __syncthreads();
         for (uint32_t t = threadIdx.x; t < ntiles; t += kThreads) cursor[t] += lcount[t];
@@ -236,7 +239,8 @@ __device__ __forceinline__ void add_bin(HistT* __restrict__ hist, size_t p, uint
 // stays below 2^26 and never carries into its neighbour.
 template <class HistT>
 __global__ void __launch_bounds__(kThreads)
-    qbin_kernel(const u64* __restrict__ sorted, const uint4* __restrict__ segments, const uint32_t* __restrict__ nseg,
+    qbin_kernel(const uint32_t* __restrict__ sorted_col, const uint16_t* __restrict__ sorted_loc,
+                const uint4* __restrict__ segments, const uint32_t* __restrict__ nseg,
                 uint32_t width, uint32_t height, uint32_t tiles_x, HistT* __restrict__ hist, bool packed64) {
     static_assert(kSegment <= (1u << 24), "32-bit halves of the LDS sums must not carry");
     __shared__ u64 acc[2][kTilePix];
@@ -250,17 +254,18 @@ __global__ void __launch_bounds__(kThreads)
     // U records per thread in flight before the first LDS add
     constexpr uint32_t U = 8;
     for (uint32_t i0 = sg.y + threadIdx.x; i0 < sg.z; i0 += kThreads * U) {
-        u64 v[U];
+        uint32_t col[U], loc[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             const uint32_t i = i0 + u * kThreads;
-            v[u] = i < sg.z ? sorted[i] : 0ull;
+            col[u] = i < sg.z ? sorted_col[i] : 0u;
+            loc[u] = i < sg.z ? sorted_loc[i] : 0u;
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             if (i0 + u * kThreads >= sg.z) break;
-            const uint32_t local = uint32_t(v[u] >> 32) & (kTilePix - 1u);
-            const uint32_t c = uint32_t(v[u]);
+            const uint32_t local = loc[u];
+            const uint32_t c = col[u];
             atomicAdd(&acc[0][local], u64(c & 0xFFu) | (u64((c >> 8) & 0xFFu) << 32));
             atomicAdd(&acc[1][local], u64((c >> 16) & 0xFFu) | (u64(c >> 24) << 32));
         }
@@ -301,13 +306,20 @@ hipError_t queue_bin(const unsigned long long* records, size_t per_part, uint32_
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&qscatter_kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
-    qscatter_kernel<<<nparts, kThreads, lds, s>>>(records, per_part, ntiles, nparts, counts, tile_start, sorted);
+    // sorted records, colour and pixel-in-tile apart (the tile is the work
+    // item's): 6 of the 8 bytes per record, in the same allocation
+    const size_t n = per_part * nparts;
+    uint32_t* sorted_col = reinterpret_cast<uint32_t*>(sorted);
+    uint16_t* sorted_loc = reinterpret_cast<uint16_t*>(sorted_col + n);
+    qscatter_kernel<<<nparts, kThreads, lds, s>>>(records, per_part, ntiles, nparts, counts, tile_start, sorted_col,
+                                                  sorted_loc);
     if (hist64)
-        qbin_kernel<<<max_segments, kThreads, 0, s>>>(sorted, reinterpret_cast<const uint4*>(segments), nseg, width,
-                                                      height, tiles_x, reinterpret_cast<u64*>(hist), false);
+        qbin_kernel<<<max_segments, kThreads, 0, s>>>(sorted_col, sorted_loc, reinterpret_cast<const uint4*>(segments),
+                                                      nseg, width, height, tiles_x, reinterpret_cast<u64*>(hist),
+                                                      false);
     else
-        qbin_kernel<<<max_segments, kThreads, 0, s>>>(sorted, reinterpret_cast<const uint4*>(segments), nseg, width,
-                                                      height, tiles_x, hist, packed64);
+        qbin_kernel<<<max_segments, kThreads, 0, s>>>(sorted_col, sorted_loc, reinterpret_cast<const uint4*>(segments),
+                                                      nseg, width, height, tiles_x, hist, packed64);
     return hipGetLastError();
 }
 

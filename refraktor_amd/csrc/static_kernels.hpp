@@ -45,7 +45,8 @@ hipError_t tonemap(const void* in, int in_kind, size_t npix, void* out, int form
 // (partition p = the records of iterate block p, 64x64-pixel tile layout,
 // see flame_kernels.h) into the histogram. counts: [ntiles][nparts] records
 // per tile and partition, as the iterate kernel wrote them (overwritten).
-// Scratch: sorted (all records), tile_scratch (2 * ntiles u32), segments
+// Scratch: sorted (8 bytes per record; holds the sorted colours and pixels),
+// tile_scratch (2 * ntiles u32), segments
 // (max_segments * 16 B), nseg (one u32). packed64: 32-bit bins updated as
 // two 64-bit pairs.
 uint32_t queue_tiles_x(uint32_t width);
