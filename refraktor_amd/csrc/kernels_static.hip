@@ -155,17 +155,23 @@ __global__ void density_histogram_kernel(const BinT* __restrict__ hist, size_t n
     if (threadIdx.x == 0) s_max = 0;
     __syncthreads();
     const float log_div = 4.0f / logf(2.0f);
+    // bin 0 (the empty and faint pixels, most of an image) and the maximum
+    // are kept per thread and added once: one LDS address for every lane
+    // would serialise the atomics
+    unsigned long long low = 0, mx = 0;
     for (size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x; g < npix; g += size_t(gridDim.x) * blockDim.x) {
         const float value = bin_units(hist[g]).w;
         if (value >= 1.0f) {
             unsigned long long b = (unsigned long long)(logf(value) * log_div) + 1ull;
             if (b >= kBins) b = kBins - 1;
             atomicAdd(&s_bins[b], 1ull);
-            atomicMax(&s_max, (unsigned long long)value);
+            mx = max(mx, (unsigned long long)value);
         } else {
-            atomicAdd(&s_bins[0], 1ull);
+            low++;
         }
     }
+    if (low) atomicAdd(&s_bins[0], low);
+    if (mx) atomicMax(&s_max, mx);
     __syncthreads();
     for (unsigned i = threadIdx.x; i < kBins; i += blockDim.x)
         if (s_bins[i]) atomicAdd(&bins97[i], s_bins[i]);
