@@ -932,26 +932,26 @@ hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
     if ((c1 % 8) || (c2 % 8) || c1 <= 0 || c2 < 0 || Cout <= 0) return hipErrorInvalidValue;
     if (x1_mode < 0 || x1_mode > 2 || (x1_mode == 1 && ((H | W) & 1))) return hipErrorInvalidValue;
     const int Cin = c1 + c2;
-    // inputs of 96 channels and more: the weight-stationary strip (measured
-    // faster there: 96->32 at 1080p 209 -> 186 us, 160->64 at 540x960
-    // 192 -> 149 us; narrower inputs pay more for reloading weights per strip
-    // than they save in LDS reads)
     // 32-bit element offsets inside one image (the strip kernels' indexing)
     const int H1 = x1_mode == 2 ? 2 * H : H, W1 = x1_mode == 2 ? 2 * W : W;
     const bool offsets32 = size_t(H1) * W1 * c1 < (size_t(1) << 31) && size_t(H) * W * std::max(c2, Cout) < (size_t(1) << 31) &&
                            size_t(Cout) * 9 * Cin < (size_t(1) << 31);
-    if (Cin >= 96 && Cout <= 96 && offsets32) {
-        launch_wstat<4>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
-        return hipGetLastError();
-    }
-    // the strip kernel wherever its weights leave room for two workgroups per
-    // CU and the output has at most three 32-channel tiles (measured: the
-    // 128-channel and the widest-input layers run faster on the tiled kernel)
+    // the input-stationary strip kernel wherever its weights leave room for
+    // two workgroups per CU (inputs up to 96 channels) and the output has at
+    // most three 32-channel tiles: measured fastest there (96->32 at 1080p
+    // 166 us against 192 us weight-stationary, 96->96 at 270x480 45 against
+    // 50 us)
     const int KC = Cin <= 16 ? 16 : 32;
     const size_t lds = strip_lds_bytes(Cin, KC);
     if (Cout <= 96 && lds <= 80 * 1024 && offsets32) {
         if (KC == 16) return launch_strip<16>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s);
         return launch_strip<32>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s);
+    }
+    // wider inputs: the weight-stationary strip (160->64 at 540x960 149 us
+    // against 192 us tiled); the 128-channel outputs run on the tiled kernel
+    if (Cin >= 96 && Cout <= 96 && offsets32) {
+        launch_wstat<4>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
+        return hipGetLastError();
     }
     if (Cout <= 32) {
         if (Cin <= 16)
@@ -973,7 +973,7 @@ hipError_t conv3x3_forward_rgb(const void* x, int cin, int N, int H, int W, cons
     if ((cin % 8) || cin <= 0 || Cout < 3 || Cout > 32 || !base || !out_rgb) return hipErrorInvalidValue;
     const int KC = cin <= 16 ? 16 : 32;
     const size_t lds = strip_lds_bytes(cin, KC);
-    if (cin >= 96 || lds > 80 * 1024 || size_t(H) * W * std::max(cin, 3) >= (size_t(1) << 31))
+    if (lds > 80 * 1024 || size_t(H) * W * std::max(cin, 3) >= (size_t(1) << 31))
         return hipErrorInvalidValue;
     if (KC == 16)
         return launch_strip<16>(x, cin, 0, nullptr, 0, N, H, W, wgt, bias, Cout, act, 0.f, nullptr, nullptr, lds, s, base,

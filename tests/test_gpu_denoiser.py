@@ -62,7 +62,7 @@ def test_conv_forward_matches_oracle(gpu, case):
                                            (1, 64, 96, 64), (1, 32, 64, 32), (2, 96, 96, 8), (1, 96, 128, 96)])
 def test_conv_fused_upsample_and_pool(gpu, mode, Co, c1, c2):
     """x1 read through a nearest 2x upsample (1) or a 2x2 average pool (2);
-    the wide inputs take the weight-stationary kernel"""
+    inputs above 96 channels take the weight-stationary kernel"""
     rng = np.random.default_rng(40 + mode + Co + c1)
     N, H, W = 1, 10, 136
     if mode == 1:
