@@ -162,8 +162,14 @@ int rfk_autotune(const rfk_db* db, const rfk_flame* f, int precision, const doub
  * samples (>= 4 packed samples, Catmull-Rom segments between the middle ones)
  * in ONE persistent launch: every frame has its own histogram and iteration
  * budget, work items (frame, block) are pulled longest-frame-first from a
- * device queue. rfk_batch_run can capture the whole sequence in a hipGraph
- * on its first call and replay it afterwards.                               */
+ * device queue. With a kernel compiled with bin_queue the batch is instead a
+ * sequence of chunk launches: each runs the next slice of passes of up to
+ * 2048 / (walker blocks per frame) frames and sorts and sums the chunk's
+ * records into those frames' histograms (the same histograms, bit for bit).
+ * rfk_batch_run can capture the whole sequence in a hipGraph on its first
+ * call and replay it afterwards. rfk_batch_info: grid = workgroups of the
+ * (largest) launch, items = (frame, block) items, or (slice, block) items
+ * with bin_queue.                                                           */
 typedef struct rfk_batch rfk_batch;
 typedef struct rfk_batch_frame_info {
     uint32_t* histogram;      /* device, height*width*4 bins of hist_bits     */

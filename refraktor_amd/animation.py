@@ -43,10 +43,10 @@ class Animation:
 
     def __init__(self, db: render.VariationDB, keyframes: list, width: int, height: int, frames: int, iters,
                  walkers: int = 1 << 18, seed: int = 1, fuse: int = 20, threads_per_block: int = 256,
-                 atomic_width: int = 32, bin_tile_log2: int = 0):
+                 atomic_width: int = 32, bin_tile_log2: int = 0, bin_queue: bool = False):
         self.aligned, self.samples = control_samples(db, keyframes, width, height)
         self.kernel = db.compile(self.aligned[0], threads_per_block=threads_per_block, atomic_width=atomic_width,
-                                 bin_tile_log2=bin_tile_log2)
+                                 bin_tile_log2=bin_tile_log2, bin_queue=bin_queue)
         self.iters = np.ascontiguousarray(np.broadcast_to(np.asarray(iters, dtype=np.uint64), (frames,)))
         self.batch = self.kernel.batch(self.samples, frames, self.iters, width, height, walkers, seed, fuse)
 
@@ -57,8 +57,9 @@ class Animation:
 
 def main(argv=None) -> int:
     """The animation workload: 4 keyframes of 6-10 xforms, 120 frames at
-    1280x720, budgets log-uniform in [2^26, 2^29], one persistent launch
-    replayed from a hipGraph. Prints one JSON line."""
+    1280x720, budgets log-uniform in [2^26, 2^29], queue-binned chunk
+    launches (or one persistent launch with --binning atomic) replayed from a
+    hipGraph. Prints one JSON line."""
     import argparse
     import json
     import time
@@ -69,28 +70,37 @@ def main(argv=None) -> int:
     ap.add_argument("--keyframes", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--walkers", type=int, default=1 << 18, help="walkers per frame")
-    ap.add_argument("--tpb", type=int, default=1024)
-    ap.add_argument("--tile", type=int, default=12, help="LDS bin cache slots (log2), 0 = off")
+    ap.add_argument("--binning", choices=("queue", "atomic"), default="queue")
+    ap.add_argument("--tpb", type=int, default=None, help="walkers per workgroup (default 512 queue, 1024 atomic)")
+    ap.add_argument("--tile", type=int, default=None,
+                    help="LDS bin cache slots (log2) of atomic binning, 0 = off (default 12)")
     ap.add_argument("--atomic-width", type=int, default=64)
     ap.add_argument("--replays", type=int, default=3)
     args = ap.parse_args(argv)
+    queue = args.binning == "queue"
+    if args.tpb is None:
+        args.tpb = 512 if queue else 1024
+    if args.tile is None:
+        args.tile = 0 if queue else 12
     db = render.VariationDB()
     budgets = frame_budgets(args.frames, args.seed)
     t0 = time.perf_counter()
     anim = Animation(db, keyframe_flames(args.keyframes, args.seed), args.width, args.height, args.frames, budgets,
                      walkers=args.walkers, seed=args.seed, threads_per_block=args.tpb,
-                     atomic_width=args.atomic_width, bin_tile_log2=args.tile)
+                     atomic_width=args.atomic_width, bin_tile_log2=args.tile, bin_queue=queue)
     setup_s = time.perf_counter() - t0
     eager_ms = anim.render(graph=False)
     graph_ms = [anim.render(graph=True) for _ in range(args.replays)]
     passes = sum(anim.batch.frame(f)["total_passes"] for f in range(args.frames))
     best = min(graph_ms)
     print(json.dumps({"workload": f"{args.keyframes} keyframes, {args.frames} frames at {args.width}x{args.height}, "
-                                  "budgets log-uniform in [2^26, 2^29], one persistent launch",
+                                  "budgets log-uniform in [2^26, 2^29], " +
+                                  ("queue-binned chunks" if queue else "one persistent launch") + ", hipGraph replay",
                       "iterations": int(budgets.sum()), "passes": int(passes), "setup_s": setup_s,
                       "eager_ms": eager_ms, "graph_ms": graph_ms, "frames_per_s": args.frames / (best * 1e-3),
                       "iterations_per_s": passes / (best * 1e-3),
-                      "config": {"walkers": args.walkers, "threads_per_block": args.tpb, "bin_tile_log2": args.tile,
+                      "config": {"binning": args.binning, "walkers": args.walkers, "threads_per_block": args.tpb,
+                                 "bin_tile_log2": args.tile, "grid": anim.batch.grid, "items": anim.batch.items,
                                  "atomic_width": args.atomic_width}}), flush=True)
     return 0
 

@@ -111,6 +111,17 @@ struct rfk_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
     bool hist64 = false;
+    // queue binning: launches of chunks, a chunk = up to slots_per_chunk slots,
+    // a slot = one frame's next slice of at most slice_passes passes
+    struct chunk {
+        uint32_t first, nslots, passes;
+    };
+    bool queue = false;
+    std::vector<chunk> chunks;
+    uint32_t slots_per_chunk = 0, slice_passes = 0, tiles_per_frame = 0, max_segments = 0;
+    uint32_t *d_slots = nullptr, *d_slot_frame = nullptr, *qcounts = nullptr, *qtiles = nullptr, *qnseg = nullptr;
+    float *wx = nullptr, *wy = nullptr, *wc = nullptr;
+    void *records = nullptr, *sorted = nullptr, *qsegments = nullptr;
     size_t frame_words() const { return size_t(width) * height * (hist64 ? 8u : 4u); }
 };
 
@@ -434,7 +445,7 @@ int rfk_compile(const rfk_db* db, const rfk_flame* f, int precision, const rfk_e
     RFK_HIP(hipModuleLoadData(&k->mod, cr.code.data()));
     RFK_HIP(hipModuleGetFunction(&k->f_iterate, k->mod, "rfk_iterate"));
     RFK_HIP(hipModuleGetFunction(&k->f_precalc, k->mod, "rfk_precalc"));
-    RFK_HIP(hipModuleGetFunction(&k->f_batch, k->mod, "rfk_iterate_batch"));
+    RFK_HIP(hipModuleGetFunction(&k->f_batch, k->mod, k->opt.bin_queue ? "rfk_iterate_qbatch" : "rfk_iterate_batch"));
     *out = k.release();
     return RFK_OK;
 }
@@ -858,7 +869,8 @@ static void free_batch(rfk_batch* b) {
     auto f = [](void* p) { if (p) (void)hipFree(p); };
     f(b->params); f(b->palettes); f(b->hist); f(b->d_item_frame); f(b->d_item_block); f(b->d_frame_row0);
     f(b->d_frame_blocks); f(b->d_frame_passes); f(b->d_queue); f(b->d_frame_counters); f(b->d_row_stream);
-    f(b->d_row_local);
+    f(b->d_row_local); f(b->d_slots); f(b->d_slot_frame); f(b->qcounts); f(b->qtiles); f(b->qnseg); f(b->wx);
+    f(b->wy); f(b->wc); f(b->records); f(b->sorted); f(b->qsegments);
     if (b->h_frame_counters) (void)hipHostFree(b->h_frame_counters);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -867,12 +879,77 @@ static void free_batch(rfk_batch* b) {
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
 }
 
+// Queue-binned batch: frame f's binned passes are cut into slices of
+// slice_passes; a chunk launch runs up to slots_per_chunk slices of different
+// frames (every walker block of each) and then sorts and sums the chunk's
+// records into the frames' histograms. Chunks take the frames with the most
+// slices left, so the longest frames set the chunk count only when they
+// must. Sizes: enough walker blocks per launch to fill the device (2048
+// workgroups) and about 2^27 records (1 GiB) per chunk.
+static int plan_queue_batch(rfk_batch* B) {
+    const uint32_t bpf = B->blocks_per_frame, frames = B->frames;
+    const uint64_t wf = uint64_t(bpf) * B->tpb;
+    const uint32_t S = std::min<uint32_t>(frames, std::max<uint32_t>(1u, (2048u + bpf - 1) / bpf));
+    const uint64_t P = std::max<uint64_t>(1, (1ull << 27) / (uint64_t(S) * wf));
+    const uint64_t records = uint64_t(S) * wf * P;
+    if (records >= (1ull << 32)) return fail(RFK_ERR_INVALID, "too many walkers per frame for queue binning");
+    B->queue = true;
+    B->slots_per_chunk = S;
+    B->slice_passes = uint32_t(P);
+    B->tiles_per_frame = rfk::sk::queue_tiles(B->width, B->height);
+    std::vector<uint32_t> left(frames), next(frames, 0), slots, slot_frame, pick;
+    uint64_t remaining = 0;
+    for (uint32_t f = 0; f < frames; f++) {
+        left[f] = uint32_t((B->frame_passes[f] + P - 1) / P);
+        remaining += left[f];
+    }
+    while (remaining) {
+        pick.clear();
+        for (uint32_t f = 0; f < frames; f++)
+            if (left[f]) pick.push_back(f);
+        std::stable_sort(pick.begin(), pick.end(), [&](uint32_t x, uint32_t y) { return left[x] > left[y]; });
+        if (pick.size() > S) pick.resize(S);
+        rfk_batch::chunk c{uint32_t(slot_frame.size()), uint32_t(pick.size()), 0u};
+        for (uint32_t f : pick) {
+            const uint32_t sl = next[f]++;
+            const uint64_t first = uint64_t(sl) * P;
+            const uint32_t n = uint32_t(std::min<uint64_t>(P, B->frame_passes[f] - first));
+            // passes are numbered as in one render: fuse passes, then the binned ones
+            slots.insert(slots.end(), {f, sl ? uint32_t(B->fuse + first) : 0u, n, sl ? 0u : B->fuse});
+            slot_frame.push_back(f);
+            c.passes = std::max(c.passes, n);
+            left[f]--;
+            remaining--;
+        }
+        B->chunks.push_back(c);
+    }
+    const uint32_t ctiles = S * B->tiles_per_frame;
+    B->max_segments = rfk::sk::queue_max_segments(records, ctiles);
+    B->nitems = uint32_t(slot_frame.size()) * bpf;
+    B->grid = S * bpf;
+    RFK_HIP(dalloc(&B->d_slots, slots.size()));
+    RFK_HIP(dalloc(&B->d_slot_frame, slot_frame.size()));
+    RFK_HIP(dalloc(&B->wx, size_t(frames) * wf));
+    RFK_HIP(dalloc(&B->wy, size_t(frames) * wf));
+    RFK_HIP(dalloc(&B->wc, size_t(frames) * wf));
+    RFK_HIP(hipMalloc(&B->records, records * 8));
+    RFK_HIP(hipMalloc(&B->sorted, records * 8));
+    RFK_HIP(dalloc(&B->qcounts, size_t(ctiles) * bpf));
+    RFK_HIP(dalloc(&B->qtiles, size_t(ctiles) * 2));
+    RFK_HIP(hipMalloc(&B->qsegments, size_t(B->max_segments) * 16));
+    RFK_HIP(dalloc(&B->qnseg, 1));
+    RFK_HIP(hipMemcpyAsync(B->d_slots, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, B->stream));
+    RFK_HIP(hipMemcpyAsync(B->d_slot_frame, slot_frame.data(), slot_frame.size() * 4, hipMemcpyHostToDevice,
+                           B->stream));
+    RFK_HIP(hipStreamSynchronize(B->stream));
+    return RFK_OK;
+}
+
 int rfk_batch_create(rfk_kernel* k, void* stream, const double* samples, size_t n_reals, uint32_t frames,
                      const uint64_t* iters, uint32_t width, uint32_t height, uint32_t walkers_per_frame, uint32_t seed,
                      uint32_t fuse, rfk_batch** out) {
     if (!k || !samples || !iters || !out || frames == 0) return fail(RFK_ERR_INVALID, "bad argument");
     if (k->L.chaos_base >= 0) return fail(RFK_ERR_INVALID, "batches need a kernel without xaos");
-    if (k->opt.bin_queue) return fail(RFK_ERR_INVALID, "batches bin with atomics: compile without bin_queue");
     if (width == 0 || height == 0 || width > 32768 || height > 32768) return fail(RFK_ERR_INVALID, "bad image size");
     const size_t sample_size = size_t(k->L.size_reals) + 768;
     if (n_reals % sample_size != 0 || n_reals / sample_size < 4)
@@ -1017,6 +1094,9 @@ int rfk_batch_create(rfk_kernel* k, void* stream, const double* samples, size_t 
         per_cu <= 0)
         per_cu = 1;
     B->grid = std::min<uint32_t>(B->nitems, uint32_t(cus) * uint32_t(per_cu));
+    if (k->opt.bin_queue) {
+        if (int rc = plan_queue_batch(B)) return cleanup(rc);
+    }
 #undef RFK_HIPB
     *out = b.release();
     return RFK_OK;
@@ -1059,6 +1139,36 @@ static int batch_enqueue(rfk_batch* B) {
     ba.nitems = B->nitems;
     ba.fuse = B->fuse;
     ba.frame_pixels = uint32_t(fpix);
+    if (B->queue) {
+        a.wx = B->wx;
+        a.wy = B->wy;
+        a.wc = B->wc;
+        a.queue = static_cast<u64*>(B->records);
+        a.qcounts = B->qcounts;
+        a.tiles_x = rfk::sk::queue_tiles_x(B->width);
+        a.ntiles = B->tiles_per_frame;
+        rfk_qbatch_args q;
+        std::memset(&q, 0, sizeof q);
+        q.slots = B->d_slots;
+        q.frame_counters = reinterpret_cast<u64*>(B->d_frame_counters);
+        q.cols = B->blocks_per_frame;
+        q.frame_pixels = uint32_t(fpix);
+        void* qargv[] = {&a, &q};
+        for (const auto& c : B->chunks) {
+            q.slot0 = c.first;
+            q.passes = c.passes;
+            const uint32_t parts = c.nslots * B->blocks_per_frame;
+            RFK_HIP(hipModuleLaunchKernel(k->f_batch, parts, 1, 1, B->tpb, 1, 1, a.ntiles * 4u, B->stream, qargv,
+                                          nullptr));
+            RFK_HIP(rfk::sk::queue_bin_groups(static_cast<const unsigned long long*>(B->records),
+                                              size_t(c.passes) * B->tpb, parts, B->blocks_per_frame,
+                                              static_cast<unsigned long long*>(B->sorted), B->qcounts, B->qtiles,
+                                              B->qsegments, B->max_segments, B->qnseg, B->width, B->height,
+                                              B->d_slot_frame + c.first, fpix * 4, B->hist, B->hist64,
+                                              k->opt.atomic_width == 64, B->stream));
+        }
+        return RFK_OK;
+    }
     void* argv[] = {&a, &ba};
     RFK_HIP(hipModuleLaunchKernel(k->f_batch, B->grid, 1, 1, B->tpb, 1, 1, 0, B->stream, argv, nullptr));
     return RFK_OK;

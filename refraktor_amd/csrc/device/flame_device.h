@@ -371,6 +371,16 @@ struct rfk_batch_args {
     u32 pad;
 };
 
+// Queue-binned batch chunk: slots [slot0, slot0 + grid / cols) of `slots`
+struct rfk_qbatch_args {
+    const u32* slots;           // [slots][4] frame, first pass, binned passes, fuse passes
+    u64* frame_counters;        // [frames][4] hits, draws, passes, bad
+    u32 slot0;                  // first slot of this chunk
+    u32 cols;                   // walker blocks per frame
+    u32 passes;                 // record passes per block in this chunk
+    u32 frame_pixels;           // width * height
+};
+
 #define RFK_FLAG_DRAW 1u
 #define RFK_FLAG_INIT 2u
 #define RFK_FLAG_TRACE 4u

@@ -67,6 +67,27 @@ RFK_INLINE void rfk_queue_count(u32 tile, bool valid) {
 }
 #endif
 
+// Where a block's queue records and tile counts go (indices into a.queue and
+// a.qcounts): rec(j) = first record of its j-th binned pass (RFK_TPB lanes),
+// passes() = record passes it owns (those it does not run are left empty),
+// cnt(k) = its count of tile k. One type per launch layout, so each kernel
+// computes its indices from values it keeps anyway.
+// rfk_iterate: part = block, npasses passes each, counts [tile][grid].
+struct rfk_qdst_grid {
+    u32 block, npasses;
+    RFK_INLINE size_t rec(u32 j) const { return ((size_t)block * npasses + j) * RFK_TPB; }
+    RFK_INLINE size_t cnt(u32 k) const { return (size_t)k * gridDim.x + block; }
+    RFK_INLINE u32 passes() const { return npasses; }
+};
+// rfk_iterate_qbatch: part = workgroup, q_passes passes each, counts of
+// slot g at [g * ntiles + tile][cols], column = the walker block.
+struct rfk_qdst_chunk {
+    u32 part, q_passes, cnt0, cols;
+    RFK_INLINE size_t rec(u32 j) const { return ((size_t)part * q_passes + j) * RFK_TPB; }
+    RFK_INLINE size_t cnt(u32 k) const { return cnt0 + (size_t)k * cols; }
+    RFK_INLINE u32 passes() const { return q_passes; }
+};
+
 // bin element: u32 fixed point (wraps past 2^32 / 255 full-opacity draws in
 // one pixel) or u64 for budgets that can put more than that into a pixel
 #if RFK_HIST64
@@ -163,10 +184,12 @@ RFK_INLINE void rfk_tile_draw(rfk_shared& sm, hist_t* __restrict__ hist, const r
 //   init:     seed walkers (Hammersley over `total` walkers) instead of loading
 //   wx..wl:   walker state arrays (null: the block keeps its walkers only
 //             for this call)
+//   q:        queue binning: the block's records and tile counts
 // Returns the passes done (bailouts may stop early); counters land in sm.
+template <class QDst>
 RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, param_ptr P, const u32* pal,
                          u32 block, u32 key1, hist_t* hist, bool init, u32 total, u32 pass0, u32 nfuse, u32 npasses,
-                         u32 flags, float* wx, float* wy, float* wc, u32* wl) {
+                         u32 flags, float* wx, float* wy, float* wc, u32* wl, const QDst& q) {
     const u32 t = threadIdx.x;
     const u32 gid = block * RFK_TPB + t;
     const u32 wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -306,7 +329,7 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, param_ptr P, co
                 draws++;
             }
 #if RFK_BIN_QUEUE
-            a.queue[((size_t)block * npasses + (i - nfuse)) * RFK_TPB + t] = rec;
+            a.queue[q.rec(i - nfuse) + t] = rec;
             rfk_queue_count(u32(rec >> 44), rec != RFK_QEMPTY);
 #endif
         }
@@ -400,10 +423,9 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, param_ptr P, co
 #if RFK_BIN_QUEUE
     // passes a bailout skipped leave no draws behind
     if (draw) {
-        for (u32 j = (done > nfuse ? done : nfuse); j < ntotal; j++)
-            a.queue[((size_t)block * npasses + (j - nfuse)) * RFK_TPB + t] = RFK_QEMPTY;
+        for (u32 j = (done > nfuse ? done - nfuse : 0u); j < q.passes(); j++) a.queue[q.rec(j) + t] = RFK_QEMPTY;
         __syncthreads();
-        for (u32 k = t; k < a.ntiles; k += RFK_TPB) a.qcounts[(size_t)k * gridDim.x + block] = rfk_qcount[k];
+        for (u32 k = t; k < a.ntiles; k += RFK_TPB) a.qcounts[q.cnt(k)] = rfk_qcount[k];
     }
 #endif
 
@@ -446,8 +468,9 @@ extern "C" __global__ __launch_bounds__(RFK_TPB, RFK_WAVES_PER_SIMD) void rfk_it
     const param_ptr P = (param_ptr)((const Real*)a.params + (size_t)block * a.param_stride);
     const bool init = (a.flags & RFK_FLAG_INIT) != 0u;
     const u32 pass0 = init ? 0u : a.block_pass[block];
+    const rfk_qdst_grid q = {block, a.npasses};
     const u32 done = rfk_block(a, sm, P, a.palettes + (size_t)block * 256u, block, a.stream, (hist_t*)a.hist, init,
-                               a.total_walkers, pass0, 0u, a.npasses, a.flags, a.wx, a.wy, a.wc, a.wlast);
+                               a.total_walkers, pass0, 0u, a.npasses, a.flags, a.wx, a.wy, a.wc, a.wlast, q);
     if (threadIdx.x == 0) {
         a.block_pass[block] = pass0 + done;
         atomicAdd(&a.counters[RFK_CNT_HITS], sm.hits - sm.published);
@@ -460,12 +483,14 @@ extern "C" __global__ __launch_bounds__(RFK_TPB, RFK_WAVES_PER_SIMD) void rfk_it
     }
 }
 
+#if !RFK_BIN_QUEUE
 // Persistent animation kernel: work item i (popped from a.queue in the
 // host-chosen order, longest first) is block b of frame f. Each item seeds its
 // walkers, runs the frame's fuse passes without drawing and then the frame's
 // binned passes into the frame's histogram; streams are keyed by (seed, f).
 extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate_batch(rfk_iter_args a, rfk_batch_args b) {
     __shared__ rfk_shared sm;
+    const rfk_qdst_grid q = {0u, 0u};  // unused: atomic binning
     for (;;) {
         if (threadIdx.x == 0) sm.item = atomicAdd(b.queue, 1u);
         __syncthreads();
@@ -479,7 +504,7 @@ extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate_batch(rfk_iter
         hist_t* hist = (hist_t*)a.hist + (size_t)f * b.frame_pixels * 4u;
         const u32 total = b.frame_blocks[f] * RFK_TPB;
         const u32 done = rfk_block(a, sm, P, pal, blk, f, hist, true, total, 0u, b.fuse, b.frame_passes[f],
-                                   RFK_FLAG_DRAW, nullptr, nullptr, nullptr, nullptr) - b.fuse;
+                                   RFK_FLAG_DRAW, nullptr, nullptr, nullptr, nullptr, q) - b.fuse;
         if (threadIdx.x == 0) {
             u64* fc = b.frame_counters + (size_t)f * 4u;
             atomicAdd(&fc[0], sm.hits);
@@ -490,3 +515,36 @@ extern "C" __global__ __launch_bounds__(RFK_TPB) void rfk_iterate_batch(rfk_iter
         __syncthreads();
     }
 }
+#else
+// Animation batch with queue binning, one launch per chunk. Block b runs
+// slot q.slot0 + b / q.cols (a frame and its next slice of passes: slots
+// hold frame, first pass, binned passes, fuse passes) on walker block
+// b % q.cols of that frame, whose walkers persist between slices in
+// a.wx/wy/wc at [frame][block][lane]. Its records go to part b of the chunk
+// (q.passes passes each) and its tile counts to the rows of slot b / q.cols:
+// chunk tiles [(b / q.cols) * ntiles, ...). Streams are keyed by (seed,
+// frame) and passes are numbered as in one uninterrupted render, so every
+// frame equals the atomic batch's.
+extern "C" __global__ __launch_bounds__(RFK_TPB, RFK_WAVES_PER_SIMD) void rfk_iterate_qbatch(rfk_iter_args a,
+                                                                                              rfk_qbatch_args q) {
+    __shared__ rfk_shared sm;
+    const u32 g = blockIdx.x / q.cols, blk = blockIdx.x - g * q.cols;
+    const u32* sl = q.slots + (size_t)(q.slot0 + g) * 4u;
+    const u32 f = sl[0], pass0 = sl[1], npasses = sl[2], nfuse = sl[3];
+    const u32 row = f * q.cols + blk;
+    const param_ptr P = (param_ptr)((const Real*)a.params + (size_t)row * a.param_stride);
+    const u32* pal = a.palettes + (size_t)row * 256u;
+    hist_t* hist = (hist_t*)a.hist + (size_t)f * q.frame_pixels * 4u;
+    const size_t w0 = (size_t)f * q.cols * RFK_TPB;
+    const rfk_qdst_chunk qd = {blockIdx.x, q.passes, g * a.ntiles * q.cols + blk, q.cols};
+    const u32 done = rfk_block(a, sm, P, pal, blk, f, hist, pass0 == 0u, q.cols * RFK_TPB, pass0, nfuse, npasses,
+                               RFK_FLAG_DRAW, a.wx + w0, a.wy + w0, a.wc + w0, nullptr, qd) - nfuse;
+    if (threadIdx.x == 0) {
+        u64* fc = q.frame_counters + (size_t)f * 4u;
+        atomicAdd(&fc[0], sm.hits);
+        atomicAdd(&fc[1], sm.draws);
+        atomicAdd(&fc[2], (u64)done * RFK_TPB);
+        atomicAdd(&fc[3], sm.bad);
+    }
+}
+#endif

@@ -16,6 +16,11 @@
 //                 plain read-add-write when the work item is the tile's only
 //                 one, one atomic per channel otherwise
 //
+// Several frames in one chunk (animation batches): the parts come in groups
+// of `ncols`, group g holding only records of one frame (group_frame[g]),
+// whose tiles are counted as chunk tiles [g * tpf, (g + 1) * tpf); the
+// counts are [chunk tile][ncols]. A single frame is one group of all parts.
+//
 // All integer sums: the histogram equals the one direct atomics give, bit
 // for bit, whatever the order.
 #include <hip/hip_runtime.h>
@@ -82,14 +87,14 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     return base + incl - v;
 }
 
-// grid = ntiles: column t of counts [tile][nparts] -> exclusive prefix in
+// grid = ntiles: row t of counts [tile][ncols] -> exclusive prefix in
 // place, tile_total[t] = its sum.
 __global__ void __launch_bounds__(kThreads)
-    qscan_col_kernel(uint32_t* __restrict__ counts, uint32_t nparts, uint32_t* __restrict__ tile_total) {
+    qscan_col_kernel(uint32_t* __restrict__ counts, uint32_t ncols, uint32_t* __restrict__ tile_total) {
     __shared__ uint32_t s_wave[kThreads / 64];
-    uint32_t* col = counts + size_t(blockIdx.x) * nparts;
-    const uint32_t per = (nparts + kThreads - 1) / kThreads;
-    const uint32_t lo = threadIdx.x * per, hi = std::min(nparts, lo + per);
+    uint32_t* col = counts + size_t(blockIdx.x) * ncols;
+    const uint32_t per = (ncols + kThreads - 1) / kThreads;
+    const uint32_t lo = threadIdx.x * per, hi = std::min(ncols, lo + per);
     uint32_t sum = 0;
     for (uint32_t i = lo; i < hi; i++) sum += col[i];
     uint32_t total = 0;
@@ -137,10 +142,12 @@ __global__ void __launch_bounds__(kThreads)
     if (threadIdx.x == 0) nseg[0] = std::min(full_all + rem_all, max_segments);
 }
 
-// grid = nparts: partition p = records [p * per_part, (p + 1) * per_part).
+// grid = nparts: partition p = records [p * per_part, (p + 1) * per_part),
+// column p % ncols of the counts of group p / ncols, whose ntiles tiles (the
+// records' tile fields) are chunk tiles from (p / ncols) * ntiles on.
 // LDS: cursor[ntiles], lcount[ntiles], lstart[ntiles], piece[kPiece].
 __global__ void __launch_bounds__(kThreads)
-    qscatter_kernel(const u64* __restrict__ rec, size_t per_part, uint32_t ntiles, uint32_t nparts,
+    qscatter_kernel(const u64* __restrict__ rec, size_t per_part, uint32_t ntiles, uint32_t ncols,
                     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ tile_start,
                     uint32_t* __restrict__ sorted_col, uint16_t* __restrict__ sorted_loc) {
     extern __shared__ uint32_t smem[];
@@ -149,8 +156,10 @@ __global__ void __launch_bounds__(kThreads)
     uint32_t* lstart = lcount + ntiles;
     u64* piece = reinterpret_cast<u64*>(lstart + ntiles + (ntiles & 1u));
     __shared__ uint32_t s_wave[kThreads / 64];
+    const uint32_t group = blockIdx.x / ncols, col = blockIdx.x - group * ncols;
+    const uint32_t tile0 = group * ntiles;
     for (uint32_t t = threadIdx.x; t < ntiles; t += kThreads)
-        cursor[t] = tile_start[t] + offsets[size_t(t) * nparts + blockIdx.x];
+        cursor[t] = tile_start[tile0 + t] + offsets[size_t(tile0 + t) * ncols + col];
     const size_t b0 = size_t(blockIdx.x) * per_part, b1 = b0 + per_part;
     constexpr uint32_t kPer = kPiece / kThreads;
     const uint32_t per = (ntiles + kThreads - 1) / kThreads;
@@ -236,12 +245,16 @@ __device__ __forceinline__ void add_bin(HistT* __restrict__ hist, size_t p, uint
 
 // LDS sums of a work item: (r | g << 32) and (b | w << 32) per pixel. A work
 // item holds at most 2^18 records of 8-bit channels, so each 32-bit half
-// stays below 2^26 and never carries into its neighbour.
+// stays below 2^26 and never carries into its neighbour. Chunk tile t is
+// tile t % tpf of frame group_frame[t / tpf] (frame 0 without a list), whose
+// bins start frame_elems elements after the previous frame's.
 template <class HistT>
 __global__ void __launch_bounds__(kThreads)
     qbin_kernel(const uint32_t* __restrict__ sorted_col, const uint16_t* __restrict__ sorted_loc,
                 const uint4* __restrict__ segments, const uint32_t* __restrict__ nseg,
-                uint32_t width, uint32_t height, uint32_t tiles_x, HistT* __restrict__ hist, bool packed64) {
+                uint32_t width, uint32_t height, uint32_t tiles_x, uint32_t tpf,
+                const uint32_t* __restrict__ group_frame, size_t frame_elems, HistT* __restrict__ hist,
+                bool packed64) {
     static_assert(kSegment <= (1u << 24), "32-bit halves of the LDS sums must not carry");
     __shared__ u64 acc[2][kTilePix];
     if (blockIdx.x >= nseg[0]) return;
@@ -271,7 +284,9 @@ __global__ void __launch_bounds__(kThreads)
         }
     }
     __syncthreads();
-    const uint32_t tx = sg.x % tiles_x, ty = sg.x / tiles_x;
+    const uint32_t group = sg.x / tpf, tile = sg.x - group * tpf;
+    if (group_frame) hist += size_t(group_frame[group]) * frame_elems;
+    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
     const bool exclusive = sg.w != 0u;
     for (uint32_t p = threadIdx.x; p < kTilePix; p += kThreads) {
         const u64 rg = acc[0][p], bw = acc[1][p];
@@ -290,18 +305,22 @@ uint32_t queue_max_segments(size_t records, uint32_t ntiles) {
     return uint32_t((records + kSegment - 1) / kSegment) + ntiles;
 }
 
-hipError_t queue_bin(const unsigned long long* records, size_t per_part, uint32_t nparts, unsigned long long* sorted,
-                     uint32_t* counts, uint32_t* tile_scratch, void* segments, uint32_t max_segments, uint32_t* nseg,
-                     uint32_t width, uint32_t height, uint32_t* hist, bool hist64, bool packed64, hipStream_t s) {
+hipError_t queue_bin_groups(const unsigned long long* records, size_t per_part, uint32_t nparts, uint32_t ncols,
+                            unsigned long long* sorted, uint32_t* counts, uint32_t* tile_scratch, void* segments,
+                            uint32_t max_segments, uint32_t* nseg, uint32_t width, uint32_t height,
+                            const uint32_t* group_frame, size_t frame_elems, uint32_t* hist, bool hist64,
+                            bool packed64, hipStream_t s) {
     if (!per_part || !nparts) return hipSuccess;
-    const uint32_t tiles_x = queue_tiles_x(width), ntiles = queue_tiles(width, height);
-    if (ntiles >= kNoTile) return hipErrorInvalidValue;
+    if (!ncols || nparts % ncols) return hipErrorInvalidValue;
+    const uint32_t tiles_x = queue_tiles_x(width), tpf = queue_tiles(width, height);
+    const uint64_t ntiles = uint64_t(tpf) * (nparts / ncols);  // chunk tiles
+    if (tpf >= kNoTile || ntiles >= (1ull << 31)) return hipErrorInvalidValue;
     uint32_t* tile_total = tile_scratch;
     uint32_t* tile_start = tile_scratch + ntiles;
-    qscan_col_kernel<<<ntiles, kThreads, 0, s>>>(counts, nparts, tile_total);
-    qseg_kernel<<<1, kThreads, 0, s>>>(tile_total, ntiles, tile_start, reinterpret_cast<uint4*>(segments),
+    qscan_col_kernel<<<uint32_t(ntiles), kThreads, 0, s>>>(counts, ncols, tile_total);
+    qseg_kernel<<<1, kThreads, 0, s>>>(tile_total, uint32_t(ntiles), tile_start, reinterpret_cast<uint4*>(segments),
                                        max_segments, nseg);
-    const size_t lds = size_t(3 * ntiles + 1) * sizeof(uint32_t) + kPiece * sizeof(u64) + 8;
+    const size_t lds = size_t(3 * tpf + 1) * sizeof(uint32_t) + kPiece * sizeof(u64) + 8;
     // more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per workgroup)
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&qscatter_kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
@@ -311,16 +330,24 @@ hipError_t queue_bin(const unsigned long long* records, size_t per_part, uint32_
     const size_t n = per_part * nparts;
     uint32_t* sorted_col = reinterpret_cast<uint32_t*>(sorted);
     uint16_t* sorted_loc = reinterpret_cast<uint16_t*>(sorted_col + n);
-    qscatter_kernel<<<nparts, kThreads, lds, s>>>(records, per_part, ntiles, nparts, counts, tile_start, sorted_col,
+    qscatter_kernel<<<nparts, kThreads, lds, s>>>(records, per_part, tpf, ncols, counts, tile_start, sorted_col,
                                                   sorted_loc);
     if (hist64)
         qbin_kernel<<<max_segments, kThreads, 0, s>>>(sorted_col, sorted_loc, reinterpret_cast<const uint4*>(segments),
-                                                      nseg, width, height, tiles_x, reinterpret_cast<u64*>(hist),
-                                                      false);
+                                                      nseg, width, height, tiles_x, tpf, group_frame, frame_elems,
+                                                      reinterpret_cast<u64*>(hist), false);
     else
         qbin_kernel<<<max_segments, kThreads, 0, s>>>(sorted_col, sorted_loc, reinterpret_cast<const uint4*>(segments),
-                                                      nseg, width, height, tiles_x, hist, packed64);
+                                                      nseg, width, height, tiles_x, tpf, group_frame, frame_elems,
+                                                      hist, packed64);
     return hipGetLastError();
+}
+
+hipError_t queue_bin(const unsigned long long* records, size_t per_part, uint32_t nparts, unsigned long long* sorted,
+                     uint32_t* counts, uint32_t* tile_scratch, void* segments, uint32_t max_segments, uint32_t* nseg,
+                     uint32_t width, uint32_t height, uint32_t* hist, bool hist64, bool packed64, hipStream_t s) {
+    return queue_bin_groups(records, per_part, nparts, nparts, sorted, counts, tile_scratch, segments, max_segments,
+                            nseg, width, height, nullptr, 0, hist, hist64, packed64, s);
 }
 
 }  // namespace rfk::sk

@@ -55,6 +55,16 @@ uint32_t queue_max_segments(size_t records, uint32_t ntiles);
 hipError_t queue_bin(const unsigned long long* records, size_t per_part, uint32_t nparts, unsigned long long* sorted,
                      uint32_t* counts, uint32_t* tile_scratch, void* segments, uint32_t max_segments, uint32_t* nseg,
                      uint32_t width, uint32_t height, uint32_t* hist, bool hist64, bool packed64, hipStream_t s);
+// The same for a chunk of several frames (animation batches): the parts come
+// in groups of ncols, group g holding records of frame group_frame[g] only
+// (tile fields inside that frame), counts [g * ntiles + tile][ncols];
+// frame f's bins start at hist + f * frame_elems elements. tile_scratch
+// holds 2 * ntiles * nparts / ncols u32.
+hipError_t queue_bin_groups(const unsigned long long* records, size_t per_part, uint32_t nparts, uint32_t ncols,
+                            unsigned long long* sorted, uint32_t* counts, uint32_t* tile_scratch, void* segments,
+                            uint32_t max_segments, uint32_t* nseg, uint32_t width, uint32_t height,
+                            const uint32_t* group_frame, size_t frame_elems, uint32_t* hist, bool hist64,
+                            bool packed64, hipStream_t s);
 
 // density_hdr (tonemap.cu:159-233) on a tonemapped float4 / half4 image,
 // with the 97-bin density histogram (device u64) of the same render.

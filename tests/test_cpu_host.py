@@ -211,6 +211,7 @@ def test_generated_kernels_compile_offline(db, maker):
     assert f.compile_check() > 0
     assert f.compile_check(threads_per_block=1024, atomic_width=64) > 0
     assert f.compile_check(precision=1) > 0
+    assert f.compile_check(threads_per_block=512, atomic_width=64, bin_queue=True) > 0
 
 
 def test_every_variation_compiles(db):

@@ -74,6 +74,34 @@ def test_batch_graph_replays_match_eager(gpu, db):
         st.bin(iters=1 << 16)
 
 
+def test_queue_batch_matches_atomic_batch(gpu, db):
+    """queue binning cuts each frame into slices of passes and runs several
+    frames' slices per chunk launch; integer sums make every frame equal to
+    the atomic batch's, bit for bit, eagerly and in graph replays"""
+    kf = _keyframes(4, 6)
+    n, walkers = 12, 1 << 16
+    iters = animation.frame_budgets(n, 5, 1 << 23, 1 << 26)
+    atomic = animation.Animation(db, kf, W, H, frames=n, iters=iters, walkers=walkers, seed=3)
+    queued = animation.Animation(db, kf, W, H, frames=n, iters=iters, walkers=walkers, seed=3, bin_queue=True)
+    bpf = walkers // queued.kernel.threads_per_block
+    # more frames than slots per chunk, and frames of several slices
+    assert queued.batch.grid < n * bpf
+    assert queued.batch.items > n * bpf
+    atomic.render(graph=False)
+    queued.render(graph=False)
+    eager = []
+    for f in range(n):
+        a, q = atomic.batch.frame(f), queued.batch.frame(f)
+        for key in ("total_passes", "total_draws", "bad_values", "passes_per_walker", "walkers"):
+            assert a[key] == q[key], (f, key)
+        eager.append(queued.batch.histogram(f))
+        np.testing.assert_array_equal(eager[f], atomic.batch.histogram(f), err_msg=f"frame {f}")
+    for rep in range(2):
+        queued.render(graph=True)
+        for f in range(n):
+            np.testing.assert_array_equal(queued.batch.histogram(f), eager[f], err_msg=f"replay {rep} frame {f}")
+
+
 def test_batch_rejects_bad_arguments(gpu, db):
     k = db.compile(fl.flame_keyframe(1))
     s = k.pack_samples(W, H)
