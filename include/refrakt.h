@@ -139,6 +139,23 @@ int rfk_rewarm(rfk_kernel* k, rfk_state* s, const double* samples, size_t n_real
                uint32_t fuse);
 int rfk_state_free(rfk_state* s);
 
+/* ---------------------------------------------------------------- jpeg --
+ * rfkt::jpeg_encoder::encode_image (interface/jpeg_encoder.h:11-24; the nvJPEG
+ * backend, impl/jpeg_encoder/nvjpeg.cpp:124-196, and the stb host backend,
+ * cpu_jpeg_encoder.cpp:14-41): an 8-bit RGB (or RGBA, alpha ignored) device
+ * image to a baseline JFIF file, 4:4:4 YCbCr, the JPEG standard's Annex K
+ * tables scaled by the IJG quality rule (1..100; above 100 = 100), a restart
+ * marker every 8 MCUs. rfk_jpeg_encode only enqueues on `stream` (like
+ * nvjpegEncodeImage); rfk_jpeg_retrieve waits for it and copies the file
+ * (out = NULL: *size only, like nvjpegEncodeRetrieveBitstream). One encode
+ * in flight per encoder: a second rfk_jpeg_encode waits for the first.     */
+typedef struct rfk_jpeg rfk_jpeg;
+int rfk_jpeg_create(rfk_jpeg** out);
+int rfk_jpeg_encode(rfk_jpeg* j, const void* image /* device */, int in_format /* RFK_PIXEL_RGB8 / RGBA8 */,
+                    size_t pitch_bytes, uint32_t width, uint32_t height, int quality, void* stream);
+int rfk_jpeg_retrieve(rfk_jpeg* j, uint8_t* out /* host */, size_t cap, size_t* size);
+int rfk_jpeg_free(rfk_jpeg* j);
+
 /* ----------------------------------------------------------- autotune --
  * No reference counterpart (librefrakt fixes its launch shape per build).
  * Compiles the flame once per candidate shape, renders probe_iters after a

@@ -14,10 +14,10 @@ LIB = HERE / "_build" / "liboracle.so"
 
 def build() -> Path:
     LIB.parent.mkdir(exist_ok=True)
-    src = HERE / "flame_oracle.c"
+    srcs = [HERE / "flame_oracle.c", HERE / "jpeg_oracle.c"]
     tmp = LIB.with_suffix(".so.tmp")
     cmd = ["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
-           "-std=gnu11", str(src), "-o", str(tmp), "-lm"]
+           "-std=gnu11", *map(str, srcs), "-o", str(tmp), "-lm"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("oracle build failed:\n" + r.stderr)

@@ -425,7 +425,8 @@ _clib = None
 def clib():
     global _clib
     if _clib is None:
-        if not LIB.exists():
+        srcs = [Path(__file__).with_name(n) for n in ("flame_oracle.c", "jpeg_oracle.c")]
+        if not LIB.exists() or any(p.stat().st_mtime > LIB.stat().st_mtime for p in srcs):
             from . import build
             build.build()
         lib = C.CDLL(str(LIB))
@@ -440,6 +441,8 @@ def clib():
         lib.orc_render.argtypes = [P, C.c_int, C.c_int, C.c_int, P, C.c_uint32, P, C.c_uint32, C.c_uint32,
                                    C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P, C.c_int]
         lib.orc_render.restype = C.c_uint64
+        lib.orc_jpeg_encode.argtypes = [P, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_size_t]
+        lib.orc_jpeg_encode.restype = C.c_long
         _clib = lib
     return _clib
 
@@ -582,3 +585,16 @@ def convert_image(img: np.ndarray, fmt: int, flip: bool = False, swap: bool = Fa
         return h.copy()
     v = np.minimum(h.astype(np.float64) * 255.0, 255.0)
     return np.trunc(np.maximum(v, 0.0)).astype(np.uint8)
+
+
+def jpeg_encode(img: np.ndarray, quality: int) -> bytes:
+    """baseline JFIF of a uint8 h x w x 3|4 image (jpeg_oracle.c)"""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w, c = img.shape
+    cap = 1 << 16
+    while True:
+        out = np.zeros(cap + 2 * 1024 * ((w + 7) // 8) * ((h + 7) // 8), dtype=np.uint8)
+        n = clib().orc_jpeg_encode(_ptr(img), w * c, c, w, h, quality, _ptr(out), out.size)
+        if n >= 0:
+            return out[:n].tobytes()
+        cap = -n

@@ -129,6 +129,10 @@ _SIGS = [
     ("rfk_tonemap_image", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_int, C.POINTER(TonemapArgs), _P]),
     ("rfk_density_hdr", C.c_int, [_P, C.c_int, C.c_size_t, C.POINTER(C.c_uint64), _P, _P]),
     ("rfk_convert_image", C.c_int, [_P, C.c_int, _P, C.c_int, C.c_uint32, C.c_uint32, C.c_int, _P]),
+    ("rfk_jpeg_create", C.c_int, [C.POINTER(_P)]),
+    ("rfk_jpeg_encode", C.c_int, [_P, _P, C.c_int, C.c_size_t, C.c_uint32, C.c_uint32, C.c_int, _P]),
+    ("rfk_jpeg_retrieve", C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("rfk_jpeg_free", C.c_int, [_P]),
     ("rfk_catmull_coefficients", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     ("rfk_fix_rotation", C.c_int, [C.POINTER(C.c_double), C.c_size_t, C.c_size_t, C.POINTER(C.c_size_t),
                                    C.c_size_t, C.c_int]),

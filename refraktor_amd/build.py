@@ -19,7 +19,7 @@ LIB = PKG / "librefrakt_amd.so"
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HOST_SOURCES = ["json.cpp", "yaml_lite.cpp", "flang.cpp", "vardb.cpp", "flame.cpp", "flam3.cpp", "compiler.cpp"]
-HIP_SOURCES = ["api.cpp", "kernels_static.hip", "queue_bin.hip", "unet.hip", "denoiser.cpp"]
+HIP_SOURCES = ["api.cpp", "kernels_static.hip", "queue_bin.hip", "unet.hip", "denoiser.cpp", "jpeg.hip"]
 # hipcc flags: gfx950 only, packed FP32 instructions disabled
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++20", "-ffp-contract=fast-honor-pragmas",

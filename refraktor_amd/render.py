@@ -429,3 +429,39 @@ def convert_image(img: np.ndarray, fmt: int, flip: bool = False, swap: bool = Fa
     check(_lib.lib().rfk_convert_image(src.ptr, _lib.PIXEL_HALF4 if c == 4 else _lib.PIXEL_HALF3, out.ptr, fmt, w, h,
                                        flags, None))
     return _host_pixels(out, fmt, h, w)
+
+
+class JpegEncoder:
+    """rfkt::jpeg_encoder on the device: 8-bit RGB / RGBA images (device
+    buffers or host arrays) to baseline JFIF bytes"""
+
+    def __init__(self):
+        self._lib = _lib.lib()
+        h = C.c_void_p()
+        check(self._lib.rfk_jpeg_create(C.byref(h)))
+        self.handle = h.value
+
+    def encode_device(self, ptr: int, width: int, height: int, quality: int = 90, fmt: int = _lib.PIXEL_RGB8,
+                      pitch: int = 0, stream=None) -> bytes:
+        bpp = 3 if fmt == _lib.PIXEL_RGB8 else 4
+        check(self._lib.rfk_jpeg_encode(self.handle, ptr, fmt, pitch or width * bpp, width, height, quality, stream))
+        n = C.c_size_t()
+        check(self._lib.rfk_jpeg_retrieve(self.handle, None, 0, C.byref(n)))
+        out = C.create_string_buffer(n.value)
+        check(self._lib.rfk_jpeg_retrieve(self.handle, out, n.value, C.byref(n)))
+        return out.raw[:n.value]
+
+    def encode(self, img: np.ndarray, quality: int = 90) -> bytes:
+        """a host uint8 h x w x 3 (RGB) or x 4 (RGBA) image"""
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        h, w, c = img.shape
+        buf = DeviceBuffer(img.nbytes)
+        buf.upload(img)
+        return self.encode_device(buf.ptr, w, h, quality, _lib.PIXEL_RGB8 if c == 3 else _lib.PIXEL_RGBA8)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self._lib.rfk_jpeg_free(self.handle)
+        except Exception:
+            pass

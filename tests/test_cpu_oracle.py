@@ -243,3 +243,59 @@ def test_convert_image_closed_forms():
     h4 = np.concatenate([x, np.full((2, 2, 1), 0.5, np.float16)], axis=-1)
     assert oracle.convert_image(h4, 4)[..., 3].tolist() == [[127, 127], [127, 127]]
     np.testing.assert_array_equal(oracle.convert_image(h4, 3, swap=True, flip=True), h4[::-1, :, [2, 1, 0]])
+
+
+def _jpeg_images():
+    y, x = np.mgrid[0:45, 0:70]
+    smooth = np.stack([x * 255 // 70, y * 255 // 45, (x + y) * 255 // 115], -1).astype(np.uint8)
+    noise = np.random.default_rng(3).integers(0, 256, size=(61, 83, 3), dtype=np.uint8)  # 11 x 8 MCUs
+    return smooth, noise
+
+
+def _decode(data: bytes) -> np.ndarray:
+    import io
+    from PIL import Image
+    im = Image.open(io.BytesIO(data))
+    assert im.format == "JPEG"
+    return np.asarray(im.convert("RGB")).astype(np.float64)
+
+
+def _psnr(a, b) -> float:
+    mse = float(((np.asarray(a, np.float64) - b) ** 2).mean())
+    return 99.0 if mse == 0 else 10 * math.log10(255.0 ** 2 / mse)
+
+
+@pytest.mark.parametrize("quality", [1, 10, 50, 75, 90, 100])
+def test_jpeg_oracle_decodes_like_libjpeg(quality):
+    """An independent decoder (PIL's libjpeg) reads the oracle's files, and
+    the decoded image is as close to the input as libjpeg's own 4:4:4
+    encoding at the same quality (same standard tables and IJG scaling):
+    within 0.5 dB and 10 % of its size."""
+    import io
+    from PIL import Image
+    smooth, _ = _jpeg_images()
+    data = oracle.jpeg_encode(smooth, quality)
+    assert data[:2] == b"\xff\xd8" and data[-2:] == b"\xff\xd9"
+    ours = _psnr(_decode(data), smooth)
+    buf = io.BytesIO()
+    Image.fromarray(smooth).save(buf, "JPEG", quality=quality, subsampling=0)
+    ref = _psnr(_decode(buf.getvalue()), smooth)
+    assert abs(ours - ref) < 0.5, (ours, ref)
+    assert abs(len(data) - len(buf.getvalue())) < 0.1 * len(buf.getvalue())
+
+
+def test_jpeg_oracle_stress_cases():
+    """noise at quality 100 (10-bit AC magnitudes, many 0xFF bytes stuffed),
+    RGBA input, sizes that are not whole blocks, a single pixel, and enough
+    MCUs for every restart marker RST0..RST7 to appear"""
+    _, noise = _jpeg_images()
+    data = oracle.jpeg_encode(noise, 100)
+    assert _psnr(_decode(data), noise) > 40.0
+    body = data[data.index(b"\xff\xda"):]
+    assert all(bytes([0xFF, 0xD0 + n]) in body for n in range(8))
+    rgba = np.concatenate([noise, np.full(noise.shape[:2] + (1,), 7, np.uint8)], -1)
+    assert oracle.jpeg_encode(rgba, 100) == data  # alpha is ignored
+    one = np.array([[[200, 30, 90]]], np.uint8)
+    np.testing.assert_allclose(_decode(oracle.jpeg_encode(one, 95))[0, 0], one[0, 0], atol=3)
+    gray = np.full((9, 17, 3), 128, np.uint8)  # mid grey: every coefficient zero
+    np.testing.assert_array_equal(_decode(oracle.jpeg_encode(gray, 50)), gray)
