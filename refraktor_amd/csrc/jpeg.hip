@@ -8,9 +8,9 @@
 // JPEG standard's Annex K with the IJG quality scaling, and a restart marker
 // every kRestart MCUs so the entropy-coded segments are independent:
 //
-//   1. jpeg_dct_kernel   one workgroup per 8x8 block, one wave per component:
+//   1. jpeg_dct_kernel   one wave per 8x8 block, one lane per coefficient:
 //                        colour conversion, integer 2-D DCT, quantisation,
-//                        zigzag order, and the block's non-zero mask (ballot)
+//                        zigzag order, and the block's non-zero masks (ballot)
 //   2. jpeg_huff_kernel  one wave per restart segment, one lane per block:
 //                        DC prediction, run-length + Huffman code lengths,
 //                        a wave prefix sum for the blocks' bit offsets, the
@@ -63,7 +63,8 @@ __constant__ uint8_t c_zigzag[64] = {0,  1,  8,  16, 9,  2,  3,  10, 17, 24, 32,
                                      58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
 
 struct quant_tables {
-    uint16_t q[2][64];  // luminance, chrominance; natural order
+    uint16_t q[2][64];      // luminance, chrominance; natural order
+    uint32_t magic[2][64];  // ceil(2^20 / q): floor(a / q) = (a * magic) >> 20 for a < 2^12
 };
 
 // Huffman code of each symbol: code | length << 16 (DC: 2 x 16, AC: 2 x 256)
@@ -72,52 +73,81 @@ struct huff_codes {
     uint32_t ac[2][256];
 };
 
-// grid = blocks (bx fastest), 192 threads: wave c = component c (Y, Cb, Cr),
-// lane k = zigzag position k. coef: [block][3][64] int16 in zigzag order;
-// nz: [block][3] bit k set when coefficient k is non-zero.
-__global__ void __launch_bounds__(192)
+constexpr uint32_t kDctWaves = 4;  // blocks per workgroup
+
+// LDS writes of a wave's lanes visible to its other lanes
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// One wave per 8x8 block (kDctWaves blocks per group, bx fastest), all three
+// components (Y, Cb, Cr) per lane. coef: [block][3][64] int16 in zigzag
+// order; nz: [block][3] bit k set when coefficient k is non-zero.
+__global__ void __launch_bounds__(64 * kDctWaves)
     jpeg_dct_kernel(const uint8_t* __restrict__ img, size_t pitch, uint32_t bpp, uint32_t width, uint32_t height,
-                    uint32_t blocks_x, quant_tables qt, int16_t* __restrict__ coef, uint64_t* __restrict__ nz) {
-    __shared__ int32_t s_px[3][64];
-    __shared__ int32_t s_t[3][64];
-    const uint32_t c = threadIdx.x >> 6, k = threadIdx.x & 63u;
-    const uint32_t b = blockIdx.x, bx = b % blocks_x, by = b / blocks_x;
-    // pixel k of the block, edges replicated
+                    uint32_t blocks_x, uint32_t blocks, quant_tables qt, int16_t* __restrict__ coef,
+                    uint64_t* __restrict__ nz) {
+    __shared__ int32_t s_px[kDctWaves][3][64];
+    __shared__ int32_t s_t[kDctWaves][3][64];
+    const uint32_t wv = threadIdx.x >> 6, k = __lane_id();
+    const uint32_t b = blockIdx.x * kDctWaves + wv;
+    if (b >= blocks) return;  // uniform per wave; only wave barriers follow
+    const uint32_t bx = b % blocks_x, by = b / blocks_x;
+    int32_t(*px)[64] = s_px[wv];
+    int32_t(*tt)[64] = s_t[wv];
+    // pixel k of the block, edges replicated; JFIF full-range YCbCr with 16
+    // fractional bits, chroma rounded with 2^15 - 1 so it stays below 256
     {
         const uint32_t x = std::min(bx * 8u + (k & 7u), width - 1u), y = std::min(by * 8u + (k >> 3), height - 1u);
         const uint8_t* p = img + size_t(y) * pitch + size_t(x) * bpp;
         const int32_t r = p[0], g = p[1], bl = p[2];
-        int32_t v;
-        if (c == 0)
-            v = (19595 * r + 38470 * g + 7471 * bl + 32768) >> 16;
-        else if (c == 1)
-            v = (-11059 * r - 21709 * g + 32768 * bl + (128 << 16) + 32767) >> 16;
-        else
-            v = (32768 * r - 27439 * g - 5329 * bl + (128 << 16) + 32767) >> 16;
-        s_px[c][k] = v - 128;
+        px[0][k] = ((19595 * r + 38470 * g + 7471 * bl + 32768) >> 16) - 128;
+        px[1][k] = ((-11059 * r - 21709 * g + 32768 * bl + (128 << 16) + 32767) >> 16) - 128;
+        px[2][k] = ((32768 * r - 27439 * g - 5329 * bl + (128 << 16) + 32767) >> 16) - 128;
     }
-    __syncthreads();
+    wave_sync();
     // rows: t[v][u] = sum_x D[u][x] s[v][x], kept with 3 fractional bits
     {
         const uint32_t v = k >> 3, u = k & 7u;
-        int32_t t = 0;
+        int32_t t0 = 0, t1 = 0, t2 = 0;
 #pragma unroll
-        for (uint32_t x = 0; x < 8; x++) t += int32_t(c_dct[u * 8 + x]) * s_px[c][v * 8 + x];
-        s_t[c][k] = (t + 1024) >> 11;
+        for (uint32_t x = 0; x < 8; x++) {
+            const int32_t dx = c_dct[u * 8 + x];
+            t0 += dx * px[0][v * 8 + x];
+            t1 += dx * px[1][v * 8 + x];
+            t2 += dx * px[2][v * 8 + x];
+        }
+        tt[0][k] = (t0 + 1024) >> 11;
+        tt[1][k] = (t1 + 1024) >> 11;
+        tt[2][k] = (t2 + 1024) >> 11;
     }
-    __syncthreads();
-    // columns at natural position p = zigzag[k], then quantise (round half away from zero)
+    wave_sync();
+    // columns at natural position p = zigzag[k] (the result scaled by 2^17),
+    // quantised with rounding half away from zero:
+    // floor((|f| + q 2^16) / (q 2^17)) = floor(((|f| + q 2^16) >> 17) / q)
     const uint32_t p = c_zigzag[k], v2 = p >> 3, u = p & 7u;
-    int32_t f = 0;
+    int32_t f[3] = {0, 0, 0};
 #pragma unroll
-    for (uint32_t v = 0; v < 8; v++) f += int32_t(c_dct[v2 * 8 + v]) * s_t[c][v * 8 + u];
-    const int32_t d = int32_t(qt.q[c ? 1 : 0][p]) << 17;
-    int32_t q = f >= 0 ? (f + (d >> 1)) / d : -((-f + (d >> 1)) / d);
-    // baseline AC magnitudes have at most 10 bits (8-bit input stays below 1000)
-    if (k) q = std::clamp(q, -1023, 1023);
-    coef[(size_t(b) * 3 + c) * 64 + k] = int16_t(q);
-    const uint64_t m = __ballot(q != 0);
-    if (k == 0) nz[size_t(b) * 3 + c] = m;
+    for (uint32_t v = 0; v < 8; v++) {
+        const int32_t dv = c_dct[v2 * 8 + v];
+        f[0] += dv * tt[0][v * 8 + u];
+        f[1] += dv * tt[1][v * 8 + u];
+        f[2] += dv * tt[2][v * 8 + u];
+    }
+#pragma unroll
+    for (uint32_t c = 0; c < 3; c++) {
+        const uint32_t t = c ? 1u : 0u;
+        const uint32_t a = (uint32_t(f[c] < 0 ? -f[c] : f[c]) + (uint32_t(qt.q[t][p]) << 16)) >> 17;
+        const int32_t mag = int32_t((a * qt.magic[t][p]) >> 20);
+        int32_t q = f[c] < 0 ? -mag : mag;
+        // baseline AC magnitudes have at most 10 bits (8-bit input stays below 1000)
+        if (k) q = std::clamp(q, -1023, 1023);
+        coef[(size_t(b) * 3 + c) * 64 + k] = int16_t(q);
+        const uint64_t m = __ballot(q != 0);
+        if (k == 0) nz[size_t(b) * 3 + c] = m;
+    }
 }
 
 __device__ __forceinline__ uint32_t magnitude_bits(int32_t v, uint32_t* size) {
@@ -238,9 +268,7 @@ __global__ void __launch_bounds__(64 * kHuffWaves)
         const uint32_t padn = 8u - (total & 7u);
         w.put((1u << padn) - 1u, padn);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    wave_sync();
     // bytes out, 0x00 after every 0xFF: lane L copies bytes [L * per, ...)
     const uint32_t nbytes = (total + 7u) >> 3, per = (nbytes + 63u) >> 6;
     const uint32_t lo = std::min(nbytes, lane * per), hi = std::min(nbytes, lo + per);
@@ -359,8 +387,10 @@ quant_tables make_quant(int quality) {
     const int scale = quality < 50 ? 5000 / quality : 200 - 2 * quality;
     quant_tables qt;
     for (int t = 0; t < 2; t++)
-        for (int i = 0; i < 64; i++)
+        for (int i = 0; i < 64; i++) {
             qt.q[t][i] = uint16_t(std::clamp((int(kStdQuant[t][i]) * scale + 50) / 100, 1, 255));
+            qt.magic[t][i] = ((1u << 20) + qt.q[t][i] - 1u) / qt.q[t][i];
+        }
     return qt;
 }
 
@@ -407,7 +437,8 @@ struct rfk_jpeg {
     uint8_t *seg_buf = nullptr, *out = nullptr;
     uint32_t *seg_len = nullptr, *seg_off = nullptr, *total_dev = nullptr;
     uint32_t* total_host = nullptr;  // pinned
-    size_t cap_blocks = 0, cap_segments = 0;
+    uint8_t* stage = nullptr;        // pinned copy of the coded segments
+    size_t cap_blocks = 0, cap_segments = 0, cap_stage = 0;
     hipEvent_t done = nullptr;
     std::vector<uint8_t> header;
     bool pending = false;
@@ -475,8 +506,8 @@ int rfk_jpeg_encode(rfk_jpeg* j, const void* image, int in_format, size_t pitch_
     }
     const quant_tables qt = make_quant(quality);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    jpeg_dct_kernel<<<uint32_t(blocks), 192, 0, s>>>(static_cast<const uint8_t*>(image), pitch_bytes, bpp, width,
-                                                      height, bx, qt, j->coef, j->nz);
+    jpeg_dct_kernel<<<uint32_t((blocks + kDctWaves - 1) / kDctWaves), 64 * kDctWaves, 0, s>>>(
+        static_cast<const uint8_t*>(image), pitch_bytes, bpp, width, height, bx, uint32_t(blocks), qt, j->coef, j->nz);
     jpeg_huff_kernel<<<(segments + kHuffWaves - 1) / kHuffWaves, 64 * kHuffWaves, 0, s>>>(
         j->coef, j->nz, uint32_t(blocks), segments, j->d_codes, j->seg_buf, j->seg_len);
     jpeg_scan_kernel<<<1, 1024, 0, s>>>(j->seg_len, segments, j->seg_off, j->total_dev);
@@ -493,12 +524,20 @@ int rfk_jpeg_retrieve(rfk_jpeg* j, uint8_t* out, size_t cap, size_t* size) {
     if (!j || !size) return fail(RFK_ERR_INVALID, "null argument");
     if (!j->pending) return fail(RFK_ERR_INVALID, "no encoded image: call rfk_jpeg_encode first");
     RFK_JHIP(hipEventSynchronize(j->done));
-    const size_t total = j->header.size() + *j->total_host + 2;
+    const size_t coded = *j->total_host, total = j->header.size() + coded + 2;
     *size = total;
     if (!out) return RFK_OK;
     if (cap < total) return fail(RFK_ERR_INVALID, "output buffer too small for the JPEG file");
+    if (coded > j->cap_stage) {
+        if (j->stage) (void)hipHostFree(j->stage);
+        j->stage = nullptr;
+        j->cap_stage = 0;
+        RFK_JHIP(hipHostMalloc(reinterpret_cast<void**>(&j->stage), coded * 2, hipHostMallocDefault));
+        j->cap_stage = coded * 2;
+    }
     std::memcpy(out, j->header.data(), j->header.size());
-    RFK_JHIP(hipMemcpy(out + j->header.size(), j->out, *j->total_host, hipMemcpyDeviceToHost));
+    RFK_JHIP(hipMemcpy(j->stage, j->out, coded, hipMemcpyDeviceToHost));
+    std::memcpy(out + j->header.size(), j->stage, coded);
     out[total - 2] = 0xFF;
     out[total - 1] = 0xD9;
     return RFK_OK;
@@ -511,6 +550,7 @@ int rfk_jpeg_free(rfk_jpeg* j) {
     if (j->d_codes) (void)hipFree(j->d_codes);
     if (j->total_dev) (void)hipFree(j->total_dev);
     if (j->total_host) (void)hipHostFree(j->total_host);
+    if (j->stage) (void)hipHostFree(j->stage);
     if (j->done) (void)hipEventDestroy(j->done);
     delete j;
     return RFK_OK;
