@@ -103,17 +103,23 @@ RFK_INLINE Real fma(Real a, Real b, Real c) { return ::fma(a, b, c); }
 #else
 #define RFK_M1(name, f) RFK_INLINE Real name(Real a) { return ::f(a); }
 #define RFK_M2(name, f) RFK_INLINE Real name(Real a, Real b) { return ::f(a, b); }
-RFK_M1(sin, sinf) RFK_M1(cos, cosf) RFK_M1(tan, tanf) RFK_M1(sinh, sinhf) RFK_M1(cosh, coshf) RFK_M1(tanh, tanhf)
-RFK_M1(asin, asinf) RFK_M1(acos, acosf) RFK_M1(atan, atanf) RFK_M1(asinh, asinhf) RFK_M1(acosh, acoshf)
-RFK_M1(atanh, atanhf) RFK_M1(sqrt, sqrtf) RFK_M1(rsqrt, rsqrtf) RFK_M1(exp, expf) RFK_M1(log, logf)
-RFK_M1(exp2, exp2f) RFK_M1(log2, log2f) RFK_M1(exp10, exp10f) RFK_M1(log10, log10f) RFK_M1(abs, fabsf)
-RFK_M1(floor, floorf) RFK_M1(sinpi, sinpif) RFK_M1(cospi, cospif)
-RFK_M2(atan2, atan2f) RFK_M2(hypot, hypotf) RFK_M2(pow, powf) RFK_M2(modf, fmodf) RFK_M2(copysign, copysignf)
-RFK_INLINE Real tanpi(Real a) { return ::tanf(a * pi); }
+// The reference compiles flame kernels with fast math
+// (src/librefrakt/src/flame_compiler.cpp:861, use_fast_math), under which
+// sin, cos, tan, sincos, exp, exp10, log, log2, log10 and pow are the
+// hardware approximations; the f32 path does the same with HIP's native
+// forms (v_sin/v_cos/v_exp/v_log based, no range reduction), the f64 path
+// keeps the library functions.
+RFK_M1(sin, __sinf) RFK_M1(cos, __cosf) RFK_M1(tan, __tanf) RFK_M1(sinh, sinhf) RFK_M1(cosh, coshf)
+RFK_M1(tanh, tanhf) RFK_M1(asin, asinf) RFK_M1(acos, acosf) RFK_M1(atan, atanf) RFK_M1(asinh, asinhf)
+RFK_M1(acosh, acoshf) RFK_M1(atanh, atanhf) RFK_M1(sqrt, sqrtf) RFK_M1(rsqrt, rsqrtf) RFK_M1(exp, __expf)
+RFK_M1(log, __logf) RFK_M1(exp2, exp2f) RFK_M1(log2, __log2f) RFK_M1(exp10, __exp10f) RFK_M1(log10, __log10f)
+RFK_M1(abs, fabsf) RFK_M1(floor, floorf) RFK_M1(sinpi, sinpif) RFK_M1(cospi, cospif)
+RFK_M2(atan2, atan2f) RFK_M2(hypot, hypotf) RFK_M2(pow, __powf) RFK_M2(modf, fmodf) RFK_M2(copysign, copysignf)
+RFK_INLINE Real tanpi(Real a) { return ::__tanf(a * pi); }
 RFK_INLINE int trunc(Real a) { return int(::truncf(a)); }
 RFK_INLINE int rint(Real a) { return int(::rintf(a)); }
-RFK_INLINE v2 sincos(Real a) { v2 r; ::sincosf(a, &r.x, &r.y); return r; }
-RFK_INLINE v2 cossin(Real a) { v2 r; ::sincosf(a, &r.y, &r.x); return r; }
+RFK_INLINE v2 sincos(Real a) { v2 r; ::__sincosf(a, &r.x, &r.y); return r; }
+RFK_INLINE v2 cossin(Real a) { v2 r; ::__sincosf(a, &r.y, &r.x); return r; }
 RFK_INLINE v2 sincospi(Real a) { v2 r; ::sincospif(a, &r.x, &r.y); return r; }
 RFK_INLINE v2 cossinpi(Real a) { v2 r; ::sincospif(a, &r.y, &r.x); return r; }
 RFK_INLINE Real fma(Real a, Real b, Real c) { return ::fmaf(a, b, c); }
