@@ -71,11 +71,24 @@ def main() -> int:
     ap.add_argument("--no-denoise", action="store_true", help="plain path only")
     ap.add_argument("--pipeline", type=int, default=3,
                     help="frames in flight, each on its own streams (both paths alike); 1 = one frame at a time")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only three 1 GiB device-to-device copies: the known-size traffic that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of the bin call into bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py runs on one GPU; --gpus above 1 is not supported",
                           "n_gpus": args.gpus}))
         return 2
+    if args.copy_calibration:
+        from refraktor_amd import _lib, render
+        n = 1 << 30
+        src, dst = render.DeviceBuffer(n), render.DeviceBuffer(n)
+        src.zero()
+        for _ in range(3):
+            _lib.check(_lib.lib().rfk_memcpy_dtod(dst.ptr, src.ptr, n))
+        _lib.check(_lib.lib().rfk_synchronize())
+        print(json.dumps({"copy_calibration": {"bytes_per_copy": n, "copies": 3}}))
+        return 0
 
     import ctypes as C
 
