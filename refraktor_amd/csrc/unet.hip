@@ -522,8 +522,11 @@ __device__ __forceinline__ bf16x8 load_px8(const bf16* __restrict__ x1n, int c1,
 // ends up with 16 channels of one pixel and stores them as 4-channel vectors.
 // MODE is x1's read (0 direct, 1 2x upsample, 2 2x2 average pool); element
 // offsets inside one image are 32-bit (the host checks they fit).
-template <int KC, int MODE>
-__global__ __launch_bounds__(256) void conv3x3_strip_kernel(const bf16* __restrict__ x1, int c1,
+// NW waves per workgroup share the layer's weights in LDS and never wait for
+// each other once the weights are staged; more of them per workgroup means
+// more resident waves per CU for the same LDS
+template <int KC, int MODE, int NW>
+__global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __restrict__ x1, int c1,
                                                             const bf16* __restrict__ x2, int c2, int H, int W,
                                                             const bf16* __restrict__ wgt,
                                                             const float* __restrict__ bias, int Cout, int act,
@@ -537,7 +540,7 @@ __global__ __launch_bounds__(256) void conv3x3_strip_kernel(const bf16* __restri
     const int t = int(threadIdx.x), wv = t >> 6, l = t & 63, m = l & 31, kh = l >> 5;
     bf16* s_w = reinterpret_cast<bf16*>(conv_smem);  // [9 taps][32 out][WP]
     bf16* s_x = s_w + 9 * 32 * WP + wv * SPX * XP;   // this wave's staged chunk [SPX][XP]
-    const int x0 = int(blockIdx.x) * 128 + wv * 32;  // the wave's first output pixel
+    const int x0 = int(blockIdx.x) * 32 * NW + wv * 32;  // the wave's first output pixel
     const int y0 = int(blockIdx.y) * rows;
     const int ntile = (Cout + 31) / 32;
     const int n = int(blockIdx.z) / ntile, co0 = (int(blockIdx.z) % ntile) * 32;
@@ -545,7 +548,7 @@ __global__ __launch_bounds__(256) void conv3x3_strip_kernel(const bf16* __restri
     const bf16* x1n = x1 + size_t(n) * H1 * W1 * c1;
     const bf16* x2n = x2 ? x2 + size_t(n) * H * W * c2 : nullptr;
 
-    for (int e = t; e < 9 * 32 * GW; e += 256) {
+    for (int e = t; e < 9 * 32 * GW; e += 64 * NW) {
         const int g = e % GW, co = (e / GW) % 32, tap = e / (32 * GW);
         bf16x8 v = zero8();
         if (co0 + co < Cout && g * 8 < Cin) v = load8(wgt + unsigned(((co0 + co) * 9 + tap) * Cin + g * 8));
@@ -858,15 +861,26 @@ static void launch_wstat(const void* x1, int c1, int mode, const void* x2, int c
 #undef RFK_WS
 }
 
-static size_t strip_lds_bytes(int Cin, int KC) {
+// the layer's weights for all nine taps, as the strip kernel keeps them in LDS
+static size_t strip_weight_bytes(int Cin, int KC) {
     const int CinP = (Cin + KC - 1) / KC * KC;
-    return (size_t(9) * 32 * (CinP + PAD) + size_t(4) * 34 * (KC + PAD)) * sizeof(bf16);
+    return size_t(9) * 32 * (CinP + PAD) * sizeof(bf16);
+}
+
+// waves per strip workgroup by x1's read mode (measured per layer at 1080p:
+// direct reads gain up to 10 % from 16 waves over 4, the upsampling read is
+// fastest with 8 (96->32 at 1080p 166 -> 131 us), the pooling read with 4)
+static int strip_waves(int mode) { return mode == 1 ? 8 : mode == 2 ? 4 : 16; }
+
+static size_t strip_lds_bytes(int Cin, int KC, int nw) {
+    const int CinP = (Cin + KC - 1) / KC * KC;
+    return (size_t(9) * 32 * (CinP + PAD) + size_t(nw) * 34 * (KC + PAD)) * sizeof(bf16);
 }
 
 // Rows per strip: a workgroup's time grows with rows + 2 (the halo rows), and
 // a launch takes ceil(workgroups / resident workgroups) rounds, so pick the
 // row count that minimises rounds x (rows + 2).
-static int strip_rows(const void* fn, size_t lds, int zdim, int H, int W) {
+static int strip_rows(const void* fn, size_t lds, int nw, int zdim, int H, int W) {
     thread_local std::unordered_map<size_t, int> resident_cache;
     int resident = 0;
     const size_t key = reinterpret_cast<size_t>(fn) * 1000003u + lds;
@@ -877,11 +891,11 @@ static int strip_rows(const void* fn, size_t lds, int zdim, int H, int W) {
         int dev = 0, cus = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds) != hipSuccess)
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * nw, lds) != hipSuccess)
             return 16;
         resident = resident_cache[key] = std::max(1, cus * per_cu);
     }
-    const long cols = (W + 127) / 128;
+    const long cols = (W + 32 * nw - 1) / (32 * nw);
     int best = 16;
     long best_cost = -1;
     for (int r = 4; r <= 64; r++) {
@@ -895,26 +909,34 @@ static int strip_rows(const void* fn, size_t lds, int zdim, int H, int W) {
     return best;
 }
 
+template <int KC, int M, int NW>
+static hipError_t launch_strip_mode(const void* x1, int c1, const void* x2, int c2, int N, int H, int W,
+                                    const void* wgt, const float* bias, int Cout, int act, float slope, void* y,
+                                    float* yf, size_t lds, hipStream_t s, const float* rbase, float* rgb) {
+    const void* fn = reinterpret_cast<const void*>(&conv3x3_strip_kernel<KC, M, NW>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+    const int rows = strip_rows(fn, lds, NW, N * ((Cout + 31) / 32), H, W);
+    dim3 grid((W + 32 * NW - 1) / (32 * NW), (H + rows - 1) / rows, N * ((Cout + 31) / 32));
+    conv3x3_strip_kernel<KC, M, NW><<<grid, 64 * NW, lds, s>>>((const bf16*)x1, c1, (const bf16*)x2, c2, H, W,
+                                                                (const bf16*)wgt, bias, Cout, act, slope, rows,
+                                                                (bf16*)y, yf, rbase, rgb);
+    return hipGetLastError();
+}
+
+// lds = strip_lds_bytes(Cin, KC, strip_waves(mode))
 template <int KC>
 static hipError_t launch_strip(const void* x1, int c1, int mode, const void* x2, int c2, int N, int H, int W,
                                const void* wgt, const float* bias, int Cout, int act, float slope, void* y, float* yf,
                                size_t lds, hipStream_t s, const float* rbase = nullptr, float* rgb = nullptr) {
-    const void* fn = mode == 1   ? reinterpret_cast<const void*>(&conv3x3_strip_kernel<KC, 1>)
-                     : mode == 2 ? reinterpret_cast<const void*>(&conv3x3_strip_kernel<KC, 2>)
-                                 : reinterpret_cast<const void*>(&conv3x3_strip_kernel<KC, 0>);
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-    if (e != hipSuccess) return e;
-    const int rows = strip_rows(fn, lds, N * ((Cout + 31) / 32), H, W);
-    dim3 grid((W + 127) / 128, (H + rows - 1) / rows, N * ((Cout + 31) / 32));
-#define RFK_ST(M)                                                                                               \
-    conv3x3_strip_kernel<KC, M><<<grid, 256, lds, s>>>((const bf16*)x1, c1, (const bf16*)x2, c2, H, W,          \
-                                                       (const bf16*)wgt, bias, Cout, act, slope, rows, (bf16*)y, yf, \
-                                                       rbase, rgb)
-    if (mode == 1) RFK_ST(1);
-    else if (mode == 2) RFK_ST(2);
-    else RFK_ST(0);
-#undef RFK_ST
-    return hipGetLastError();
+    if (mode == 1)
+        return launch_strip_mode<KC, 1, 8>(x1, c1, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s, rbase,
+                                           rgb);
+    if (mode == 2)
+        return launch_strip_mode<KC, 2, 4>(x1, c1, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s, rbase,
+                                           rgb);
+    return launch_strip_mode<KC, 0, 16>(x1, c1, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s, rbase,
+                                        rgb);
 }
 
 template <int R, int C, int KCH>
@@ -936,14 +958,13 @@ hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
     const int H1 = x1_mode == 2 ? 2 * H : H, W1 = x1_mode == 2 ? 2 * W : W;
     const bool offsets32 = size_t(H1) * W1 * c1 < (size_t(1) << 31) && size_t(H) * W * std::max(c2, Cout) < (size_t(1) << 31) &&
                            size_t(Cout) * 9 * Cin < (size_t(1) << 31);
-    // the input-stationary strip kernel wherever its weights leave room for
-    // two workgroups per CU (inputs up to 96 channels) and the output has at
-    // most three 32-channel tiles: measured fastest there (96->32 at 1080p
-    // 166 us against 192 us weight-stationary, 96->96 at 270x480 45 against
-    // 50 us)
+    // the input-stationary strip kernel wherever the layer's weights fit in
+    // 64 KB of LDS (inputs up to 96 channels) and the output has at most three
+    // 32-channel tiles: measured fastest there (96->32 at 1080p 131 us against
+    // 192 us weight-stationary, 96->96 at 270x480 40 against 50 us)
     const int KC = Cin <= 16 ? 16 : 32;
-    const size_t lds = strip_lds_bytes(Cin, KC);
-    if (Cout <= 96 && lds <= 80 * 1024 && offsets32) {
+    const size_t lds = strip_lds_bytes(Cin, KC, strip_waves(x1_mode));
+    if (Cout <= 96 && strip_weight_bytes(Cin, KC) <= 64 * 1024 && lds <= 160 * 1024 && offsets32) {
         if (KC == 16) return launch_strip<16>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s);
         return launch_strip<32>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s);
     }
@@ -972,8 +993,9 @@ hipError_t conv3x3_forward_rgb(const void* x, int cin, int N, int H, int W, cons
                                int Cout, int act, const float* base, float* out_rgb, hipStream_t s) {
     if ((cin % 8) || cin <= 0 || Cout < 3 || Cout > 32 || !base || !out_rgb) return hipErrorInvalidValue;
     const int KC = cin <= 16 ? 16 : 32;
-    const size_t lds = strip_lds_bytes(cin, KC);
-    if (lds > 80 * 1024 || size_t(H) * W * std::max(cin, 3) >= (size_t(1) << 31))
+    const size_t lds = strip_lds_bytes(cin, KC, strip_waves(0));
+    if (strip_weight_bytes(cin, KC) > 64 * 1024 || lds > 160 * 1024 ||
+        size_t(H) * W * std::max(cin, 3) >= (size_t(1) << 31))
         return hipErrorInvalidValue;
     if (KC == 16)
         return launch_strip<16>(x, cin, 0, nullptr, 0, N, H, W, wgt, bias, Cout, act, 0.f, nullptr, nullptr, lds, s, base,
