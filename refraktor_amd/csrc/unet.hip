@@ -959,17 +959,18 @@ hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
     const bool offsets32 = size_t(H1) * W1 * c1 < (size_t(1) << 31) && size_t(H) * W * std::max(c2, Cout) < (size_t(1) << 31) &&
                            size_t(Cout) * 9 * Cin < (size_t(1) << 31);
     // the input-stationary strip kernel wherever the layer's weights fit in
-    // 64 KB of LDS (inputs up to 96 channels) and the output has at most three
-    // 32-channel tiles: measured fastest there (96->32 at 1080p 131 us against
-    // 192 us weight-stationary, 96->96 at 270x480 40 against 50 us)
+    // 140 KB of LDS next to the staging rows (inputs up to 240 channels) and
+    // the output has at most three 32-channel tiles: measured fastest there
+    // (96->32 at 1080p 131 us against 192 us weight-stationary, 160->64 at
+    // 540x960 116 against 150 us, 224->96 at 270x480 82 against 94 us)
     const int KC = Cin <= 16 ? 16 : 32;
     const size_t lds = strip_lds_bytes(Cin, KC, strip_waves(x1_mode));
-    if (Cout <= 96 && strip_weight_bytes(Cin, KC) <= 64 * 1024 && lds <= 160 * 1024 && offsets32) {
+    if (Cout <= 96 && strip_weight_bytes(Cin, KC) <= 140 * 1024 && lds <= 160 * 1024 && offsets32) {
         if (KC == 16) return launch_strip<16>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s);
         return launch_strip<32>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s);
     }
-    // wider inputs: the weight-stationary strip (160->64 at 540x960 149 us
-    // against 192 us tiled); the 128-channel outputs run on the tiled kernel
+    // wider inputs: the weight-stationary strip; the 128-channel outputs run
+    // on the tiled kernel (128->128 at 135x240 39 us against 44 us strip)
     if (Cin >= 96 && Cout <= 96 && offsets32) {
         launch_wstat<4>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
         return hipGetLastError();
@@ -994,7 +995,7 @@ hipError_t conv3x3_forward_rgb(const void* x, int cin, int N, int H, int W, cons
     if ((cin % 8) || cin <= 0 || Cout < 3 || Cout > 32 || !base || !out_rgb) return hipErrorInvalidValue;
     const int KC = cin <= 16 ? 16 : 32;
     const size_t lds = strip_lds_bytes(cin, KC, strip_waves(0));
-    if (strip_weight_bytes(cin, KC) > 64 * 1024 || lds > 160 * 1024 ||
+    if (strip_weight_bytes(cin, KC) > 140 * 1024 || lds > 160 * 1024 ||
         size_t(H) * W * std::max(cin, 3) >= (size_t(1) << 31))
         return hipErrorInvalidValue;
     if (KC == 16)

@@ -59,10 +59,12 @@ def test_conv_forward_matches_oracle(gpu, case):
 
 
 @pytest.mark.parametrize("mode,Co,c1,c2", [(1, 64, 32, 16), (2, 64, 32, 16), (1, 32, 32, 16), (2, 32, 32, 16),
-                                           (1, 64, 96, 64), (1, 32, 64, 32), (2, 96, 96, 8), (1, 96, 128, 96)])
+                                           (1, 64, 96, 64), (1, 32, 64, 32), (2, 96, 96, 8), (1, 96, 128, 96),
+                                           (1, 96, 160, 96)])
 def test_conv_fused_upsample_and_pool(gpu, mode, Co, c1, c2):
     """x1 read through a nearest 2x upsample (1) or a 2x2 average pool (2);
-    inputs above 96 channels take the weight-stationary kernel"""
+    the strip kernel at every read mode, and the weight-stationary kernel
+    for 256 input channels (weights too large for the strip kernel's LDS)"""
     rng = np.random.default_rng(40 + mode + Co + c1)
     N, H, W = 1, 10, 136
     if mode == 1:
