@@ -102,6 +102,26 @@ def test_queue_batch_matches_atomic_batch(gpu, db):
             np.testing.assert_array_equal(queued.batch.histogram(f), eager[f], err_msg=f"replay {rep} frame {f}")
 
 
+def test_queue_batch_64bit_bins(gpu, db):
+    """the queue batch's per-frame histogram offsets in 64-bit bins (the
+    converged-render format): every frame equal to the atomic 64-bit batch"""
+    kf = _keyframes(4, 8)
+    aligned, samples = animation.control_samples(db, kf, W, H)
+    n, walkers = 10, 1 << 16
+    iters = animation.frame_budgets(n, 9, 1 << 23, 1 << 25)
+    ka = db.compile(aligned[0], hist_bits=64)
+    kq = db.compile(aligned[0], hist_bits=64, bin_queue=True)
+    a = ka.batch(samples, n, iters, W, H, walkers=walkers, seed=4)
+    q = kq.batch(samples, n, iters, W, H, walkers=walkers, seed=4)
+    a.run(graph=False)
+    q.run(graph=True)
+    for f in range(n):
+        assert q.frame(f)["hist_bits"] == 64
+        hq = q.histogram(f)
+        assert hq.dtype == np.uint64 and int(hq[..., 3].sum()) > 0
+        np.testing.assert_array_equal(hq, a.histogram(f), err_msg=f"frame {f}")
+
+
 def test_batch_rejects_bad_arguments(gpu, db):
     k = db.compile(fl.flame_keyframe(1))
     s = k.pack_samples(W, H)
