@@ -5,7 +5,7 @@ seeded synthetic flame). Two paths, each timed over --steps frames:
   plain     warmup (sample evaluation, precalc, walker seeding, 20 fuse
             passes), 2^30 binned iterations, density estimation (radius 9),
             log-density tonemap to half4;
-  denoised  the same pipeline at 2^26 iterations, then the U-Net (features,
+  denoised  the same pipeline at 2^25 iterations, then the U-Net (features,
             forward) to RGB.
 Both are scored by PSNR against a converged 2^32-iteration render of the same
 flame. `value` is the frames/s of the denoised path when it reaches 40 dB
@@ -64,7 +64,7 @@ def main() -> int:
     ap.add_argument("--walkers", type=int, default=1 << 20, help="walkers of the plain path")
     ap.add_argument("--low-walkers", type=int, default=1 << 19,
                     help="walkers of the denoised path (its small budget pays less warmup with fewer)")
-    ap.add_argument("--low-iters", type=int, default=1 << 26, help="budget of the denoised path")
+    ap.add_argument("--low-iters", type=int, default=1 << 25, help="budget of the denoised path")
     ap.add_argument("--weights", type=str, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
     ap.add_argument("--no-psnr", action="store_true", help="skip the converged reference render")

@@ -2,6 +2,10 @@
 
     python -m refraktor_amd.train_denoiser --flames 256 --steps 4000 --out weights.bin
 
+The shipped weights (weights/unet_1080.bin) came from
+    python -m refraktor_amd.train_denoiser --flames 600 --holdout 8 --steps 60000 \
+        --batch 16 --lr 2e-3 --low 33554432 --out unet_1080.bin
+
 Pairs: random seeded flames (keyframe flames of 6-10 xforms and 1080p flames
 with a final xform, alternating), the low-budget render as input features and
 the converged render as target, both through density estimation and the
@@ -118,7 +122,7 @@ def main(argv=None) -> int:
     ap.add_argument("--holdout", type=int, default=8)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--low", type=int, default=1 << 26)
+    ap.add_argument("--low", type=int, default=1 << 25)
     ap.add_argument("--high", type=int, default=1 << 32)
     ap.add_argument("--de-radius", type=float, default=9.0)
     ap.add_argument("--steps", type=int, default=4000)

@@ -168,13 +168,14 @@ def test_training_reduces_loss(gpu):
 @pytest.mark.parametrize("flame_maker,seed", [(fl.flame_1080, 1), (fl.flame_keyframe, 7)])
 def test_trained_denoiser_gain(gpu, db, flame_maker, seed):
     """on flames outside the training set, the trained network brings a
-    2^26-iteration frame closer to the 2^32-iteration render than the plain
-    low-budget pipeline (density estimation + tonemap) does"""
+    2^25-iteration frame (the budget it is trained and shipped for) closer
+    to the 2^32-iteration render than the plain low-budget pipeline (density
+    estimation + tonemap) does"""
     W_, H_ = 1920, 1080
     f = flame_maker(seed)
     shape = dict(threads_per_block=1024, atomic_width=64, bin_tile_log2=12, hist_bits=64)
     k = db.compile(f, **shape)
-    lo = denoise.render_frame(db, f, W_, H_, 1 << 26, seed=3, de_radius=9.0, kernel=k)
+    lo = denoise.render_frame(db, f, W_, H_, 1 << 25, seed=3, de_radius=9.0, kernel=k)
     hi = denoise.render_frame(db, f, W_, H_, 1 << 32, seed=4, de_radius=9.0, kernel=k)
     den = denoise.Denoiser(weights=denoise.DEFAULT_WEIGHTS)
     out = denoise.denoise_frame(den, lo)
