@@ -149,7 +149,9 @@ __global__ void density_histogram_kernel(const BinT* __restrict__ hist, size_t n
                                          unsigned long long* __restrict__ bins97,
                                          unsigned long long* __restrict__ max_density) {
     constexpr unsigned kBins = 4 * 24 + 1;
-    __shared__ unsigned long long s_bins[kBins];
+    // a workgroup counts fewer than 2^32 pixels: 32-bit LDS counters, added
+    // into the 64-bit global bins once
+    __shared__ unsigned s_bins[kBins];
     __shared__ unsigned long long s_max;
     for (unsigned i = threadIdx.x; i < kBins; i += blockDim.x) s_bins[i] = 0;
     if (threadIdx.x == 0) s_max = 0;
@@ -164,17 +166,17 @@ __global__ void density_histogram_kernel(const BinT* __restrict__ hist, size_t n
         if (value >= 1.0f) {
             unsigned long long b = (unsigned long long)(logf(value) * log_div) + 1ull;
             if (b >= kBins) b = kBins - 1;
-            atomicAdd(&s_bins[b], 1ull);
+            atomicAdd(&s_bins[b], 1u);
             mx = max(mx, (unsigned long long)value);
         } else {
             low++;
         }
     }
-    if (low) atomicAdd(&s_bins[0], low);
+    if (low) atomicAdd(&s_bins[0], unsigned(low));
     if (mx) atomicMax(&s_max, mx);
     __syncthreads();
     for (unsigned i = threadIdx.x; i < kBins; i += blockDim.x)
-        if (s_bins[i]) atomicAdd(&bins97[i], s_bins[i]);
+        if (s_bins[i]) atomicAdd(&bins97[i], (unsigned long long)s_bins[i]);
     if (threadIdx.x == 0) atomicMax(max_density, s_max);
 }
 
@@ -493,7 +495,10 @@ hipError_t bin_points(const float* screen6, const uint32_t* palette, const float
 
 hipError_t density_histogram(const uint32_t* hist, bool hist64, size_t npix, unsigned long long* bins97,
                              unsigned long long* max_density, hipStream_t s) {
-    unsigned grid = std::min(nblocks(npix), 2048u);
+    // one workgroup per CU: every workgroup ends with one global atomic per
+    // non-empty bin, on the same 98 addresses (2048 workgroups: 33.5 us at
+    // 1080p, 256: 19.6 us)
+    unsigned grid = std::min(nblocks(npix), 256u);
     if (grid == 0) grid = 1;
     if (hist64)
         density_histogram_kernel<<<grid, kBlock, 0, s>>>(reinterpret_cast<const bin64*>(hist), npix, bins97,
