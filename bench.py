@@ -18,7 +18,8 @@ Each path is also timed one frame at a time (`sequential_ms`, the latency).
 
 Prints one JSON line with both paths, iterations/s, the binning roofline
 (16 B of histogram increment per draw over the bin call's device time; the
-HBM traffic of the same call from the committed rocprofv3 counter summary)
+HBM traffic of the same call from the committed rocprofv3 counter summary),
+the denoiser's MFMA roofline (the U-Net forward's FLOPs over its device time)
 and the host baseline (the oracle's OpenMP render of the same flame on a
 bounded sample).
 """
@@ -47,6 +48,46 @@ def psnr(a, b) -> float:
     import numpy as np
     mse = float(np.mean((a[..., :3].astype(np.float64) - b[..., :3].astype(np.float64)) ** 2))
     return float("inf") if mse == 0 else 10.0 * math.log10(1.0 / mse)
+
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
+
+
+def unet_flops(h: int, w: int) -> float:
+    """multiply-adds x 2 of the denoiser's 15 3x3 convolutions at h x w
+    (widths 32/64/96/128, 8 input and 8 output channels, skips concatenated)"""
+    c = (32, 64, 96, 128)
+    lv = [(h >> i, w >> i) for i in range(4)]
+    layers = [(0, 8, c[0]), (0, c[0], c[0]), (1, c[0], c[1]), (1, c[1], c[1]), (2, c[1], c[2]), (2, c[2], c[2]),
+              (3, c[2], c[3]), (3, c[3], c[3]), (2, c[3] + c[2], c[2]), (2, c[2], c[2]), (1, c[2] + c[1], c[1]),
+              (1, c[1], c[1]), (0, c[1] + c[0], c[0]), (0, c[0], c[0]), (0, c[0], 8)]
+    return float(sum(2 * 9 * lv[l][0] * lv[l][1] * ci * co for l, ci, co in layers))
+
+
+def unet_roofline(lane, h: int, w: int, reps: int = 10) -> dict:
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    check_hip = lambda e: None if e == 0 else (_ for _ in ()).throw(RuntimeError(f"hip error {e}"))
+    check_hip(hip.hipEventCreate(C.byref(ev0)))
+    check_hip(hip.hipEventCreate(C.byref(ev1)))
+    lane.den.forward(lane.feat, lane.base, 1, h, w, lane.rgb)  # warm
+    check_hip(hip.hipEventRecord(ev0, None))
+    for _ in range(reps):
+        lane.den.forward(lane.feat, lane.base, 1, h, w, lane.rgb)
+    check_hip(hip.hipEventRecord(ev1, None))
+    check_hip(hip.hipEventSynchronize(ev1))
+    ms = C.c_float()
+    check_hip(hip.hipEventElapsedTime(C.byref(ms), ev0, ev1))
+    hip.hipEventDestroy(ev0)
+    hip.hipEventDestroy(ev1)
+    per = ms.value / reps
+    flops = unet_flops(h, w)
+    tflops = flops / (per * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tflops / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
+            "kernel": "U-Net forward: 15 conv3x3 (BF16 MFMA, FP32 accumulate) + 3 average pools",
+            "flops_per_launch": flops, "launch_ms": per}
 
 
 def main() -> int:
@@ -204,6 +245,12 @@ def main() -> int:
     if use_den:
         seq_den = timed(denoised, args.steps, args.warmup, 1)[0] if len(lanes) > 1 else ms_den
 
+    # the U-Net forward alone: its FLOPs over its device time (HIP events on
+    # one stream, ten forwards of the last denoised frame's features)
+    unet = None
+    if use_den:
+        unet = unet_roofline(lanes[0], H, W)
+
     # binning: algorithmic bytes = 16 B per draw (one RGBA u32 increment) over
     # the bin call's device time (HIP events on the state's stream: the
     # iterate launches and, with queue binning, the sort and tile-sum kernels)
@@ -255,6 +302,7 @@ def main() -> int:
         "iterations_per_s": it_per_s,
         "iterate_kernel_ms": kms,
         "psnr_db_vs_converged": psnr_den if den_ok else psnr_plain,
+        "roofline_denoiser": unet,
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "rfk_iterate" + (" + queue sort/bin kernels" if queue else ""),
