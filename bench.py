@@ -68,7 +68,11 @@ def unet_roofline(lane, h: int, w: int, reps: int = 10) -> dict:
     import ctypes as C
     hip = C.CDLL("libamdhip64.so")
     ev0, ev1 = C.c_void_p(), C.c_void_p()
-    check_hip = lambda e: None if e == 0 else (_ for _ in ()).throw(RuntimeError(f"hip error {e}"))
+
+    def check_hip(err: int) -> None:
+        if err != 0:
+            raise RuntimeError(f"HIP error {err} while timing the U-Net forward")
+
     check_hip(hip.hipEventCreate(C.byref(ev0)))
     check_hip(hip.hipEventCreate(C.byref(ev1)))
     lane.den.forward(lane.feat, lane.base, 1, h, w, lane.rgb)  # warm
