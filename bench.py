@@ -5,8 +5,9 @@ seeded synthetic flame). Two paths, each timed over --steps frames:
   plain     warmup (sample evaluation, precalc, walker seeding, 20 fuse
             passes), 2^30 binned iterations, density estimation (radius 9),
             log-density tonemap to half4;
-  denoised  the same pipeline at 2^25 iterations, then the U-Net (features,
-            forward) to RGB.
+  denoised  2^25 iterations, the log-density tonemap of the histogram itself
+            (no density estimation: the network, trained on such frames,
+            does the smoothing), then the U-Net (features, forward) to RGB.
 Both are scored by PSNR against a converged 2^32-iteration render of the same
 flame. `value` is the frames/s of the denoised path when it reaches 40 dB
 (else of the plain path), the metric "1080p frames/sec at PSNR>=40dB".
@@ -110,6 +111,8 @@ def main() -> int:
     ap.add_argument("--low-walkers", type=int, default=1 << 19,
                     help="walkers of the denoised path (its small budget pays less warmup with fewer)")
     ap.add_argument("--low-iters", type=int, default=1 << 25, help="budget of the denoised path")
+    ap.add_argument("--low-de-radius", type=float, default=0.0,
+                    help="density estimation radius of the denoised path (0: tonemap the histogram itself)")
     ap.add_argument("--weights", type=str, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
     ap.add_argument("--no-psnr", action="store_true", help="skip the converged reference render")
@@ -178,7 +181,7 @@ def main() -> int:
 
     lanes = [Lane() for _ in range(max(1, args.pipeline))]
 
-    def plain(lane: Lane, seed: int, kern=k, iters=ITERS, fmt=_lib.PIXEL_HALF4, walkers=args.walkers):
+    def plain(lane: Lane, seed: int, kern=k, iters=ITERS, fmt=_lib.PIXEL_HALF4, walkers=args.walkers, de_r=DE_R):
         st = lane.states.get((id(kern), walkers))
         if st is None:
             st = lane.states[(id(kern), walkers)] = kern.warmup(samples, W, H, seed=seed, fuse=20, walkers=walkers)
@@ -187,7 +190,8 @@ def main() -> int:
         # density estimation is enqueued behind the bin call; the host only
         # waits for the call's counters (the tonemap needs its quality)
         st.bin_async(iters=iters)
-        check(lib.rfk_density_estimate(st.handle, DE_R, 0.0, 0.4, lane.de.ptr))
+        if de_r > 0:
+            check(lib.rfk_density_estimate(st.handle, de_r, 0.0, 0.4, lane.de.ptr))
         res = st.bin_wait()
         out = lane.images.get(fmt)
         if out is None:
@@ -195,11 +199,14 @@ def main() -> int:
         inf = st.info()
         ta = _lib.TonemapArgs(inf.quality, gamma, bright, vib, 0.0)
         # everything of a frame on the state's stream: the next re-warm waits for it
-        check(lib.rfk_tonemap_image(lane.de.ptr, W, H, out.ptr, fmt, C.byref(ta), inf.stream))
+        if de_r > 0:
+            check(lib.rfk_tonemap_image(lane.de.ptr, W, H, out.ptr, fmt, C.byref(ta), inf.stream))
+        else:
+            check(lib.rfk_tonemap(st.handle, out.ptr, fmt, C.byref(ta)))  # the histogram itself
         return st, res, out
 
     def denoised(lane: Lane, seed: int):
-        st, res, tm = plain(lane, seed, k, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers)
+        st, res, tm = plain(lane, seed, k, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers, args.low_de_radius)
         inf = st.info()
         check(lib.rfk_denoiser_features(inf.histogram, inf.hist_bits, tm.ptr, npx, lane.feat.ptr, lane.base.ptr,
                                         inf.stream))
@@ -300,7 +307,7 @@ def main() -> int:
                             "psnr_db": psnr_plain},
                   "denoised": None if fps_den is None else
                   {"fps": fps_den, "ms": ms_den, "sequential_ms": seq_den, "iterations": args.low_iters,
-                   "walkers": args.low_walkers,
+                   "walkers": args.low_walkers, "de_radius": args.low_de_radius,
                    "psnr_db": psnr_den,
                    "speedup_vs_plain": fps_den / fps_plain, "weights": str(weights.name)}},
         "iterations_per_s": it_per_s,

@@ -4,7 +4,7 @@
 
 The shipped weights (weights/unet_1080.bin) came from
     python -m refraktor_amd.train_denoiser --flames 600 --holdout 8 --steps 60000 \
-        --batch 16 --lr 2e-3 --low 33554432 --out unet_1080.bin
+        --batch 16 --lr 2e-3 --low 33554432 --de-radius 0 --out unet_1080.bin
 
 Pairs: random seeded flames (keyframe flames of 6-10 xforms and 1080p flames
 with a final xform, alternating), the low-budget render as input features and
@@ -124,7 +124,7 @@ def main(argv=None) -> int:
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--low", type=int, default=1 << 25)
     ap.add_argument("--high", type=int, default=1 << 32)
-    ap.add_argument("--de-radius", type=float, default=9.0)
+    ap.add_argument("--de-radius", type=float, default=0.0)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--crop", type=int, default=128)
