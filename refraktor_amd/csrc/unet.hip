@@ -683,6 +683,123 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
     }
 }
 
+// The network's output layer (at most 8 channels, rgb = clamp(base + act(
+// channels 0..2 + bias))): the three row taps' weights are stacked into one
+// 32-row MFMA tile, row 8 dy + co (rows 24..31 zero), so one MFMA per
+// (dx, k step) does the work of three. The accumulator of input row r holds
+// in register group dy (elements 4 dy + j, j = co for the lanes with kh = 0)
+// its contribution to output row r + 1 - dy, so output row y is group 0 of
+// input row y - 1 plus group 1 of row y plus group 2 of row y + 1, all in
+// the same lane. Direct reads only, 32-channel chunks, NW waves per group.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void conv3x3_rgb8_kernel(const bf16* __restrict__ x, int Cin, int H, int W,
+                                                               const bf16* __restrict__ wgt,
+                                                               const float* __restrict__ bias, int Cout, int act,
+                                                               int rows, const float* __restrict__ rbase,
+                                                               float* __restrict__ rgb) {
+    constexpr int KC = 32, SPX = 34, XP = KC + PAD, G = KC / 8, NQ = (SPX * G + 63) / 64;
+    extern __shared__ __attribute__((aligned(16))) unsigned char conv_smem[];
+    const int nch = (Cin + KC - 1) / KC, CinP = nch * KC, WP = CinP + PAD, GW = CinP / 8;
+    const int t = int(threadIdx.x), wv = t >> 6, l = t & 63, m = l & 31, kh = l >> 5;
+    bf16* s_w = reinterpret_cast<bf16*>(conv_smem);  // [3 dx][32 rows 8 dy + co][WP]
+    bf16* s_x = s_w + 3 * 32 * WP + wv * SPX * XP;   // this wave's staged chunk [SPX][XP]
+    const int x0 = int(blockIdx.x) * 32 * NW + wv * 32;
+    const int y0 = int(blockIdx.y) * rows;
+    const int n = int(blockIdx.z);
+    const bf16* xn = x + size_t(n) * H * W * Cin;
+
+    for (int e = t; e < 3 * 32 * GW; e += 64 * NW) {
+        const int g = e % GW, r = (e / GW) % 32, dx = e / (32 * GW);
+        const int dy = r >> 3, co = r & 7;
+        bf16x8 v = zero8();
+        if (dy < 3 && co < Cout && g * 8 < Cin) v = load8(wgt + unsigned((co * 9 + dy * 3 + dx) * Cin + g * 8));
+        *reinterpret_cast<bf16x8*>(s_w + (dx * 32 + r) * WP + g * 8) = v;
+    }
+    auto load_chunk = [&](int jj, int ch, bf16x8* v) {
+        const int gy = y0 - 1 + jj;
+        const bool row_ok = gy >= 0 && gy < H;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const int e = l + q * 64;
+            const int g = e % G, col = e / G;
+            const int gx = x0 - 1 + col, c = ch * KC + g * 8;
+            v[q] = load_px8<0>(xn, Cin, nullptr, 0, H, W, gy, gx, c,
+                               row_ok && col < SPX && gx >= 0 && gx < W && c < Cin);
+        }
+    };
+    const float b0 = bias ? bias[0] : 0.f, b1 = bias && Cout > 1 ? bias[1] : 0.f, b2 = bias && Cout > 2 ? bias[2] : 0.f;
+    const int px = x0 + m;
+    bf16x8 nxt[NQ];
+    load_chunk(0, 0, nxt);
+    __syncthreads();  // weights staged; from here on every wave runs on its own
+
+    // group 0 of the two previous input rows, group 1 of the previous one
+    float q0a[3] = {0.f, 0.f, 0.f}, q0b[3] = {0.f, 0.f, 0.f}, q1a[3] = {0.f, 0.f, 0.f};
+    for (int jj = 0; jj < rows + 2; jj++) {
+        const int gy = y0 - 1 + jj;
+        const bool in_img = gy >= 0 && gy < H;
+        // one accumulator per dx tap: three independent MFMA chains
+        f32x16 acc[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) acc[d][i] = 0.f;
+        for (int ch = 0; ch < nch; ch++) {
+            // the wave's previous reads of s_x are issued before these writes
+            __asm__ volatile("" ::: "memory");
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int e = l + q * 64;
+                const int g = e % G, col = e / G;
+                if (col < SPX) *reinterpret_cast<bf16x8*>(s_x + col * XP + g * 8) = nxt[q];
+            }
+            __asm__ volatile("" ::: "memory");
+            {
+                int nj = jj, nc = ch + 1;
+                if (nc == nch) {
+                    nc = 0;
+                    nj++;
+                }
+                if (nj < rows + 2) load_chunk(nj, nc, nxt);
+            }
+            if (!in_img) continue;
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) {
+#pragma unroll
+                for (int ks = 0; ks < KC / 16; ks++) {
+                    const bf16x8 xb = *reinterpret_cast<const bf16x8*>(s_x + (m + dx) * XP + ks * 16 + kh * 8);
+                    const bf16x8 wa =
+                        *reinterpret_cast<const bf16x8*>(s_w + (dx * 32 + m) * WP + ch * KC + ks * 16 + kh * 8);
+                    acc[dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa, xb, acc[dx], 0, 0, 0);
+                }
+            }
+        }
+        float g0[3], g1[3], g2[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            g0[c] = (acc[0][c] + acc[1][c]) + acc[2][c];
+            g1[c] = (acc[0][4 + c] + acc[1][4 + c]) + acc[2][4 + c];
+            g2[c] = (acc[0][8 + c] + acc[1][8 + c]) + acc[2][8 + c];
+        }
+        // output row y0 + jj - 2 is complete: group 0 of input row jj - 2,
+        // group 1 of row jj - 1, group 2 of this row
+        const int yy = y0 + jj - 2;
+        if (jj >= 2 && yy < H && kh == 0 && px < W) {
+            const size_t q = ((size_t(n) * H + yy) * W + px) * 3;
+            const float v0 = q0b[0] + q1a[0] + g2[0], v1 = q0b[1] + q1a[1] + g2[1], v2 = q0b[2] + q1a[2] + g2[2];
+            rgb[q + 0] = fminf(fmaxf(rbase[q + 0] + act_fwd(v0 + b0, act, 0.f), 0.f), 1.f);
+            rgb[q + 1] = fminf(fmaxf(rbase[q + 1] + act_fwd(v1 + b1, act, 0.f), 0.f), 1.f);
+            rgb[q + 2] = fminf(fmaxf(rbase[q + 2] + act_fwd(v2 + b2, act, 0.f), 0.f), 1.f);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            q0b[c] = q0a[c];
+            q0a[c] = g0[c];
+            q1a[c] = g1[c];
+        }
+    }
+}
+
 // Weight-stationary strip: like the strip kernel, but the loop over 32-channel
 // input chunks is outside and the chunk's weights for all nine taps sit in
 // VGPRs (A operands, 18 fragments), while R output rows accumulate in AGPRs.
@@ -993,6 +1110,21 @@ hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
 hipError_t conv3x3_forward_rgb(const void* x, int cin, int N, int H, int W, const void* wgt, const float* bias,
                                int Cout, int act, const float* base, float* out_rgb, hipStream_t s) {
     if ((cin % 8) || cin <= 0 || Cout < 3 || Cout > 32 || !base || !out_rgb) return hipErrorInvalidValue;
+    if (Cout <= 8 && cin <= 96 && size_t(H) * W * cin < (size_t(1) << 31)) {
+        // the stacked-tap output kernel (32->8 + rgb at 1080p inside a denoised
+        // frame: 60 -> 53 us against the strip kernel)
+        constexpr int NW = 16;
+        const int CinP = (cin + 31) / 32 * 32;
+        const size_t lds = (size_t(3) * 32 * (CinP + PAD) + size_t(NW) * 34 * (32 + PAD)) * sizeof(bf16);
+        const void* fn = reinterpret_cast<const void*>(&conv3x3_rgb8_kernel<NW>);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess) return e;
+        const int rows = strip_rows(fn, lds, NW, N, H, W);
+        dim3 grid((W + 32 * NW - 1) / (32 * NW), (H + rows - 1) / rows, N);
+        conv3x3_rgb8_kernel<NW><<<grid, 64 * NW, lds, s>>>((const bf16*)x, cin, H, W, (const bf16*)wgt, bias, Cout, act,
+                                                          rows, base, out_rgb);
+        return hipGetLastError();
+    }
     const int KC = cin <= 16 ? 16 : 32;
     const size_t lds = strip_lds_bytes(cin, KC, strip_waves(0));
     if (strip_weight_bytes(cin, KC) > 140 * 1024 || lds > 160 * 1024 ||
