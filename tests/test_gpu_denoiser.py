@@ -84,10 +84,12 @@ def test_conv_fused_upsample_and_pool(gpu, mode, Co, c1, c2):
     np.testing.assert_allclose(yf.download(np.float32, (N, H, W, Co)), want, rtol=1e-4, atol=1e-4)
 
 
-def test_unet_inference_matches_oracle(gpu):
-    """the fused inference forward of the whole network vs the float64 network"""
-    rng = np.random.default_rng(12)
-    N, H, W = 1, 24, 40
+@pytest.mark.parametrize("N,H,W", [(1, 24, 40), (2, 16, 72), (1, 40, 536)])
+def test_unet_inference_matches_oracle(gpu, N, H, W):
+    """the fused inference forward of the whole network vs the float64 network;
+    the output layer at a batch of two and at widths that end in a partial
+    512-pixel strip (one strip of 72, a full strip and one of 24)"""
+    rng = np.random.default_rng(12 + N + H + W)
     den = denoise.Denoiser(seed=5)
     w, b = den.params()
     b = (rng.normal(size=b.shape) * 0.01).astype(np.float32)
