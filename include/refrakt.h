@@ -372,9 +372,14 @@ int rfk_nn_conv3x3_wgrad(const void* x1, int c1, const void* x2, int c2, const v
 int rfk_nn_conv3x3_dgrad(const void* dy, int Cout, int N, int H, int W, const void* wgt, int Cin, void* dx,
                          void* stream);
 /* training crops: n crops of ch x cw pixels starting at (image, y, x) triples
- * (host array) from a device stack of H x W images of pixel_bytes each      */
-int rfk_nn_gather_crops(const void* src, int H, int W, int pixel_bytes, const int* offsets, int n, int ch, int cw,
-                        void* dst, void* stream);
+ * (host array) from a device stack of n_images H x W images of pixel_bytes
+ * each; an image index outside [0, n_images) or a crop outside the image is
+ * refused before anything is launched                                       */
+int rfk_nn_gather_crops(const void* src, int n_images, int H, int W, int pixel_bytes, const int* offsets, int n,
+                        int ch, int cw, void* dst, void* stream);
+/* error-path tests: the next `count` activation allocations of any denoiser
+ * fail as hipMalloc failing would (RFK_ERR_HIP, nothing launched)           */
+int rfk_nn_inject_alloc_failures(int count);
 
 /* Microbenchmark of uniform-random RGBA u32 atomic increments into a buffer of
  * `footprint_bytes`; returns increments per second (the binning roofline).  */

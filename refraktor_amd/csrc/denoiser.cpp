@@ -7,6 +7,7 @@
 // Adam), so a frame renders with a fraction of the iterations.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -164,15 +165,29 @@ void init_weights(rfk_denoiser* d, uint64_t seed, std::vector<float>& w, std::ve
     }
 }
 
+// Allocations that the next act_buf calls refuse without calling hipMalloc
+// (rfk_nn_inject_alloc_failures: error-path tests).
+std::atomic<int> g_inject_alloc_failures{0};
+
+// The activation buffer `name` for shape (N, H, W), grown to `bytes`. On
+// failure rc is set (and never cleared here: callers check once after a run
+// of calls), the returned buffer is null and its recorded size stays 0 so the
+// next call tries again.
 buf& act_buf(rfk_denoiser* d, int N, int H, int W, const std::string& name, size_t bytes, int& rc) {
     auto& m = d->acts[{N, H, W}];
     buf& b = m[name];
-    rc = RFK_OK;
-    if (b.bytes < bytes) {
+    if (b.bytes < bytes || !b.p) {
         if (b.p) (void)hipFree(b.p);
         b.p = nullptr;
-        if (hipMalloc(&b.p, bytes + 256) != hipSuccess) rc = fail(RFK_ERR_HIP, "activation allocation failed");
-        b.bytes = bytes;
+        b.bytes = 0;
+        int left = g_inject_alloc_failures.load();
+        while (left > 0 && !g_inject_alloc_failures.compare_exchange_weak(left, left - 1)) {}
+        if (left > 0 || hipMalloc(&b.p, bytes + 256) != hipSuccess) {
+            b.p = nullptr;
+            rc = fail(RFK_ERR_HIP, "activation allocation failed");
+        } else {
+            b.bytes = bytes;
+        }
     }
     return b;
 }
@@ -233,11 +248,15 @@ int forward(rfk_denoiser* d, int N, int H, int W, const void* feat, float* rout,
         if ((rc = conv(13, c0, nullptr, 0, d0, nullptr))) return rc;
         if (out_rgb) {
             const conv_def& c = d->convs[14];
-            if (rfk::nn::conv3x3_forward_rgb(d0, c.c1, N, H, W, static_cast<char*>(d->wbf) + c.w_off * 2,
-                                             d->b + c.b_off, c.cout, c.act, base, out_rgb, s) == hipSuccess) {
+            const hipError_t e = rfk::nn::conv3x3_forward_rgb(d0, c.c1, N, H, W, static_cast<char*>(d->wbf) + c.w_off * 2,
+                                                              d->b + c.b_off, c.cout, c.act, base, out_rgb, s);
+            if (e == hipSuccess) {
                 if (rgb_done) *rgb_done = true;
                 return RFK_OK;
             }
+            // only an unsupported shape (refused before any launch) falls back
+            // to the plain output layer; every other error is the caller's
+            if (e != hipErrorInvalidValue) return fail(RFK_ERR_HIP, std::string("output layer: ") + hipGetErrorString(e));
             (void)hipGetLastError();
         }
         return conv(14, d0, nullptr, 0, nullptr, rout);
@@ -567,11 +586,12 @@ int rfk_nn_conv3x3_dgrad(const void* dy, int Cout, int N, int H, int W, const vo
     return RFK_OK;
 }
 
-int rfk_nn_gather_crops(const void* src, int H, int W, int pixel_bytes, const int* offsets /* host, 3 per crop */,
-                        int n, int ch, int cw, void* dst, void* stream) {
-    if (!src || !offsets || !dst || n <= 0 || pixel_bytes <= 0) return fail(RFK_ERR_INVALID, "bad argument");
+int rfk_nn_gather_crops(const void* src, int n_images, int H, int W, int pixel_bytes,
+                        const int* offsets /* host, 3 per crop */, int n, int ch, int cw, void* dst, void* stream) {
+    if (!src || !offsets || !dst || n <= 0 || pixel_bytes <= 0 || n_images <= 0)
+        return fail(RFK_ERR_INVALID, "bad argument");
     for (int k = 0; k < n; k++)
-        if (offsets[3 * k + 1] < 0 || offsets[3 * k + 2] < 0 || offsets[3 * k + 1] + ch > H || offsets[3 * k + 2] + cw > W)
+        if (offsets[3 * k] < 0 || offsets[3 * k] >= n_images || offsets[3 * k + 1] < 0 || offsets[3 * k + 2] < 0 || offsets[3 * k + 1] + ch > H || offsets[3 * k + 2] + cw > W)
             return fail(RFK_ERR_INVALID, "crop outside the image");
     hipStream_t s = static_cast<hipStream_t>(stream);
     int* d_off = nullptr;
@@ -581,6 +601,11 @@ int rfk_nn_gather_crops(const void* src, int H, int W, int pixel_bytes, const in
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(d_off);
     if (e != hipSuccess) return fail(RFK_ERR_HIP, hipGetErrorString(e));
+    return RFK_OK;
+}
+
+int rfk_nn_inject_alloc_failures(int count) {
+    g_inject_alloc_failures.store(count < 0 ? 0 : count);
     return RFK_OK;
 }
 

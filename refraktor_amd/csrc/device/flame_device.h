@@ -7,8 +7,10 @@
 // Semantics kept from librefrakt's device library
 // (assets/kernels/include/refrakt/flamelib.h:4-26, 326-490 and random.h:6-30):
 // eps(x) = x == 0 ? 1e-20 : x; sincos -> (sin, cos); cossin -> (cos, sin);
-// rand01 = u32 / 4294967295 in float; randbit = bit 24; randgauss draws the
-// angle first, then the sum of four uniforms.
+// rand01 = u32 / 4294967295 in the build's precision (in float the divisor
+// rounds to 2^32, so the f32 form is the exact product u32 * 2^-32);
+// randbit = bit 24; randgauss draws the angle first, then the sum of four
+// uniforms.
 #pragma once
 
 #ifndef RFK_DEVICE_H
@@ -186,7 +188,11 @@ RFK_INLINE u32x4 philox4x32_10(u32x4 c, u32 k0, u32 k1) {
 #define RFK_TAG_INIT 4u    // per walker thread: initial colour
 #define RFK_TAG_JITTER 5u  // per (sample, k): anti-alias jitter
 
-RFK_INLINE Real rfk_u01(u32 u) { return Real(float(u) * 2.3283064365386963e-10f); }
+#ifdef RFK_DOUBLE
+RFK_INLINE Real rfk_u01(u32 u) { return double(u) / 4294967295.0; }
+#else
+RFK_INLINE Real rfk_u01(u32 u) { return float(u) * 2.3283064365386963e-10f; }
+#endif
 
 // Per-walker draw stream for one pass: word j of the pass is word j%4 of
 // philox({pass, gid, j/4, tag}, key), generated lazily four at a time.

@@ -206,11 +206,11 @@ def train(den: Denoiser, data: Dataset, steps: int, batch: int = 8, crop: int = 
     for step in range(steps):
         offs = _crop_offsets(rng, data, batch, crop, importance)
         op = offs.ctypes.data
-        check(lib.rfk_nn_gather_crops(data.feat.ptr, data.height, data.width, FEAT_C * 2, op, batch, crop, crop,
+        check(lib.rfk_nn_gather_crops(data.feat.ptr, data.n, data.height, data.width, FEAT_C * 2, op, batch, crop, crop,
                                       bf.ptr, None))
-        check(lib.rfk_nn_gather_crops(data.base.ptr, data.height, data.width, 12, op, batch, crop, crop, bb.ptr,
+        check(lib.rfk_nn_gather_crops(data.base.ptr, data.n, data.height, data.width, 12, op, batch, crop, crop, bb.ptr,
                                       None))
-        check(lib.rfk_nn_gather_crops(data.target.ptr, data.height, data.width, 12, op, batch, crop, crop, bt.ptr,
+        check(lib.rfk_nn_gather_crops(data.target.ptr, data.n, data.height, data.width, 12, op, batch, crop, crop, bt.ptr,
                                       None))
         cur_lr = lr * (0.5 * (1 + np.cos(np.pi * step / max(steps, 1))))
         losses.append(den.train_step(bf, bb, bt, batch, crop, crop, float(cur_lr)))
