@@ -59,13 +59,17 @@ typedef struct {
     u32 k0, k1, c0, c1, c2, tag;
     u32 buf[4];
     int avail, pos;
+    const u32* list; /* when set: the words are list[0..nlist) in order */
+    int nlist, lpos;
 } lane_rng;
 
 static inline void rng_init(lane_rng* r, u32 k0, u32 k1, u32 c1, u32 tag) {
     r->k0 = k0; r->k1 = k1; r->c0 = 0; r->c1 = c1; r->c2 = 0; r->tag = tag; r->avail = 0; r->pos = 0;
+    r->list = NULL; r->nlist = 0; r->lpos = 0;
 }
 static inline void rng_begin(lane_rng* r, u32 pass) { r->c0 = pass; r->c2 = 0; r->avail = 0; r->pos = 0; }
 static inline u32 rng_next(lane_rng* r) {
+    if (r->list) return r->lpos < r->nlist ? r->list[r->lpos++] : 0u;
     if (r->avail == 0) {
         u32 ctr[4] = {r->c0, r->c1, r->c2, r->tag}, key[2] = {r->k0, r->k1};
         orc_philox(ctr, key, r->buf);
@@ -239,10 +243,13 @@ static void apply_variation(int id, double w, const double* prm, const double* a
         case V_CROSS: { double s = px * px - py * py; double k = w / sqrt(eps_(s * s)); *rx += k * px; *ry += k * py; } break;
         case V_EXP: { double e = w * exp(px); *rx += e * cos(py); *ry += e * sin(py); } break;
         case V_LOG: *rx += 0.5 * w * log(sumsq); *ry += w * phi; break;
-        case V_SINH: *rx += w * sinh(px) * cos(py); *ry += w * cosh(px) * sin(py); break;
+        /* sinh and cos as librefrakt's programs (config/variations_new.yml:880-885,
+         * 841-846): sinh takes the hyperbolic factors of y, cos adds the
+         * imaginary part; flam3 publishes sinh(x), cosh(x) and a minus */
+        case V_SINH: *rx += w * sinh(py) * cos(py); *ry += w * cosh(py) * sin(py); break;
         case V_COSH: *rx += w * cosh(px) * cos(py); *ry += w * sinh(px) * sin(py); break;
         case V_SIN: *rx += w * sin(px) * cosh(py); *ry += w * cos(px) * sinh(py); break;
-        case V_COS: *rx += w * cos(px) * cosh(py); *ry -= w * sin(px) * sinh(py); break;
+        case V_COS: *rx += w * cos(px) * cosh(py); *ry += w * sin(px) * sinh(py); break;
         case V_TANGENT: *rx += w * sin(px) / cos(py); *ry += w * tan(py); break;
         case V_SQUARE: { double a = rand01(rs), b = rand01(rs); *rx += w * (a - 0.5); *ry += w * (b - 0.5); } break;
         case V_POLAR2: *rx += w / M_PI * atn; *ry += 0.5 * w / M_PI * log(sumsq); break;
@@ -291,6 +298,23 @@ static void apply_variation(int id, double w, const double* prm, const double* a
             *ry += w * py * amp;
         } break;
         default: break;
+    }
+}
+
+/* One variation at n points p (after the vlink affine): prm in the sorted
+ * order of all the variation's parameter names, aff as (a, d, b, e, c, f),
+ * words: nwords random words per point, consumed in order. */
+void orc_eval_variation(int id, double w, const double* prm, const double* aff, size_t n, const double* px,
+                        const double* py, const u32* words, int nwords, double* rx, double* ry) {
+    for (size_t i = 0; i < n; i++) {
+        lane_rng rs;
+        rng_init(&rs, 0u, 0u, 0u, 0u);
+        rs.list = words + i * (size_t)nwords;
+        rs.nlist = nwords;
+        double x = 0.0, y = 0.0;
+        apply_variation(id, w, prm, aff, px[i], py[i], &x, &y, &rs);
+        rx[i] = x;
+        ry[i] = y;
     }
 }
 

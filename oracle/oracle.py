@@ -441,6 +441,7 @@ def clib():
         lib.orc_render.argtypes = [P, C.c_int, C.c_int, C.c_int, P, C.c_uint32, P, C.c_uint32, C.c_uint32,
                                    C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P, C.c_int]
         lib.orc_render.restype = C.c_uint64
+        lib.orc_eval_variation.argtypes = [C.c_int, C.c_double, P, P, C.c_size_t, P, P, P, C.c_int, P, P]
         lib.orc_jpeg_encode.argtypes = [P, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_size_t]
         lib.orc_jpeg_encode.restype = C.c_long
         _clib = lib
@@ -457,6 +458,21 @@ def c_philox(ctr, key) -> np.ndarray:
     out = np.zeros(4, dtype=np.uint32)
     clib().orc_philox(_ptr(ctr), _ptr(key), _ptr(out))
     return out
+
+
+def eval_variation(name: str, weight: float, params: dict, aff: dict, px, py, words) -> tuple:
+    """the C oracle's hand-written formula of one variation at points p
+    (after the vlink affine); params holds every slot, user and precalc;
+    words: per-point random words consumed in order"""
+    prm = np.array([params[k] for k in sorted(params)] or [0.0], dtype=np.float64)
+    a = np.array([aff[k] for k in "adbecf"], dtype=np.float64)
+    px = np.ascontiguousarray(px, dtype=np.float64)
+    py = np.ascontiguousarray(py, dtype=np.float64)
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    rx, ry = np.zeros_like(px), np.zeros_like(px)
+    clib().orc_eval_variation(VARIATIONS.index(name), float(weight), _ptr(prm), _ptr(a), px.size, _ptr(px), _ptr(py),
+                              _ptr(w), w.shape[1], _ptr(rx), _ptr(ry))
+    return rx, ry
 
 
 def bin_points(screen6, palette, pts, width, height) -> np.ndarray:
