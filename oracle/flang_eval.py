@@ -237,24 +237,33 @@ def parse(src: str):
 @dataclass
 class Val:
     kind: str          # 'f' float, 'i' int, 'b' bool, 'v' vec2
-    x: np.ndarray      # float64 (ints and bools held exactly as floats/bools)
+    x: np.ndarray      # the float type (ints and bools held exactly as floats/bools)
     y: np.ndarray | None = None
 
 
+# the float type of the evaluation in progress: float64 normally, float32 to
+# measure how a program behaves in single precision (evaluate(dtype=...))
+_FT = [np.float64]
+
+
 def _f(a) -> Val:
-    return Val("f", np.asarray(a, dtype=np.float64))
+    return Val("f", np.asarray(a, dtype=_FT[0]))
 
 
 def _v(x, y) -> Val:
-    return Val("v", np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64))
+    return Val("v", np.asarray(x, dtype=_FT[0]), np.asarray(y, dtype=_FT[0]))
 
 
 class Random:
-    """per-lane word streams: words[lane, k] is lane's k-th 32-bit word"""
+    """per-lane word streams: words[lane, k] is lane's k-th 32-bit word.
+    rand01 is u / 4294967295 in the float type of the kernel being checked:
+    in float the divisor rounds to 2^32 (the default), in double it does not
+    (double_divide=True, random.h:17-18)."""
 
-    def __init__(self, words: np.ndarray):
+    def __init__(self, words: np.ndarray, double_divide: bool = False):
         self.words = np.asarray(words, dtype=np.uint64)
         self.cur = np.zeros(self.words.shape[0], dtype=np.int64)
+        self.double_divide = double_divide
 
     def next(self, mask: np.ndarray) -> np.ndarray:
         n = self.words.shape[0]
@@ -266,11 +275,14 @@ class Random:
         return w
 
     def rand01(self, mask):
+        u = self.next(mask)
+        if self.double_divide:
+            return (u.astype(np.float64) / 4294967295.0).astype(_FT[0])
         # float(u) rounded to float32, times 2^-32 (exact): random.h:16-22
-        return self.next(mask).astype(np.float32).astype(np.float64) * 2.0 ** -32
+        return (u.astype(np.float32).astype(np.float64) * 2.0 ** -32).astype(_FT[0])
 
     def randbit(self, mask):
-        return ((self.next(mask) >> 24) & 1).astype(np.float64)
+        return ((self.next(mask) >> 24) & 1).astype(_FT[0])
 
     def randgauss(self, mask):
         a = self.rand01(mask) * 2.0
@@ -312,6 +324,37 @@ class Context:
     commons: dict = field(default_factory=dict)   # name -> AST
     rng: Random | None = None
     precalc: bool = False
+    mca: "Perturb | None" = None
+
+
+@dataclass
+class Perturb:
+    """Monte Carlo arithmetic (D. S. Parker, "Monte Carlo Arithmetic",
+    UCLA CSD-970002, 1997): every float result of an operator is multiplied
+    by (1 + op_rel * u) and every library function's result by
+    (1 + fn_rel * u) plus fn_abs * u for the periodic functions, with u
+    uniform in [-1, 1] per lane and operation. A float64 evaluation so
+    perturbed behaves like one in a lower precision with less exact
+    functions; the spread of several such runs bounds how far a float32
+    kernel with hardware transcendentals may land from the exact result."""
+    rng: np.random.Generator
+    op_rel: float = 2.0 ** -24
+    fn_rel: float = 2.0 ** -21
+    fn_abs: float = 2.0 ** -21
+
+    def op(self, x):
+        x = np.asarray(x)
+        return x * (1.0 + self.op_rel * self.rng.uniform(-1.0, 1.0, x.shape))
+
+    def fn(self, x, periodic: bool):
+        x = np.asarray(x)
+        y = x * (1.0 + self.fn_rel * self.rng.uniform(-1.0, 1.0, x.shape))
+        if periodic:
+            y = y + self.fn_abs * self.rng.uniform(-1.0, 1.0, x.shape)
+        return y
+
+
+_PERIODIC = {"sin", "cos", "tan", "sinpi", "cospi", "tanpi", "sincos", "cossin", "sincospi", "cossinpi"}
 
 
 class Interp:
@@ -322,6 +365,7 @@ class Interp:
         n = ctx.n
         self.alive = np.ones(n, dtype=bool)
         self.result = _v(np.zeros(n), np.zeros(n))
+        self.ft = _FT[0]
 
     # ---- statements
     def run(self, ast, mask=None):
@@ -410,8 +454,8 @@ class Interp:
 
     def merge(self, old: Val, new: Val, m) -> Val:
         if new.kind == "v":
-            ox = old.x if old.kind == "v" else np.zeros(self.c.n)
-            oy = old.y if old.kind == "v" else np.zeros(self.c.n)
+            ox = old.x if old.kind == "v" else np.zeros(self.c.n, self.ft)
+            oy = old.y if old.kind == "v" else np.zeros(self.c.n, self.ft)
             return Val("v", np.where(m, new.x, ox), np.where(m, new.y, oy))
         return Val(new.kind, np.where(m, new.x, old.x))
 
@@ -426,9 +470,9 @@ class Interp:
     def expr(self, e, m) -> Val:
         k = e[0]
         if k == "dec":
-            return Val("f", np.float64(e[1]))
+            return Val("f", self.ft(e[1]))
         if k == "int":
-            return Val("i", np.float64(e[1]))
+            return Val("i", self.ft(e[1]))
         if k == "bool":
             return Val("b", np.bool_(e[1]))
         if k == "var":
@@ -468,7 +512,7 @@ class Interp:
         if name == "result":
             return self.result
         if name == "weight":
-            return Val("f", np.float64(self.c.weight))
+            return Val("f", self.ft(self.c.weight))
         sc = self._lookup_scope(name)
         if sc is None:
             raise NameError(name)
@@ -476,11 +520,11 @@ class Interp:
 
     def member(self, grp, mem, m) -> Val:
         if grp == "param":
-            return Val("f", np.asarray(self.c.params[mem], dtype=np.float64))
+            return Val("f", np.asarray(self.c.params[mem], dtype=self.ft))
         if grp == "aff":
-            return Val("f", np.float64(self.c.aff[mem]))
+            return Val("f", self.ft(self.c.aff[mem]))
         if grp == "math":
-            return Val("f", np.float64({"pi": math.pi, "inv_pi": 1.0 / math.pi}[mem]))
+            return Val("f", self.ft({"pi": math.pi, "inv_pi": 1.0 / math.pi}[mem]))
         if grp == "common":
             if mem not in self.common_cache:
                 self.common_cache[mem] = self.expr(self.c.commons[mem], m)
@@ -527,7 +571,9 @@ class Interp:
             r = np.power(a.x, b.x)
         else:
             raise ValueError(op)
-        return Val("i" if both_int else "f", np.asarray(r, dtype=np.float64))
+        if self.c.mca is not None and not both_int:
+            r = self.c.mca.op(r)
+        return Val("i" if both_int else "f", np.asarray(r, dtype=self.ft))
 
     def call(self, name, args, m) -> Val:
         rng = self.c.rng
@@ -549,17 +595,23 @@ class Interp:
             return Val("i", f(vals[0].x))
         if name == "is_even":
             return Val("b", np.fmod(vals[0].x, 2.0) == 0.0)
+        mca = self.c.mca
+        per = name in _PERIODIC
+
+        def fz(x):
+            return x if mca is None or name in ("abs", "floor", "eps", "copysign") else mca.fn(x, per)
+
         if name in _DUAL:
             s, c = _DUAL[name](vals[0].x)
-            return _v(s, c)
+            return _v(fz(s), fz(c))
         if name in _BINARY:
-            return _f(_BINARY[name](vals[0].x, vals[1].x))
+            return _f(fz(_BINARY[name](vals[0].x, vals[1].x)))
         if name in _UNARY:
             f = _UNARY[name]
             v = vals[0]
             if v.kind == "v":
-                return _v(f(v.x), f(v.y))
-            return _f(f(v.x))
+                return _v(fz(f(v.x)), fz(f(v.y)))
+            return _f(fz(f(v.x)))
         raise NameError(f"unknown function {name}")
 
 
@@ -620,23 +672,31 @@ def precalc(v: Variation, user: dict, aff: dict) -> dict:
 
 
 def evaluate(v: Variation, commons: dict, px, py, weight: float, params: dict, aff: dict,
-             words: np.ndarray | None = None, rng: Random | None = None, with_precalc: bool = True):
+             words: np.ndarray | None = None, rng: Random | None = None, with_precalc: bool = True,
+             dtype=np.float64, mca: Perturb | None = None):
     """result (x, y) of one variation at points p (after the vlink affine);
     `params` holds the user parameters (precalc runs here unless
-    with_precalc is False, in which case it must hold every slot)"""
-    px = np.asarray(px, dtype=np.float64)
-    py = np.asarray(py, dtype=np.float64)
-    n = px.shape[0]
+    with_precalc is False, in which case it must hold every slot). With
+    dtype=np.float32 every value and operation is single precision (numpy's
+    float32 functions), which measures the program's own float32 error;
+    with `mca` every operation is perturbed (Perturb)."""
+    n = np.asarray(px).shape[0]
     prm = precalc(v, params, aff) if with_precalc else dict(params)
     if rng is None:
         rng = Random(words if words is not None else np.zeros((n, 0), np.uint32))
-    ctx = Context(n=n, p=_v(px, py), weight=float(weight), params={k: np.float64(x) for k, x in prm.items()},
-                  aff=aff, commons=commons, rng=rng)
-    it = Interp(ctx)
-    with np.errstate(all="ignore"):
-        it.run(v.ast)
-    r = it.result
-    return np.broadcast_to(r.x, (n,)).copy(), np.broadcast_to(r.y, (n,)).copy()
+    _FT[0] = dtype
+    try:
+        px = np.asarray(px, dtype=dtype)
+        py = np.asarray(py, dtype=dtype)
+        ctx = Context(n=n, p=_v(px, py), weight=float(weight), params={k: dtype(x) for k, x in prm.items()},
+                      aff=aff, commons=commons, rng=rng, mca=mca)
+        it = Interp(ctx)
+        with np.errstate(all="ignore"):
+            it.run(v.ast)
+        r = it.result
+        return (np.broadcast_to(r.x, (n,)).astype(np.float64), np.broadcast_to(r.y, (n,)).astype(np.float64))
+    finally:
+        _FT[0] = np.float64
 
 
 def uses_random(v: Variation) -> bool:
