@@ -194,6 +194,13 @@ typedef struct rfk_batch_frame_info {
     uint64_t total_draws, total_passes, bad_values;
     uint32_t passes_per_walker, walkers;
     uint32_t hist_bits;       /* 32 or 64                                     */
+    /* what renders the frame: its temporal samples' parameter rows (device,
+     * `blocks` rows of param_stride reals of the kernel's precision, after
+     * precalc) and palettes (device, blocks x 256 RGBA8); its walkers run the
+     * streams (seed, stream_id) with `fuse` passes before binning          */
+    const void* params;
+    const uint32_t* palettes;
+    uint32_t param_stride, blocks, seed, stream_id, fuse;
 } rfk_batch_frame_info;
 int rfk_batch_create(rfk_kernel* k, void* stream, const double* samples, size_t n_reals, uint32_t frames,
                      const uint64_t* iters_per_frame /* host */, uint32_t width, uint32_t height,

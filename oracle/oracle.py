@@ -393,29 +393,33 @@ def de_levels(hist: np.ndarray, max_r: float, min_r: float, curve: float, max_le
 
 def density_estimate(hist: np.ndarray, max_r: float, min_r: float, curve: float) -> np.ndarray:
     """brute-force scatter: every non-empty bin spreads its RGBA (draw units)
-    with exp(-2 (r/h)^2), r <= h, normalised to sum 1 over the disc"""
+    with exp(-2 (r/h)^2), r <= h, normalised to sum 1 over the disc
+    (h = 0: the bin keeps its value). Summed offset by offset: for each
+    window offset, every source adds its level's normalised weight there."""
     H, W = hist.shape[:2]
     hmax = max(max_r, min_r)
     nlev = int(hmax * 16.0 + 0.5) + 1
     R = (nlev - 1 + 15) // 16
     lv = de_levels(hist, max_r, min_r, curve, nlev - 1)
     col = hist.astype(np.float64) / 255.0
-    out = np.zeros((H, W, 4))
     dy, dx = np.mgrid[-R:R + 1, -R:R + 1]
-    d2 = (dx * dx + dy * dy).astype(np.float64)
-    for y in range(H):
-        for x in range(W):
-            l = lv[y, x]
-            if l < 0:
-                continue
-            h = l / 16.0
-            m = d2 <= h * h
-            k = np.where(m, np.where(d2 == 0, 1.0, np.exp(-2.0 * d2 / max(h * h, 1e-300))), 0.0)
-            k /= k.sum()
-            for (yy, xx), wgt in zip(zip((dy + y).ravel(), (dx + x).ravel()), k.ravel()):
-                if wgt and 0 <= yy < H and 0 <= xx < W:
-                    out[yy, xx] += wgt * col[y, x]
-    return out
+    d2 = (dx * dx + dy * dy).astype(np.float64).ravel()
+    h2 = (np.arange(nlev) / 16.0) ** 2
+    # kernel[level, offset], each row normalised over its disc
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ker = np.where(d2[None, :] <= h2[:, None],
+                       np.where(d2[None, :] == 0, 1.0, np.exp(-2.0 * d2[None, :] / np.maximum(h2[:, None], 1e-300))),
+                       0.0)
+    ker /= ker.sum(axis=1, keepdims=True)
+    src = lv >= 0
+    lvs = np.where(src, lv, 0)
+    pad = np.zeros((H + 2 * R, W + 2 * R, 4))
+    for j, (oy, ox) in enumerate(zip(dy.ravel(), dx.ravel())):
+        wgt = np.where(src, ker[lvs, j], 0.0)
+        if not wgt.any():
+            continue
+        pad[R + oy:R + oy + H, R + ox:R + ox + W] += wgt[..., None] * col
+    return pad[R:R + H, R:R + W]
 
 
 # ------------------------------------------------------------ C oracle

@@ -56,7 +56,9 @@ class StateInfo(C.Structure):
 class BatchFrameInfo(C.Structure):
     _fields_ = [("histogram", C.c_void_p), ("quality", C.c_double), ("total_draws", C.c_uint64),
                 ("total_passes", C.c_uint64), ("bad_values", C.c_uint64), ("passes_per_walker", C.c_uint32),
-                ("walkers", C.c_uint32), ("hist_bits", C.c_uint32)]
+                ("walkers", C.c_uint32), ("hist_bits", C.c_uint32), ("params", C.c_void_p),
+                ("palettes", C.c_void_p), ("param_stride", C.c_uint32), ("blocks", C.c_uint32), ("seed", C.c_uint32),
+                ("stream_id", C.c_uint32), ("fuse", C.c_uint32)]
 
 
 class TuneCandidate(C.Structure):

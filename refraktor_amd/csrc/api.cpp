@@ -1216,6 +1216,15 @@ int rfk_batch_frame(const rfk_batch* B, uint32_t frame, rfk_batch_frame_info* ou
     out->passes_per_walker = B->frame_passes[frame];
     out->walkers = B->blocks_per_frame * B->tpb;
     out->hist_bits = B->hist64 ? 64u : 32u;
+    const size_t elem = B->k->opt.precision ? sizeof(double) : sizeof(float);
+    const size_t row0 = size_t(frame) * B->blocks_per_frame;
+    out->params = static_cast<const char*>(B->params) + row0 * B->param_stride * elem;
+    out->palettes = B->palettes + row0 * 256;
+    out->param_stride = B->param_stride;
+    out->blocks = B->blocks_per_frame;
+    out->seed = B->seed;
+    out->stream_id = frame;
+    out->fuse = B->fuse;
     return RFK_OK;
 }
 

@@ -367,6 +367,16 @@ class Batch:
         dt = np.uint64 if inf["hist_bits"] == 64 else np.uint32
         return download(inf["histogram"], dt, (self.height, self.width, 4))
 
+    def params(self, f: int) -> np.ndarray:
+        """frame f's parameter rows after precalc, one per temporal sample"""
+        inf = self.frame(f)
+        dt = np.float64 if self.kernel.precision == 1 else np.float32
+        return download(inf["params"], dt, (inf["blocks"], inf["param_stride"]))
+
+    def palettes(self, f: int) -> np.ndarray:
+        inf = self.frame(f)
+        return download(inf["palettes"], np.uint32, (inf["blocks"], 256))
+
     def __del__(self):
         try:
             if self.handle:

@@ -317,7 +317,8 @@ def test_density_estimate_matches_oracle(small):
     st = k.warmup(k.pack_samples(96, 64), 96, 64, seed=3, fuse=10, walkers=4096)
     st.bin(iters=4096 * 16)
     h = st.histogram()
-    for max_r, min_r, curve in ((0.0, 0.0, 0.4), (3.0, 0.0, 0.4), (6.0, 1.0, 0.6)):
+    # radii of the 1080p (0-9) and 4K (0-12) configurations included
+    for max_r, min_r, curve in ((0.0, 0.0, 0.4), (3.0, 0.0, 0.4), (6.0, 1.0, 0.6), (9.0, 0.0, 0.4), (12.0, 0.5, 0.4)):
         got = st.density_estimate(max_r, min_r, curve).astype(np.float64)
         want = oracle.density_estimate(h, max_r, min_r, curve)
         np.testing.assert_allclose(got, want, rtol=2e-4, atol=1e-4 * want.max())
