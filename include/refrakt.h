@@ -363,6 +363,17 @@ int rfk_denoiser_features(const uint32_t* hist, int hist_bits, const float* tone
 /* out_rgb = clamp(base + U-Net(feat), 0, 1) */
 int rfk_denoiser_forward(rfk_denoiser* d, const void* feat, const float* base, int N, int H, int W, float* out_rgb,
                          void* stream);
+/* rfkt::denoiser::denoise (interface/denoiser.h:35-38): in = a tonemapped
+ * image (RFK_PIXEL_FLOAT4 / HALF4 / FLOAT3 / HALF3, device, width x height,
+ * any size), out = the denoised image in out_format (same set, device, must
+ * not alias in), enqueued on `stream` (the reference returns a future; wait
+ * on the stream or record an event). The network's density features come
+ * from the tonemap's alpha, sat(log2(w) / 24) (tonemap.cu:95-98): a 4-channel
+ * input gives the denoiser everything the histogram path gives it; a
+ * 3-channel input has no alpha and is denoised from colour and coverage
+ * alone. Output alpha: the input's (1 for a 3-channel input).             */
+int rfk_denoise(rfk_denoiser* d, const void* in, int in_format, void* out, int out_format, uint32_t width,
+                uint32_t height, void* stream);
 /* one Adam step on mean squared error vs target; *loss = the batch MSE */
 int rfk_denoiser_train_step(rfk_denoiser* d, const void* feat, const float* base, const float* target, int N, int H,
                             int W, float lr, float* loss, void* stream);

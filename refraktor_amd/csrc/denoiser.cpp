@@ -514,6 +514,41 @@ int rfk_denoiser_forward(rfk_denoiser* d, const void* feat, const float* base, i
     return RFK_OK;
 }
 
+// The reference's denoise call (interface/denoiser.h:35-38): a tonemapped
+// image in, the denoised image out, on the caller's stream. The features come
+// from the image itself (the alpha of a 4-channel image carries the density);
+// the image is padded to multiples of 8 for the network and cropped back.
+int rfk_denoise(rfk_denoiser* d, const void* in, int in_format, void* out, int out_format, uint32_t width,
+                uint32_t height, void* stream) {
+    if (!d || !in || !out) return fail(RFK_ERR_INVALID, "null argument");
+    auto fmt_ok = [](int f) {
+        return f == RFK_PIXEL_FLOAT4 || f == RFK_PIXEL_HALF4 || f == RFK_PIXEL_FLOAT3 || f == RFK_PIXEL_HALF3;
+    };
+    if (!fmt_ok(in_format) || !fmt_ok(out_format))
+        return fail(RFK_ERR_INVALID, "formats must be float4, half4, float3 or half3");
+    if (width == 0 || height == 0 || width > 32768 || height > 32768) return fail(RFK_ERR_INVALID, "bad image size");
+    if (in == out) return fail(RFK_ERR_INVALID, "in and out must not alias");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int W = int(width), H = int(height), Wp = (W + 7) / 8 * 8, Hp = (H + 7) / 8 * 8;
+    const size_t npix = size_t(Wp) * Hp;
+    int rc = 0;
+    void* feat = act_buf(d, 1, Hp, Wp, "dn_feat", npix * kFeat * 2, rc).p;
+    float* base = static_cast<float*>(act_buf(d, 1, Hp, Wp, "dn_base", npix * 3 * 4, rc).p);
+    float* rgb = static_cast<float*>(act_buf(d, 1, Hp, Wp, "dn_rgb", npix * 3 * 4, rc).p);
+    float* r = static_cast<float*>(act_buf(d, 1, Hp, Wp, "r", npix * kOut * 4, rc).p);
+    if (rc) return rc;
+    // formats share their numbering with the kernels: 0 float4, 1 half4, 2 float3, 3 half3
+    NN_HIP(rfk::nn::image_features(in, in_format, W, H, Wp, Hp, kFeat, feat, base, s));
+    bool done = false;
+    if ((rc = forward(d, 1, Hp, Wp, feat, r, s, true, base, rgb, &done))) return rc;
+    if (!done) {
+        NN_HIP(rfk::nn::residual_out(base, 3, r, kOut, npix, rgb, s));
+        NN_HIP(rfk::nn::clamp_rgb(rgb, npix * 3, rgb, s));
+    }
+    NN_HIP(rfk::nn::image_output(rgb, Wp, in, in_format, out, out_format, W, H, s));
+    return RFK_OK;
+}
+
 // One training step on a batch: MSE between base + U-Net(feat) and target
 // (FP32 RGB), backward through every layer, Adam update. Returns the loss.
 int rfk_denoiser_train_step(rfk_denoiser* d, const void* feat, const float* base, const float* target, int N, int H,

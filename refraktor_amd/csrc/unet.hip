@@ -4,6 +4,7 @@
 // gradients, loss, Adam, feature preparation). The librefrakt denoisers are
 // third-party binaries (OIDN / OptiX); this network has no reference
 // counterpart and is checked against the oracle's float64 convolution.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -454,6 +455,73 @@ __global__ void features_kernel(const uint32_t* __restrict__ hist, bool hist64, 
     base[p * 3 + 0] = c.x;
     base[p * 3 + 1] = c.y;
     base[p * 3 + 2] = c.z;
+}
+
+// Denoiser input from a tonemapped image alone (the reference's denoise call
+// takes images): RGB as they are, the density from the tonemap's alpha,
+// sat(log2(w) / 24) (tonemap.cu:95-98), as w = 2^(24 alpha); a covered pixel
+// with alpha 0 holds at most one draw and counts as one, and a 3-channel
+// image, which has no alpha, gives every covered pixel one draw. The image is
+// padded with empty pixels to Wp x Hp (multiples of 8). fmt: 0 float4,
+// 1 half4, 2 float3, 3 half3.
+__device__ __forceinline__ float4 load_pixel(const void* img, int fmt, size_t i) {
+    switch (fmt) {
+        case 0: return reinterpret_cast<const float4*>(img)[i];
+        case 1: {
+            const __half* h = reinterpret_cast<const __half*>(img) + 4 * i;
+            return {__half2float(h[0]), __half2float(h[1]), __half2float(h[2]), __half2float(h[3])};
+        }
+        case 2: {
+            const float* f = reinterpret_cast<const float*>(img) + 3 * i;
+            return {f[0], f[1], f[2], 0.f};
+        }
+        default: {
+            const __half* h = reinterpret_cast<const __half*>(img) + 3 * i;
+            return {__half2float(h[0]), __half2float(h[1]), __half2float(h[2]), 0.f};
+        }
+    }
+}
+
+__global__ void image_features_kernel(const void* __restrict__ img, int fmt, int W, int H, int Wp, int Hp, int Cpad,
+                                      bf16* __restrict__ feat, float* __restrict__ base) {
+    const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= size_t(Wp) * Hp) return;
+    const int y = int(p / size_t(Wp)), x = int(p % size_t(Wp));
+    float4 c = {0.f, 0.f, 0.f, 0.f};
+    if (x < W && y < H) c = load_pixel(img, fmt, size_t(y) * W + x);
+    const bool covered = c.w > 0.f || c.x > 0.f || c.y > 0.f || c.z > 0.f;
+    const float dens = c.w > 0.f ? exp2f(24.0f * c.w) : (covered ? 1.0f : 0.0f);
+    const float f[8] = {c.x, c.y, c.z, c.w, log2f(1.0f + dens) * (1.0f / 16.0f), covered ? 1.0f : 0.0f, 0.f, 0.f};
+    for (int k = 0; k < Cpad; k++) feat[p * Cpad + k] = bf16(k < 6 ? f[k] : 0.f);
+    base[p * 3 + 0] = c.x;
+    base[p * 3 + 1] = c.y;
+    base[p * 3 + 2] = c.z;
+}
+
+// The denoised RGB (Wp x Hp, FP32) into the caller's image format; the alpha
+// of a 4-channel output is the input's (1 when the input has none).
+__global__ void image_output_kernel(const float* __restrict__ rgb, int Wp, const void* __restrict__ img, int in_fmt,
+                                    void* __restrict__ out, int out_fmt, int W, int H) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(W) * H) return;
+    const int y = int(i / size_t(W)), x = int(i % size_t(W));
+    const float* c = rgb + (size_t(y) * Wp + x) * 3;
+    const float a = (in_fmt == 0 || in_fmt == 1) ? load_pixel(img, in_fmt, i).w : 1.0f;
+    switch (out_fmt) {
+        case 0: reinterpret_cast<float4*>(out)[i] = {c[0], c[1], c[2], a}; break;
+        case 1: {
+            __half* h = reinterpret_cast<__half*>(out) + 4 * i;
+            h[0] = __float2half(c[0]); h[1] = __float2half(c[1]); h[2] = __float2half(c[2]); h[3] = __float2half(a);
+        } break;
+        case 2: {
+            float* f = reinterpret_cast<float*>(out) + 3 * i;
+            f[0] = c[0]; f[1] = c[1]; f[2] = c[2];
+        } break;
+        default: {
+            __half* h = reinterpret_cast<__half*>(out) + 3 * i;
+            h[0] = __float2half(c[0]); h[1] = __float2half(c[1]); h[2] = __float2half(c[2]);
+        } break;
+    }
 }
 
 __global__ void clamp_rgb_kernel(const float* __restrict__ in, size_t n, float* __restrict__ out) {
@@ -1253,6 +1321,18 @@ hipError_t gather_crops(const void* src, int H, int W, int pixel_bytes, const in
     dim3 grid(std::max(1, std::min(64, (cw * pixel_bytes + 255) / 256)), ch, n);
     gather_crops_kernel<<<grid, 256, 0, s>>>((const unsigned char*)src, H, W, pixel_bytes, offs, ch, cw,
                                              (unsigned char*)dst);
+    return hipGetLastError();
+}
+
+hipError_t image_features(const void* img, int fmt, int W, int H, int Wp, int Hp, int Cpad, void* feat, float* base,
+                          hipStream_t s) {
+    image_features_kernel<<<nb(size_t(Wp) * Hp), 256, 0, s>>>(img, fmt, W, H, Wp, Hp, Cpad, (bf16*)feat, base);
+    return hipGetLastError();
+}
+
+hipError_t image_output(const float* rgb, int Wp, const void* img, int in_fmt, void* out, int out_fmt, int W, int H,
+                        hipStream_t s) {
+    image_output_kernel<<<nb(size_t(W) * H), 256, 0, s>>>(rgb, Wp, img, in_fmt, out, out_fmt, W, H);
     return hipGetLastError();
 }
 

@@ -47,6 +47,15 @@ hipError_t features(const uint32_t* hist, bool hist64, const float* tm_rgba, siz
                     float* base,
                     hipStream_t s);
 hipError_t clamp_rgb(const float* in, size_t n, float* out, hipStream_t s);
+// features (Cpad BF16 channels) and FP32 RGB base, Wp x Hp (padded with empty
+// pixels), from a tonemapped image of W x H pixels in format fmt (0 float4,
+// 1 half4, 2 float3, 3 half3; the alpha of a 4-channel image gives the
+// density)
+hipError_t image_features(const void* img, int fmt, int W, int H, int Wp, int Hp, int Cpad, void* feat, float* base,
+                          hipStream_t s);
+// rgb (Wp-wide FP32 rows) into a W x H image of out_fmt, alpha from img
+hipError_t image_output(const float* rgb, int Wp, const void* img, int in_fmt, void* out, int out_fmt, int W, int H,
+                        hipStream_t s);
 // crops (img, y, x) from a stack of H x W images with pixel_bytes per pixel
 hipError_t gather_crops(const void* src, int H, int W, int pixel_bytes, const int* offs, int n, int ch, int cw,
                         void* dst, hipStream_t s);

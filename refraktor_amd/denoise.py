@@ -70,6 +70,13 @@ class Denoiser:
         """enqueued on `stream` (a state's stream keeps the frame's work in order)"""
         check(self._lib.rfk_denoiser_forward(self.handle, feat.ptr, base.ptr, n, h, w, out.ptr, stream))
 
+    def denoise(self, image: render.DeviceBuffer, in_format: int, out: render.DeviceBuffer, out_format: int,
+                width: int, height: int, stream=None) -> None:
+        """denoiser::denoise (interface/denoiser.h:35-38): a tonemapped device
+        image (float4 / half4 / float3 / half3) in, the denoised one out,
+        enqueued on `stream`"""
+        check(self._lib.rfk_denoise(self.handle, image.ptr, in_format, out.ptr, out_format, width, height, stream))
+
     def train_step(self, feat, base, target, n: int, h: int, w: int, lr: float) -> float:
         loss = C.c_float()
         check(self._lib.rfk_denoiser_train_step(self.handle, feat.ptr, base.ptr, target.ptr, n, h, w, lr,
