@@ -181,12 +181,13 @@ def main() -> int:
 
     lanes = [Lane() for _ in range(max(1, args.pipeline))]
 
-    def plain(lane: Lane, seed: int, kern=k, iters=ITERS, fmt=_lib.PIXEL_HALF4, walkers=args.walkers, de_r=DE_R):
+    def plain(lane: Lane, seed: int, kern=k, iters=ITERS, fmt=_lib.PIXEL_HALF4, walkers=args.walkers, de_r=DE_R,
+              smp=samples, tone=(gamma, bright, vib)):
         st = lane.states.get((id(kern), walkers))
         if st is None:
-            st = lane.states[(id(kern), walkers)] = kern.warmup(samples, W, H, seed=seed, fuse=20, walkers=walkers)
+            st = lane.states[(id(kern), walkers)] = kern.warmup(smp, W, H, seed=seed, fuse=20, walkers=walkers)
         else:
-            st.rewarm(samples, seed=seed, fuse=20)
+            st.rewarm(smp, seed=seed, fuse=20)
         # density estimation is enqueued behind the bin call; the host only
         # waits for the call's counters (the tonemap needs its quality)
         st.bin_async(iters=iters)
@@ -197,7 +198,7 @@ def main() -> int:
         if out is None:
             out = lane.images[fmt] = render.DeviceBuffer(npx * _lib.PIXEL_BYTES[fmt])
         inf = st.info()
-        ta = _lib.TonemapArgs(inf.quality, gamma, bright, vib, 0.0)
+        ta = _lib.TonemapArgs(inf.quality, tone[0], tone[1], tone[2], 0.0)
         # everything of a frame on the state's stream: the next re-warm waits for it
         if de_r > 0:
             check(lib.rfk_tonemap_image(lane.de.ptr, W, H, out.ptr, fmt, C.byref(ta), inf.stream))
@@ -205,8 +206,9 @@ def main() -> int:
             check(lib.rfk_tonemap(st.handle, out.ptr, fmt, C.byref(ta)))  # the histogram itself
         return st, res, out
 
-    def denoised(lane: Lane, seed: int):
-        st, res, tm = plain(lane, seed, k, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers, args.low_de_radius)
+    def denoised(lane: Lane, seed: int, kern=k, smp=samples, tone=(gamma, bright, vib)):
+        st, res, tm = plain(lane, seed, kern, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers,
+                            args.low_de_radius, smp, tone)
         inf = st.info()
         check(lib.rfk_denoiser_features(inf.histogram, inf.hist_bits, tm.ptr, npx, lane.feat.ptr, lane.base.ptr,
                                         inf.stream))
@@ -236,15 +238,29 @@ def main() -> int:
         ms_den, _, _, _, img_den = timed(denoised, args.steps, args.warmup)
 
     psnr_plain = psnr_den = None
+    judge = held_out = None
     if not args.no_psnr:
-        # converged reference: 2^32 iterations into u64 bins (no wrap in hot pixels)
-        plain_img = img_plain.download(np.float16, (H, W, 4)).astype(np.float32)
-        den_img = img_den.download(np.float32, (H, W, 3)) if img_den is not None else None
+        # the judge: the converged plain render, 2^32 iterations into u64 bins
+        # (no wrap in hot pixels), of the same flame. Renders are
+        # deterministic (integer bins, counter-based streams), so the timed
+        # frames are scored by rendering their seeds again, untimed.
         _, _, ref_buf = plain(lanes[0], args.seed + 5000, k64, 1 << 32, _lib.PIXEL_FLOAT4)
         ref = ref_buf.download(np.float32, (H, W, 4))
-        psnr_plain = psnr(plain_img, ref)
-        if den_img is not None:
-            psnr_den = psnr(den_img, ref)
+        frames_plain, frames_den = [], []
+        for i in range(args.steps):
+            _, _, img = plain(lanes[0], args.seed + i)
+            lib.rfk_synchronize()
+            frames_plain.append(score(img.download(np.float16, (H, W, 4)).astype(np.float32), ref))
+            if use_den:
+                _, _, img = denoised(lanes[0], args.seed + i)
+                lib.rfk_synchronize()
+                frames_den.append(score(img.download(np.float32, (H, W, 3)), ref))
+        judge = {"reference": "plain render of the same flame, 2^32 iterations, DE radius 9",
+                 "plain": summarize(frames_plain), "denoised": summarize(frames_den) if use_den else None}
+        psnr_plain = judge["plain"]["psnr_db_min"]
+        if use_den:
+            psnr_den = judge["denoised"]["psnr_db_min"]
+            held_out = held_out_flames(args, db, lanes, plain, denoised, score, summarize)
 
     # one frame at a time: the per-frame latency of each path, and the bin
     # call's device time without other frames' kernels beside it
@@ -282,8 +298,12 @@ def main() -> int:
 
     fps_plain = 1e3 / ms_plain
     fps_den = 1e3 / ms_den if ms_den else None
-    den_ok = fps_den is not None and psnr_den is not None and psnr_den >= 40.0
-    value = fps_den if den_ok else fps_plain
+    # a denoised frame under 40 dB falls back to the plain full-budget render:
+    # charged one plain frame more (frames in flight as timed)
+    fallbacks = judge["denoised"]["below_40db"] if judge and judge["denoised"] else None
+    eff_ms_den = None if ms_den is None or fallbacks is None else ms_den + fallbacks / args.steps * ms_plain
+    den_ok = eff_ms_den is not None
+    value = 1e3 / eff_ms_den if den_ok else fps_plain
     line = {
         "metric": "1080p frames/sec at PSNR>=40dB vs converged plain render; iterations/sec",
         "value": value,
@@ -291,14 +311,19 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_den if den_ok else ms_plain,
+        "ms_per_step": eff_ms_den if den_ok else ms_plain,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic seeded flame (SplitMix64 seed %d)" % args.seed,
-        "config": {"workload": "plain render 1920x1080, 8 xforms x 2-4 variations + final xform, 2^30 iterations, "
-                               "DE radius 9, log-density tonemap, vibrancy 1",
+        "config": {"workload": (f"denoised render 1920x1080: {pow2(args.low_iters)} iterations, log-density "
+                                "tonemap of the histogram, U-Net; "
+                                "judged per frame against the 2^32-iteration plain render (PSNR >= 40 dB, else the "
+                                "frame falls back to the plain render: 2^30 iterations, DE radius 9)"
+                                if den_ok else
+                                "plain render 1920x1080, 2^30 iterations, DE radius 9, log-density tonemap") +
+                               "; flame: 8 xforms x 2-4 variations + final xform, vibrancy 1",
                    "width": W, "height": H, "iterations": ITERS, "walkers": args.walkers,
                    "threads_per_block": args.tpb, "atomic_width": args.atomic_width, "bin_tile_log2": args.tile,
                    "hist_bits": args.hist_bits, "binning": args.binning,
@@ -308,8 +333,12 @@ def main() -> int:
                   "denoised": None if fps_den is None else
                   {"fps": fps_den, "ms": ms_den, "sequential_ms": seq_den, "iterations": args.low_iters,
                    "walkers": args.low_walkers, "de_radius": args.low_de_radius,
-                   "psnr_db": psnr_den,
-                   "speedup_vs_plain": fps_den / fps_plain, "weights": str(weights.name)}},
+                   "psnr_db": psnr_den, "fallback_frames": fallbacks,
+                   "fps_with_fallback": None if eff_ms_den is None else 1e3 / eff_ms_den,
+                   "speedup_vs_plain": (1e3 / eff_ms_den if eff_ms_den else fps_den) / fps_plain,
+                   "weights": str(weights.name)}},
+        "judge": judge,
+        "held_out": held_out,
         "iterations_per_s": it_per_s,
         "iterate_kernel_ms": kms,
         "psnr_db_vs_converged": psnr_den if den_ok else psnr_plain,
@@ -324,6 +353,66 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def pow2(n: int) -> str:
+    return f"2^{n.bit_length() - 1}" if n > 0 and n & (n - 1) == 0 else str(n)
+
+
+def score(img, ref) -> dict:
+    """PSNR (MAX 1, MSE over RGB of every pixel) and relative L1 (sum |a - b|
+    / sum |b| over RGB) of a tonemapped frame against the converged render"""
+    import numpy as np
+    a = img[..., :3].astype(np.float64)
+    b = ref[..., :3].astype(np.float64)
+    return {"psnr_db": psnr(a, b), "rel_l1": float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-30))}
+
+
+def summarize(frames: list) -> dict:
+    p = [f["psnr_db"] for f in frames]
+    r = [f["rel_l1"] for f in frames]
+    return {"frames": len(frames), "psnr_db": p, "rel_l1": r, "psnr_db_min": min(p), "psnr_db_mean": sum(p) / len(p),
+            "rel_l1_max": max(r), "rel_l1_mean": sum(r) / len(r), "below_40db": sum(1 for v in p if v < 40.0)}
+
+
+HELD_OUT_SEEDS = range(10001, 10009)
+
+
+def held_out_flames(args, db, lanes, plain, denoised, score, summarize) -> dict:
+    """the denoised path on flames the network never saw (1080p flames of
+    seeds 10001-10008, the bench's launch shape), each judged against its own
+    2^32-iteration plain render"""
+    import numpy as np
+    from refraktor_amd import _lib
+    from refraktor_amd import flame as fl
+    W, H = 1920, 1080
+    lib = _lib.lib()
+    queue = args.binning == "queue"
+    frames = []
+    for s in HELD_OUT_SEEDS:
+        f = fl.flame_1080(s)
+        kern = db.compile(f, threads_per_block=args.tpb, atomic_width=args.atomic_width, bin_tile_log2=args.tile,
+                          hist_bits=args.hist_bits, bin_queue=queue)
+        k64 = db.compile(f, threads_per_block=args.tpb, bin_tile_log2=args.tile, hist_bits=64, bin_queue=queue)
+        smp = kern.pack_samples(W, H)
+        tone = (f["gamma"], f["brightness"], f["vibrancy"])
+        _, _, rb = plain(lanes[0], s + 5000, k64, 1 << 32, _lib.PIXEL_FLOAT4, smp=smp, tone=tone)
+        ref = rb.download(np.float32, (H, W, 4))
+        _, _, img = denoised(lanes[0], s, kern, smp, tone)
+        lib.rfk_synchronize()
+        sc = score(img.download(np.float32, (H, W, 3)), ref)
+        _, _, lo = plain(lanes[0], s, kern, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers, 9.0, smp, tone)
+        lib.rfk_synchronize()
+        sc["plain_same_budget_psnr_db"] = score(lo.download(np.float32, (H, W, 4)), ref)["psnr_db"]
+        sc["seed"] = s
+        frames.append(sc)
+        # the states of these flames are not needed again
+        for key in [key for key in lanes[0].states if key[0] in (id(kern), id(k64))]:
+            del lanes[0].states[key]
+    out = summarize(frames)
+    out["seeds"] = [f["seed"] for f in frames]
+    out["plain_same_budget_psnr_db"] = [f["plain_same_budget_psnr_db"] for f in frames]
+    return out
 
 
 def measured_traffic(args, queue: bool):
@@ -345,22 +434,35 @@ def measured_traffic(args, queue: bool):
 
 
 def cpu_baseline(f: dict, W: int, H: int, seed: int, it_per_frame: int) -> dict:
-    """the oracle's OpenMP render of the same flame on a bounded sample"""
+    """the oracle's OpenMP render of the same flame in single precision,
+    compiled here for this host (-O3 -march=native), on a bounded sample"""
+    import tempfile
+    from oracle import build as obuild
     from oracle import oracle
     from refraktor_amd import render
     blocks, tpb = 1024, 256
-    passes = 1024  # 1024*256*1024 = 2^28 iterations, a few seconds on 16 threads
+    passes = 1024  # 1024*256*1024 = 2^28 iterations
     L, P32, pals = oracle.prepare(f, render.CONFIG_DIR, W, H, blocks, seed)
-    threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or min(os.cpu_count() or 1, 16)
-    t0 = time.perf_counter()
-    _, draws = oracle.render(L, P32, P32.shape[1], pals, blocks, tpb, W, H, seed, 20, passes, threads=threads)
-    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or (os.cpu_count() or 1)
+    model = "unknown"
+    try:
+        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    with tempfile.TemporaryDirectory() as tmp:
+        lib = obuild.build_f32(Path(tmp) / "liboracle_f32.so")
+        oracle.render_f32(lib, L, P32, P32.shape[1], pals, 64, tpb, W, H, seed, 20, 16, threads=threads)  # warm
+        t0 = time.perf_counter()
+        _, draws = oracle.render_f32(lib, L, P32, P32.shape[1], pals, blocks, tpb, W, H, seed, 20, passes,
+                                     threads=threads)
+        dt = time.perf_counter() - t0
     iters = blocks * tpb * passes
     ips = iters / dt
     return {"value": ips / it_per_frame, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"{iters} iterations (+20 fuse passes) of the same 1080p flame on {threads} threads in "
-                      f"{dt:.2f} s ({ips:.3e} it/s), scaled to a 2^30-iteration frame",
-            "iterations_per_s": ips}
+            "sample": f"{iters} iterations (+20 fuse passes) of the same 1080p flame, single precision, "
+                      f"gcc -O3 -march=native -fopenmp, {threads} threads on {model} ({os.cpu_count()} "
+                      f"logical CPUs visible) in {dt:.2f} s ({ips:.3e} it/s), scaled to a 2^30-iteration frame",
+            "iterations_per_s": ips, "cpu_model": model}
 
 
 if __name__ == "__main__":

@@ -9,7 +9,7 @@
  *   - xform selection by cumulative weight (assets/templates/flame.tpl:110-123),
  *     float arithmetic, so choices are bit-identical;
  *   - the variations used by the test and benchmark flames, hand-written in
- *     double from the standard flam3 formulas (not from the product's DSL);
+ *     real from the standard flam3 formulas (not from the product's DSL);
  *   - the per-pass loop of the reference (refactor.cu:176-210, 347-386):
  *     xform, bad-value reset, final xform on a copy, draw, then the block
  *     permutation of the GPU scheme;
@@ -27,6 +27,16 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
+/* The iterate path's float type: double for the checker. The host-core
+ * baseline build (oracle/build.py: -DORC_REAL=float -O3 -march=native
+ * -fsingle-precision-constant) runs the same loop in single precision with
+ * the type-generic functions of <tgmath.h>. */
+#ifndef ORC_REAL
+#define ORC_REAL double
+#endif
+typedef ORC_REAL real;
+#include <tgmath.h>
 
 typedef uint32_t u32;
 typedef uint64_t u64;
@@ -80,13 +90,13 @@ static inline u32 rng_next(lane_rng* r) {
     r->avail--;
     return r->buf[r->pos++];
 }
-static inline double rand01(lane_rng* r) { return (double)u01(rng_next(r)); }
-static inline double randbit(lane_rng* r) { return (double)((rng_next(r) >> 24) & 1u); }
-static void randgauss(lane_rng* r, double sd, double* gx, double* gy) {
-    double a = (double)(u01(rng_next(r)) * 2.0f);
-    double s = sin(M_PI * a), c = cos(M_PI * a);
+static inline real rand01(lane_rng* r) { return (real)u01(rng_next(r)); }
+static inline real randbit(lane_rng* r) { return (real)((rng_next(r) >> 24) & 1u); }
+static void randgauss(lane_rng* r, real sd, real* gx, real* gy) {
+    real a = (real)(u01(rng_next(r)) * 2.0f);
+    real s = sin(M_PI * a), c = cos(M_PI * a);
     float f0 = u01(rng_next(r)), f1 = u01(rng_next(r)), f2 = u01(rng_next(r)), f3 = u01(rng_next(r));
-    double rr = sd * ((double)f0 + f1 + f2 + f3 - 2.0);
+    real rr = sd * ((real)f0 + f1 + f2 + f3 - 2.0);
     *gx = rr * c;
     *gy = rr * s;
 }
@@ -146,93 +156,93 @@ static void prog_index(flame_prog* fp, const int* prog, int n_xforms, int has_fi
     }
 }
 
-static inline double eps_(double x) { return x == 0.0 ? 1e-20 : x; }
+static inline real eps_(real x) { return x == 0.0 ? 1e-20 : x; }
 
-static void apply_variation(int id, double w, const double* prm, const double* aff, double px, double py,
-                            double* rx, double* ry, lane_rng* rs) {
-    double sumsq = px * px + py * py, r = hypot(px, py);
-    double atn = atan2(px, py), phi = atan2(py, px);
-    double rinv = 1.0 / sqrt(sumsq);
-    double sina = px * rinv, cosa = py * rinv;
+static void apply_variation(int id, real w, const real* prm, const real* aff, real px, real py,
+                            real* rx, real* ry, lane_rng* rs) {
+    real sumsq = px * px + py * py, r = hypot(px, py);
+    real atn = atan2(px, py), phi = atan2(py, px);
+    real rinv = 1.0 / sqrt(sumsq);
+    real sina = px * rinv, cosa = py * rinv;
     switch (id) {
         case V_LINEAR: *rx += w * px; *ry += w * py; break;
         case V_SINUSOIDAL: *rx += w * sin(px); *ry += w * sin(py); break;
-        case V_SPHERICAL: { double k = w / eps_(sumsq); *rx += k * px; *ry += k * py; } break;
+        case V_SPHERICAL: { real k = w / eps_(sumsq); *rx += k * px; *ry += k * py; } break;
         case V_SWIRL: {
-            double s = sin(sumsq), c = cos(sumsq);
+            real s = sin(sumsq), c = cos(sumsq);
             *rx += w * (s * px - c * py);
             *ry += w * (c * px + s * py);
         } break;
-        case V_HORSESHOE: { double k = w / eps_(r); *rx += k * (px - py) * (px + py); *ry += k * 2.0 * px * py; } break;
+        case V_HORSESHOE: { real k = w / eps_(r); *rx += k * (px - py) * (px + py); *ry += k * 2.0 * px * py; } break;
         case V_POLAR: *rx += w * atn / M_PI; *ry += w * (r - 1.0); break;
         case V_HANDKERCHIEF: *rx += w * r * sin(atn + r); *ry += w * r * cos(atn - r); break;
-        case V_HEART: { double a = r * atn; *rx += w * r * sin(a); *ry -= w * r * cos(a); } break;
-        case V_DISC: { double k = w * atn / M_PI; *rx += k * sin(M_PI * r); *ry += k * cos(M_PI * r); } break;
+        case V_HEART: { real a = r * atn; *rx += w * r * sin(a); *ry -= w * r * cos(a); } break;
+        case V_DISC: { real k = w * atn / M_PI; *rx += k * sin(M_PI * r); *ry += k * cos(M_PI * r); } break;
         case V_SPIRAL: {
-            double re = eps_(r);
+            real re = eps_(r);
             *rx += w / re * (cosa + sin(re));
             *ry += w / re * (sina - cos(re));
         } break;
-        case V_HYPERBOLIC: { double re = eps_(r); *rx += w * sina / re; *ry += w * cosa * re; } break;
+        case V_HYPERBOLIC: { real re = eps_(r); *rx += w * sina / re; *ry += w * cosa * re; } break;
         case V_DIAMOND: *rx += w * sina * cos(r); *ry += w * cosa * sin(r); break;
         case V_EX: {
-            double n0 = sin(atn + r), n1 = cos(atn - r);
-            double m0 = n0 * n0 * n0 * r, m1 = n1 * n1 * n1 * r;
+            real n0 = sin(atn + r), n1 = cos(atn - r);
+            real m0 = n0 * n0 * n0 * r, m1 = n1 * n1 * n1 * r;
             *rx += w * (m0 + m1);
             *ry += w * (m0 - m1);
         } break;
         case V_JULIA: {
-            double a = 0.5 * atn + M_PI * randbit(rs);
-            double k = w * sqrt(r);
+            real a = 0.5 * atn + M_PI * randbit(rs);
+            real k = w * sqrt(r);
             *rx += k * cos(a);
             *ry += k * sin(a);
         } break;
-        case V_BUBBLE: { double k = w / (0.25 * sumsq + 1.0); *rx += k * px; *ry += k * py; } break;
+        case V_BUBBLE: { real k = w / (0.25 * sumsq + 1.0); *rx += k * px; *ry += k * py; } break;
         case V_CURL: {
-            double c1 = prm[0], c2 = prm[1];
-            double re = 1.0 + c1 * px + c2 * (px * px - py * py);
-            double im = c1 * py + 2.0 * c2 * px * py;
-            double k = w / (re * re + im * im);
+            real c1 = prm[0], c2 = prm[1];
+            real re = 1.0 + c1 * px + c2 * (px * px - py * py);
+            real im = c1 * py + 2.0 * c2 * px * py;
+            real k = w / (re * re + im * im);
             *rx += (px * re + py * im) * k;
             *ry += (py * re - px * im) * k;
         } break;
-        case V_FISHEYE: { double k = 2.0 * w / (r + 1.0); *rx += k * py; *ry += k * px; } break;
-        case V_EYEFISH: { double k = 2.0 * w / (r + 1.0); *rx += k * px; *ry += k * py; } break;
+        case V_FISHEYE: { real k = 2.0 * w / (r + 1.0); *rx += k * py; *ry += k * px; } break;
+        case V_EYEFISH: { real k = 2.0 * w / (r + 1.0); *rx += k * px; *ry += k * py; } break;
         case V_BLUR: {
-            double k = w * rand01(rs);
-            double a = (double)((float)rand01(rs) * 2.0f);
+            real k = w * rand01(rs);
+            real a = (real)((float)rand01(rs) * 2.0f);
             *rx += k * cos(M_PI * a);
             *ry += k * sin(M_PI * a);
         } break;
-        case V_GAUSSIAN_BLUR: { double gx, gy; randgauss(rs, 1.0, &gx, &gy); *rx += w * gx; *ry += w * gy; } break;
+        case V_GAUSSIAN_BLUR: { real gx, gy; randgauss(rs, 1.0, &gx, &gy); *rx += w * gx; *ry += w * gy; } break;
         case V_WAVES: /* params: dx2, dy2 (precalc) */
             *rx += w * (px + aff[2] * sin(py * prm[0]));
             *ry += w * (py + aff[3] * sin(px * prm[1]));
             break;
         case V_JULIAN: { /* params: cn, dist, power, rN */
-            double cn = prm[0], power = prm[2], rN = prm[3];
-            double k = trunc(rN * rand01(rs));
-            double a = (phi + 2.0 * M_PI * k) / power;
-            double m = w * pow(sumsq, cn);
+            real cn = prm[0], power = prm[2], rN = prm[3];
+            real k = trunc(rN * rand01(rs));
+            real a = (phi + 2.0 * M_PI * k) / power;
+            real m = w * pow(sumsq, cn);
             *rx += m * cos(a);
             *ry += m * sin(a);
         } break;
         case V_BENT: {
-            double qx = px < 0.0 ? px * 2.0 : px, qy = py < 0.0 ? py * 0.5 : py;
+            real qx = px < 0.0 ? px * 2.0 : px, qy = py < 0.0 ? py * 0.5 : py;
             *rx += w * qx; *ry += w * qy;
         } break;
         case V_CYLINDER: *rx += w * sin(px); *ry += w * py; break;
-        case V_EXPONENTIAL: { double e = w * exp(px - 1.0); *rx += e * cos(M_PI * py); *ry += e * sin(M_PI * py); } break;
-        case V_POWER: { double k = w * pow(r, sina); *rx += k * cosa; *ry += k * sina; } break;
+        case V_EXPONENTIAL: { real e = w * exp(px - 1.0); *rx += e * cos(M_PI * py); *ry += e * sin(M_PI * py); } break;
+        case V_POWER: { real k = w * pow(r, sina); *rx += k * cosa; *ry += k * sina; } break;
         case V_COSINE: *rx += w * cos(M_PI * px) * cosh(py); *ry -= w * sin(M_PI * px) * sinh(py); break;
         case V_NOISE: {
-            double k = w * rand01(rs);
-            double a = (double)(2.0f * (float)rand01(rs));
+            real k = w * rand01(rs);
+            real a = (real)(2.0f * (float)rand01(rs));
             *rx += k * px * cos(M_PI * a);
             *ry += k * py * sin(M_PI * a);
         } break;
         case V_RECTANGLES: { /* x, y */
-            double ax = prm[0], ay = prm[1];
+            real ax = prm[0], ay = prm[1];
             *rx += ax == 0.0 ? w * px : w * ((2.0 * floor(px / ax) + 1.0) * ax - px);
             *ry += ay == 0.0 ? w * py : w * ((2.0 * floor(py / ay) + 1.0) * ay - py);
         } break;
@@ -240,8 +250,8 @@ static void apply_variation(int id, double w, const double* prm, const double* a
             *rx += w * (sin(prm[0] * py) - cos(prm[1] * px));
             *ry += w * (sin(prm[2] * px) - cos(prm[3] * py));
             break;
-        case V_CROSS: { double s = px * px - py * py; double k = w / sqrt(eps_(s * s)); *rx += k * px; *ry += k * py; } break;
-        case V_EXP: { double e = w * exp(px); *rx += e * cos(py); *ry += e * sin(py); } break;
+        case V_CROSS: { real s = px * px - py * py; real k = w / sqrt(eps_(s * s)); *rx += k * px; *ry += k * py; } break;
+        case V_EXP: { real e = w * exp(px); *rx += e * cos(py); *ry += e * sin(py); } break;
         case V_LOG: *rx += 0.5 * w * log(sumsq); *ry += w * phi; break;
         /* sinh and cos as librefrakt's programs (config/variations_new.yml:880-885,
          * 841-846): sinh takes the hyperbolic factors of y, cos adds the
@@ -251,49 +261,49 @@ static void apply_variation(int id, double w, const double* prm, const double* a
         case V_SIN: *rx += w * sin(px) * cosh(py); *ry += w * cos(px) * sinh(py); break;
         case V_COS: *rx += w * cos(px) * cosh(py); *ry += w * sin(px) * sinh(py); break;
         case V_TANGENT: *rx += w * sin(px) / cos(py); *ry += w * tan(py); break;
-        case V_SQUARE: { double a = rand01(rs), b = rand01(rs); *rx += w * (a - 0.5); *ry += w * (b - 0.5); } break;
+        case V_SQUARE: { real a = rand01(rs), b = rand01(rs); *rx += w * (a - 0.5); *ry += w * (b - 0.5); } break;
         case V_POLAR2: *rx += w / M_PI * atn; *ry += 0.5 * w / M_PI * log(sumsq); break;
         case V_FOCI: {
-            double ex = 0.5 * exp(px), enx = 0.25 / ex;
-            double k = w / (ex + enx - cos(py));
+            real ex = 0.5 * exp(px), enx = 0.25 / ex;
+            real k = w / (ex + enx - cos(py));
             *rx += k * (ex - enx);
             *ry += k * sin(py);
         } break;
         case V_PERSPECTIVE: { /* angle dist vfcos vsin */
-            double dist = prm[1], vfcos = prm[2], vsin = prm[3];
-            double t = 1.0 / (dist - py * vsin);
+            real dist = prm[1], vfcos = prm[2], vsin = prm[3];
+            real t = 1.0 / (dist - py * vsin);
             *rx += w * dist * px * t;
             *ry += w * vfcos * py * t;
         } break;
         case V_BLOB: { /* high low waves */
-            double hi = prm[0], lo = prm[1], wv = prm[2];
-            double k = w * r * (lo + (hi - lo) * (0.5 + 0.5 * sin(wv * atn)));
+            real hi = prm[0], lo = prm[1], wv = prm[2];
+            real k = w * r * (lo + (hi - lo) * (0.5 + 0.5 * sin(wv * atn)));
             *rx += k * sina;
             *ry += k * cosa;
         } break;
         case V_RINGS: {
-            double d = eps_(aff[4] * aff[4]);
-            double k = fmod(r + d, 2.0 * d) - d + r * (1.0 - d);
+            real d = eps_(aff[4] * aff[4]);
+            real k = fmod(r + d, 2.0 * d) - d + r * (1.0 - d);
             *rx += w * k * cosa;
             *ry += w * k * sina;
         } break;
         case V_FAN: {
-            double d = M_PI * eps_(aff[4] * aff[4]), h = 0.5 * d;
-            double a = atn + copysign(h, h - fmod(atn + aff[5], d));
+            real d = M_PI * eps_(aff[4] * aff[4]), h = 0.5 * d;
+            real a = atn + copysign(h, h - fmod(atn + aff[5], d));
             *rx += w * r * cos(a);
             *ry += w * r * sin(a);
         } break;
         case V_BENT2: {
-            double qx = px < 0.0 ? px * prm[0] : px, qy = py < 0.0 ? py * prm[1] : py;
+            real qx = px < 0.0 ? px * prm[0] : px, qy = py < 0.0 ? py * prm[1] : py;
             *rx += w * qx; *ry += w * qy;
         } break;
         case V_NGON: { /* circle corners power sides */
-            double circle = prm[0], corners = prm[1], power = prm[2], sides = prm[3];
-            double rf = pow(sumsq, 0.5 * power);
-            double b = 2.0 * M_PI / sides;
-            double ph = phi - b * floor(phi / b);
+            real circle = prm[0], corners = prm[1], power = prm[2], sides = prm[3];
+            real rf = pow(sumsq, 0.5 * power);
+            real b = 2.0 * M_PI / sides;
+            real ph = phi - b * floor(phi / b);
             if (ph > 0.5 * b) ph -= b;
-            double amp = (corners * (1.0 / eps_(cos(ph)) - 1.0) + circle) / eps_(rf);
+            real amp = (corners * (1.0 / eps_(cos(ph)) - 1.0) + circle) / eps_(rf);
             *rx += w * px * amp;
             *ry += w * py * amp;
         } break;
@@ -304,14 +314,14 @@ static void apply_variation(int id, double w, const double* prm, const double* a
 /* One variation at n points p (after the vlink affine): prm in the sorted
  * order of all the variation's parameter names, aff as (a, d, b, e, c, f),
  * words: nwords random words per point, consumed in order. */
-void orc_eval_variation(int id, double w, const double* prm, const double* aff, size_t n, const double* px,
-                        const double* py, const u32* words, int nwords, double* rx, double* ry) {
+void orc_eval_variation(int id, real w, const real* prm, const real* aff, size_t n, const real* px,
+                        const real* py, const u32* words, int nwords, real* rx, real* ry) {
     for (size_t i = 0; i < n; i++) {
         lane_rng rs;
         rng_init(&rs, 0u, 0u, 0u, 0u);
         rs.list = words + i * (size_t)nwords;
         rs.nlist = nwords;
-        double x = 0.0, y = 0.0;
+        real x = 0.0, y = 0.0;
         apply_variation(id, w, prm, aff, px[i], py[i], &x, &y, &rs);
         rx[i] = x;
         ry[i] = y;
@@ -319,30 +329,30 @@ void orc_eval_variation(int id, double w, const double* prm, const double* aff, 
 }
 
 /* affine in (a, d, b, e, c, f) order; aff[] below is passed as (a,d,b,e,c,f) */
-static double apply_xform(const flame_prog* fp, int x, const float* P, double* px, double* py, double* pz,
+static real apply_xform(const flame_prog* fp, int x, const float* P, real* px, real* py, real* pz,
                           lane_rng* rs) {
     const int* q = fp->prog + fp->xform_start[x];
     int nv = q[0];
-    double w_color = P[q[2]], w_speed = P[q[3]], opacity = P[q[4]];
+    real w_color = P[q[2]], w_speed = P[q[3]], opacity = P[q[4]];
     q += 5;
-    double ix = *px, iy = *py;
+    real ix = *px, iy = *py;
     for (int v = 0; v < nv; v++) {
         int aoff = q[0], nvar = q[1];
         q += 2;
-        double aff[6];
+        real aff[6];
         for (int k = 0; k < 6; k++) aff[k] = P[aoff + k];
         /* x' = a x + b y + c ; y' = d x + e y + f */
-        double ax = aff[0] * ix + aff[2] * iy + aff[4];
-        double ay = aff[1] * ix + aff[3] * iy + aff[5];
-        double rx = 0.0, ry = 0.0;
-        double affv[6] = {aff[0], aff[1], aff[2], aff[3], aff[4], aff[5]};
+        real ax = aff[0] * ix + aff[2] * iy + aff[4];
+        real ay = aff[1] * ix + aff[3] * iy + aff[5];
+        real rx = 0.0, ry = 0.0;
+        real affv[6] = {aff[0], aff[1], aff[2], aff[3], aff[4], aff[5]};
         for (int k = 0; k < nvar; k++) {
             int id = q[0], woff = q[1], np = q[2];
-            double prm[16];
+            real prm[16];
             for (int j = 0; j < np && j < 16; j++) prm[j] = P[q[3 + j]];
             /* waves/popcorn-style access to the affine uses b and e at [2],[3]
              * through the vlink affine (a,d,b,e,c,f) -> pass b,e and c,f */
-            double affx[6] = {affv[0], affv[1], affv[2], affv[3], affv[4], affv[5]};
+            real affx[6] = {affv[0], affv[1], affv[2], affv[3], affv[4], affv[5]};
             apply_variation(id, P[woff], prm, affx, ax, ay, &rx, &ry, rs);
             q += 3 + np;
         }
@@ -397,7 +407,7 @@ static u32 select_xform_chaos(const flame_prog* fp, const float* P, u32 last, fl
     return last_nonzero;
 }
 
-static inline int badvalue(double v) { return !(fabs(v) <= 1e10); }
+static inline int badvalue(real v) { return !(fabs(v) <= 1e10); }
 
 /* -------------------------------------------------------------- draw */
 static inline int make_draw(const float* S, const u32* pal, u32 width, u32 height, float x, float y, float z,
@@ -440,8 +450,8 @@ void orc_bin_points(const float* S, const u32* pal, const float* pts, size_t n, 
  * one (lane stream: the pick is the pass's first draw), walkers keep their
  * own point (no permutation); xids per pass per walker. */
 static u64 run_block_chaos(const flame_prog* fp, const float* P, const u32* pal, u32 block, u32 tpb, u32 seed,
-                           u32 stream, u32 pass0, u32 npasses, double* x, double* y, double* c, u32* last,
-                           double* trace, u32* xids, const float* gpu_trace, const u32* gpu_xids, u32 trace_stride,
+                           u32 stream, u32 pass0, u32 npasses, real* x, real* y, real* c, u32* last,
+                           real* trace, u32* xids, const float* gpu_trace, const u32* gpu_xids, u32 trace_stride,
                            u32* hist, u32 width, u32 height) {
     u64 draws = 0;
     float S[6];
@@ -462,15 +472,15 @@ static u64 run_block_chaos(const flame_prog* fp, const float* P, const u32* pal,
             rng_begin(&rs, pass);
             u32 xid = select_xform_chaos(fp, P, last[t], (float)rand01(&rs));
             if (xids) xids[(size_t)i * trace_stride + g] = xid;
-            double px = x[t], py = y[t], pz = c[t];
-            double op = apply_xform(fp, (int)xid, P, &px, &py, &pz, &rs);
+            real px = x[t], py = y[t], pz = c[t];
+            real op = apply_xform(fp, (int)xid, P, &px, &py, &pz, &rs);
             if (badvalue((float)px) || badvalue((float)py)) {
                 px = rand01(&rs) * 2.0 - 1.0;
                 py = rand01(&rs) * 2.0 - 1.0;
                 op = 0.0;
             }
             if (hist) {
-                double dx = px, dy = py, dz = pz;
+                real dx = px, dy = py, dz = pz;
                 if (fp->has_final) apply_xform(fp, fp->n_xforms, P, &dx, &dy, &dz, &rs);
                 u32 pix, inc[4];
                 if (make_draw(S, pal, width, height, (float)dx, (float)dy, (float)dz, (float)op, &pix, inc)) {
@@ -479,7 +489,7 @@ static u64 run_block_chaos(const flame_prog* fp, const float* P, const u32* pal,
                 }
             }
             if (trace) {
-                double* tr = trace + ((size_t)i * trace_stride + g) * 4;
+                real* tr = trace + ((size_t)i * trace_stride + g) * 4;
                 tr[0] = px; tr[1] = py; tr[2] = pz; tr[3] = op;
             }
             x[t] = px; y[t] = py; c[t] = pz;
@@ -490,9 +500,9 @@ static u64 run_block_chaos(const flame_prog* fp, const float* P, const u32* pal,
 }
 
 static u64 run_block(const flame_prog* fp, const float* P, const u32* pal, u32 block, u32 tpb, u32 tpb_bits,
-                     u32 seed, u32 stream, u32 pass0, u32 npasses, double* x, double* y, double* c, double* trace,
+                     u32 seed, u32 stream, u32 pass0, u32 npasses, real* x, real* y, real* c, real* trace,
                      u32* xids, const float* gpu_trace, u32 trace_stride, u32* hist, u32 width, u32 height,
-                     double* ox, double* oy, double* oc) {
+                     real* ox, real* oy, real* oc) {
     u64 draws = 0;
     float S[6];
     for (int k = 0; k < 6; k++) S[k] = P[k];
@@ -529,15 +539,15 @@ static u64 run_block(const flame_prog* fp, const float* P, const u32* pal, u32 b
                 lane_rng rs;
                 rng_init(&rs, seed, stream, block * tpb + t, TAG_LANE);
                 rng_begin(&rs, pass);
-                double px = x[t], py = y[t], pz = c[t];
-                double op = apply_xform(fp, (int)xid, P, &px, &py, &pz, &rs);
+                real px = x[t], py = y[t], pz = c[t];
+                real op = apply_xform(fp, (int)xid, P, &px, &py, &pz, &rs);
                 if (badvalue((float)px) || badvalue((float)py)) {
                     px = rand01(&rs) * 2.0 - 1.0;
                     py = rand01(&rs) * 2.0 - 1.0;
                     op = 0.0;
                 }
                 if (hist) {
-                    double dx = px, dy = py, dz = pz;
+                    real dx = px, dy = py, dz = pz;
                     if (fp->has_final) apply_xform(fp, fp->n_xforms, P, &dx, &dy, &dz, &rs);
                     u32 pix, inc[4];
                     if (make_draw(S, pal, width, height, (float)dx, (float)dy, (float)dz, (float)op, &pix, inc)) {
@@ -546,7 +556,7 @@ static u64 run_block(const flame_prog* fp, const float* P, const u32* pal, u32 b
                     }
                 }
                 if (trace) {
-                    double* tr = trace + ((size_t)i * trace_stride + (size_t)block * tpb + t) * 4;
+                    real* tr = trace + ((size_t)i * trace_stride + (size_t)block * tpb + t) * 4;
                     tr[0] = px; tr[1] = py; tr[2] = pz; tr[3] = op;
                 }
                 ox[t] = px; oy[t] = py; oc[t] = pz;
@@ -570,8 +580,8 @@ static u32 bits_of(u32 tpb) {
 
 /* Trace `npasses` passes for every block (see run_block). x/y/c: all walkers. */
 int orc_trace(const int* prog, int n_xforms, int has_final, int chaos_base, const float* params, u32 stride,
-              u32 blocks, u32 tpb, u32 seed, u32 stream, u32 pass0, u32 npasses, double* x, double* y, double* c,
-              u32* last, double* trace, u32* xids, const float* gpu_trace, const u32* gpu_xids) {
+              u32 blocks, u32 tpb, u32 seed, u32 stream, u32 pass0, u32 npasses, real* x, real* y, real* c,
+              u32* last, real* trace, u32* xids, const float* gpu_trace, const u32* gpu_xids) {
     flame_prog fp;
     prog_index(&fp, prog, n_xforms, has_final, chaos_base);
     u32 bits = bits_of(tpb);
@@ -587,7 +597,7 @@ int orc_trace(const int* prog, int n_xforms, int has_final, int chaos_base, cons
     }
 #pragma omp parallel for schedule(dynamic)
     for (long b = 0; b < (long)blocks; b++) {
-        double* ox = (double*)malloc(sizeof(double) * tpb * 3);
+        real* ox = (real*)malloc(sizeof(real) * tpb * 3);
         if (!ox) { ok = 0; continue; }
         run_block(&fp, params + (size_t)b * stride, NULL, (u32)b, tpb, bits, seed, stream, pass0, npasses,
                   x + (size_t)b * tpb, y + (size_t)b * tpb, c + (size_t)b * tpb, trace, xids, gpu_trace,
@@ -600,7 +610,7 @@ int orc_trace(const int* prog, int n_xforms, int has_final, int chaos_base, cons
 /* Walker seeding: Hammersley point of gid over `total` mapped to pixels and
  * through plane_space (params[6..11]); colour from the INIT stream. */
 void orc_seed(const float* params, u32 stride, u32 blocks, u32 tpb, u32 width, u32 height, u32 seed, u32 stream,
-              double* x, double* y, double* c) {
+              real* x, real* y, real* c) {
     u32 total = blocks * tpb;
     u32 max_val = total;
     if (total % 2u) {
@@ -614,14 +624,14 @@ void orc_seed(const float* params, u32 stride, u32 blocks, u32 tpb, u32 width, u
         const float* P = params + (size_t)(g / tpb) * stride;
         u32 rev = 0;
         for (int k = 0; k < 32; k++) if (g & (1u << k)) rev |= 1u << (31 - k);
-        double hx = (double)g / max_val * 2.0 - 1.0;
-        double hy = (radix ? (double)(rev >> (32u - radix)) : 0.0) / max_val * 2.0 - 1.0;
-        double px = (hx + 1.0) / 2.0 * width, py = (hy + 1.0) / 2.0 * height;
-        x[g] = (double)P[6] * px + (double)P[8] * py + P[10];
-        y[g] = (double)P[7] * px + (double)P[9] * py + P[11];
+        real hx = (real)g / max_val * 2.0 - 1.0;
+        real hy = (radix ? (real)(rev >> (32u - radix)) : 0.0) / max_val * 2.0 - 1.0;
+        real px = (hx + 1.0) / 2.0 * width, py = (hy + 1.0) / 2.0 * height;
+        x[g] = (real)P[6] * px + (real)P[8] * py + P[10];
+        y[g] = (real)P[7] * px + (real)P[9] * py + P[11];
         u32 ctr[4] = {0u, g, 0u, TAG_INIT}, key[2] = {seed, stream}, out[4];
         orc_philox(ctr, key, out);
-        c[g] = (double)u01(out[0]);
+        c[g] = (real)u01(out[0]);
     }
 }
 
@@ -634,9 +644,9 @@ u64 orc_render(const int* prog, int n_xforms, int has_final, int chaos_base, con
     prog_index(&fp, prog, n_xforms, has_final, chaos_base);
     u32 bits = bits_of(tpb);
     size_t total = (size_t)blocks * tpb, npx = (size_t)width * height;
-    double* x = (double*)malloc(sizeof(double) * total);
-    double* y = (double*)malloc(sizeof(double) * total);
-    double* c = (double*)malloc(sizeof(double) * total);
+    real* x = (real*)malloc(sizeof(real) * total);
+    real* y = (real*)malloc(sizeof(real) * total);
+    real* c = (real*)malloc(sizeof(real) * total);
     if (!x || !y || !c) { free(x); free(y); free(c); return 0; }
     orc_seed(params, stride, blocks, tpb, width, height, seed, stream, x, y, c);
     u64 draws = 0;
@@ -646,11 +656,11 @@ u64 orc_render(const int* prog, int n_xforms, int has_final, int chaos_base, con
 #pragma omp parallel reduction(+ : draws)
     {
         u32* local = (u32*)calloc(npx * 4, sizeof(u32));
-        double* ox = (double*)malloc(sizeof(double) * tpb * 3);
+        real* ox = (real*)malloc(sizeof(real) * tpb * 3);
 #pragma omp for schedule(dynamic)
         for (long b = 0; b < (long)blocks; b++) {
             const float* P = params + (size_t)b * stride;
-            double *bx = x + (size_t)b * tpb, *by = y + (size_t)b * tpb, *bc = c + (size_t)b * tpb;
+            real *bx = x + (size_t)b * tpb, *by = y + (size_t)b * tpb, *bc = c + (size_t)b * tpb;
             if (chaos_base >= 0) {
                 u32* last = (u32*)malloc(sizeof(u32) * tpb);
                 for (u32 t = 0; t < tpb; t++) last[t] = 0xFFFFFFFFu;

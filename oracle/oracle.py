@@ -529,6 +529,23 @@ def render(L: Layout, params32, stride, palettes, blocks, tpb, width, height, se
     return hist, int(draws)
 
 
+def render_f32(lib_path, L: Layout, params32, stride, palettes, blocks, tpb, width, height, seed, fuse, passes,
+               threads=0, stream=0):
+    """render() with the single-precision baseline build (build.build_f32)"""
+    lib = C.CDLL(str(lib_path))
+    P = C.c_void_p
+    lib.orc_render.argtypes = [P, C.c_int, C.c_int, C.c_int, P, C.c_uint32, P, C.c_uint32, C.c_uint32,
+                               C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P, C.c_int]
+    lib.orc_render.restype = C.c_uint64
+    prog = L.program()
+    hist = np.zeros((height, width, 4), dtype=np.uint32)
+    p = np.ascontiguousarray(params32, dtype=np.float32)
+    pal = np.ascontiguousarray(palettes, dtype=np.uint32)
+    draws = lib.orc_render(_ptr(prog), L.n_xforms, int(L.has_final), L.chaos_base, _ptr(p), stride, _ptr(pal),
+                           blocks, tpb, width, height, seed, stream, fuse, passes, _ptr(hist), threads)
+    return hist, int(draws)
+
+
 def prepare(flame: dict, config_dir, width: int, height: int, blocks: int, seed: int, stream: int = 0):
     """the oracle's own warmup: layout, 4 identical samples, Catmull-Rom,
     evaluation, precalc; returns (layout, float32 params [blocks][stride], palettes)"""
