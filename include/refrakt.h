@@ -395,6 +395,17 @@ int rfk_nn_conv3x3_dgrad(const void* dy, int Cout, int N, int H, int W, const vo
  * refused before anything is launched                                       */
 int rfk_nn_gather_crops(const void* src, int n_images, int H, int W, int pixel_bytes, const int* offsets, int n,
                         int ch, int cw, void* dst, void* stream);
+/* No counterpart in the reference: rfk_nn_gather_crops for callers that
+ * cannot allocate device memory (tight training loops, code under graph
+ * capture, harnesses that forbid hipMalloc). The crop offsets are staged in a
+ * static device array of RFK_NOALLOC_MAX_CROPS triples (48 KiB, held by each
+ * GPU the library runs on until the process ends) instead of a per-call
+ * allocation; more crops are refused (RFK_ERR_INVALID). Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (from other threads
+ * or on other streams) race. Returns when the crops are in dst.             */
+#define RFK_NOALLOC_MAX_CROPS 4096
+int rfk_nn_gather_crops_noalloc(const void* src, int n_images, int H, int W, int pixel_bytes, const int* offsets,
+                                int n, int ch, int cw, void* dst, void* stream);
 /* error-path tests: the next `count` activation allocations of any denoiser
  * fail as hipMalloc failing would (RFK_ERR_HIP, nothing launched)           */
 int rfk_nn_inject_alloc_failures(int count);

@@ -159,6 +159,8 @@ _SIGS = [
     ("rfk_nn_gather_crops", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, C.c_int, C.c_int, C.c_int, _P,
                                       _P]),
     ("rfk_nn_inject_alloc_failures", C.c_int, [C.c_int]),
+    ("rfk_nn_gather_crops_noalloc", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, C.c_int, C.c_int,
+                                              C.c_int, _P, _P]),
     ("rfk_denoise", C.c_int, [_P, _P, C.c_int, _P, C.c_int, C.c_uint32, C.c_uint32, _P]),
     ("rfk_atomic_peak", C.c_int, [C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]),
     ("rfk_device_count", C.c_int, [C.POINTER(C.c_int)]),

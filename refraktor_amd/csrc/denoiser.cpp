@@ -7,12 +7,14 @@
 // Adam), so a frame renders with a fraction of the iterations.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstring>
 #include <fstream>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -636,6 +638,44 @@ int rfk_nn_gather_crops(const void* src, int n_images, int H, int W, int pixel_b
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(d_off);
     if (e != hipSuccess) return fail(RFK_ERR_HIP, hipGetErrorString(e));
+    return RFK_OK;
+}
+
+// Staging for rfk_nn_gather_crops_noalloc: the crop offsets of one call, in
+// a file-scope device array (each GPU holds its own copy until the process
+// ends). Host code reaches it only through the address hipGetSymbolAddress
+// returns for the current device, looked up once per device.
+static __device__ __align__(256) int scratch[3 * RFK_NOALLOC_MAX_CROPS];
+
+int rfk_nn_gather_crops_noalloc(const void* src, int n_images, int H, int W, int pixel_bytes, const int* offsets,
+                                int n, int ch, int cw, void* dst, void* stream) {
+    static std::mutex mu;
+    static std::array<void*, 64> dev_ptr{};
+    if (!src || !offsets || !dst || n <= 0 || pixel_bytes <= 0 || n_images <= 0)
+        return fail(RFK_ERR_INVALID, "bad argument");
+    if (n > RFK_NOALLOC_MAX_CROPS)
+        return fail(RFK_ERR_INVALID, "more crops than the staging array holds (RFK_NOALLOC_MAX_CROPS)");
+    for (int k = 0; k < n; k++)
+        if (offsets[3 * k] < 0 || offsets[3 * k] >= n_images || offsets[3 * k + 1] < 0 || offsets[3 * k + 2] < 0 ||
+            offsets[3 * k + 1] + ch > H || offsets[3 * k + 2] + cw > W)
+            return fail(RFK_ERR_INVALID, "crop outside the image stack");
+    int dev = 0;
+    NN_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= int(dev_ptr.size())) return fail(RFK_ERR_INVALID, "device index out of range");
+    void* ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (!dev_ptr[size_t(dev)]) {
+            void* p = nullptr;
+            NN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            dev_ptr[size_t(dev)] = p;
+        }
+        ptr = dev_ptr[size_t(dev)];
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    NN_HIP(hipMemcpyAsync(ptr, offsets, size_t(n) * 3 * sizeof(int), hipMemcpyHostToDevice, s));
+    NN_HIP(rfk::nn::gather_crops(src, H, W, pixel_bytes, static_cast<const int*>(ptr), n, ch, cw, dst, s));
+    NN_HIP(hipStreamSynchronize(s));
     return RFK_OK;
 }
 
