@@ -1,5 +1,6 @@
 """Pins the oracle itself: published vectors, closed forms and invariants."""
 import math
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -299,3 +300,81 @@ def test_jpeg_oracle_stress_cases():
     np.testing.assert_allclose(_decode(oracle.jpeg_encode(one, 95))[0, 0], one[0, 0], atol=3)
     gray = np.full((9, 17, 3), 128, np.uint8)  # mid grey: every coefficient zero
     np.testing.assert_array_equal(_decode(oracle.jpeg_encode(gray, 50)), gray)
+
+
+# ------------------------------------------------ independent pins (restated
+# functions checked against textbook forms or hand-computed values)
+
+def test_hsv_to_rgb8_matches_colorsys():
+    """refactor.cu:119-151 f(n) = v - v s max(0, min(k, 4 - k, 1)) with
+    k = fmod(n + h / 60, 6) is the textbook HSV -> RGB (colorsys) for hues in
+    [0, 360); bytes are trunc(255 c), R in the low byte, alpha 255"""
+    import colorsys
+    rng = np.random.default_rng(4)
+    h, s, v = rng.uniform(0, 360, 2000), rng.uniform(0, 1, 2000), rng.uniform(0, 1, 2000)
+    packed = oracle.hsv_to_rgb8(h, s, v)
+    got = np.stack([(packed >> sh) & 0xFF for sh in (0, 8, 16, 24)], axis=1)
+    want = np.array([colorsys.hsv_to_rgb(a / 360.0, b, c) for a, b, c in zip(h, s, v)]) * 255.0
+    near_int = np.abs(want - np.round(want)) < 1e-6
+    d = got[:, :3].astype(np.int64) - np.floor(want).astype(np.int64)
+    assert np.all((d == 0) | (near_int & (np.abs(d) <= 1)))
+    assert np.all(got[:, 3] == 255)
+    # primaries, grey, and a negative hue (C fmod keeps the sign: -60 wraps
+    # to magenta here, but below -300 a channel's k goes negative and the
+    # channel stays at v, as the reference computes it)
+    cases = {(0.0, 1.0, 1.0): (255, 0, 0), (120.0, 1.0, 1.0): (0, 255, 0), (240.0, 1.0, 1.0): (0, 0, 255),
+             (77.0, 0.0, 0.5): (127, 127, 127), (-60.0, 1.0, 1.0): (255, 0, 255), (-400.0, 1.0, 1.0): (255, 255, 255)}
+    for (hh, ss, vv), rgb in cases.items():
+        p = int(oracle.hsv_to_rgb8(np.array([hh]), np.array([ss]), np.array([vv]))[0])
+        assert (p & 0xFF, (p >> 8) & 0xFF, (p >> 16) & 0xFF) == rgb, (hh, ss, vv)
+
+
+def test_density_histogram_hand_computed():
+    """density_histo.cu:1-41: bin = trunc(ln(v) * 4 / ln 2) + 1 for a density
+    v >= 1 draw (v = W / 255), else bin 0; max = trunc(max v)"""
+    W = np.array([0, 127, 255, 383, 765, 255000], dtype=np.uint32)
+    # v: 0, 0.498, 1, 1.502 (log2 0.587 -> 2.35), 3 (log2 1.585 -> 6.34), 1000 (log2 9.966 -> 39.86)
+    hist = np.zeros((1, W.size, 4), np.uint32)
+    hist[0, :, 3] = W
+    bins, mx = oracle.density_histogram(hist)
+    want = np.zeros(97, np.uint64)
+    for b, n in {0: 2, 1: 1, 3: 1, 7: 1, 40: 1}.items():
+        want[b] = n
+    assert np.array_equal(bins, want)
+    assert mx == 1000
+
+
+def test_precalc_lambdas_match_the_database_programs():
+    """the oracle's precalc of waves, julian and perspective (used by its
+    warmup) against the database's precalc programs run by the interpreter,
+    which the golden vectors pin to librefrakt's"""
+    from oracle import flang_eval as fe
+    from refraktor_amd import render
+    cfg = Path(render.CONFIG_DIR)
+    mdb, _ = fe.load_db((cfg / "variations_new.yml").read_text(), (cfg / "common.yml").read_text())
+    rng = np.random.default_rng(8)
+    for trial in range(20):
+        aff = rng.uniform(-1.5, 1.5, 6)
+        user = {"power": float(rng.choice([-3.0, -2.0, 2.0, 5.0]) * rng.uniform(0.5, 1.5)),
+                "dist": float(rng.uniform(-2, 2)), "angle": float(rng.uniform(-3, 3))}
+        f = fl.make_flame([fl.xform(1.0, 0.0, 0.5, [fl.vlink(list(aff), {
+            "waves": 1.0,
+            "julian": {"weight": 1.0, "parameters": {"power": user["power"], "dist": user["dist"]}},
+            "perspective": {"weight": 1.0, "parameters": {"angle": user["angle"], "dist": user["dist"]}}})])],
+            [[0.0, 0.0, 1.0]] * 256)
+        L = oracle.Layout(f, oracle.load_variation_params(cfg))
+        P = np.zeros((1, L.size_reals + 8))
+        vs = L.xforms[0]["vlinks"][0]
+        P[0, vs["aff"]:vs["aff"] + 6] = aff
+        for slot in vs["vars"]:
+            for pn in slot["params"]:
+                if pn in user:
+                    P[0, slot["params"][pn]] = user[pn]
+        got = oracle.precalc(P, L, 1, 0)[0]
+        a, d, b, e, c, ff = aff  # (a, d, b, e, c, f) order in the layout
+        affd = {"a": a, "b": b, "c": c, "d": d, "e": e, "f": ff}
+        for slot in vs["vars"]:
+            v = mdb[slot["name"]]
+            want = fe.precalc(v, {pn: P[0, slot["params"][pn]] for pn in v.user_params}, affd)
+            for pn, off in slot["params"].items():
+                assert abs(got[off] - want[pn]) <= 1e-12 * max(1.0, abs(want[pn])), (slot["name"], pn)
