@@ -79,9 +79,56 @@ def load_variation_params(config_dir: str | Path) -> dict:
 
 
 def _anima(v, t=0.0) -> float:
-    if isinstance(v, dict):
-        return float(v["t0"])
-    return float(v)
+    """an animated value at time t (anima::sample): its initial value t0,
+    through its call when it has one"""
+    if not isinstance(v, dict):
+        return float(v)
+    iv = float(v["t0"])
+    name = v.get("call")
+    return _anima_call(name, t, iv, v.get("args", {}), "") if name else iv
+
+
+def _anima_call(name, t, iv, args, prefix) -> float:
+    """`increase` and `sine` as the reference's front end registers them
+    (src/refraktor/src/refraktor.cpp:351-367), `mix` the built-in quintic
+    smoothstep cross-fade (src/librefrakt/src/anima.cpp:35-105); anything
+    missing or unknown gives the initial value, as the reference does after
+    logging an error"""
+    def num(k, d=None):
+        v = args.get(prefix + k, d)
+        return float(v) if isinstance(v, (int, float)) and not isinstance(v, bool) else d
+    if name == "increase":
+        return iv + t * num("per_loop", 360.0)
+    if name == "sine":
+        v = math.sin(t * num("frequency", 1.0) * math.pi * 2.0 + math.radians(num("phase", 0.0)))
+        sharp = num("sharpness", 0.0)
+        if sharp > 0:
+            v = math.copysign(1.0, v) * abs(v) ** sharp
+        if args.get(prefix + "absolute", False) is True:
+            v = abs(v)
+        return iv + v * num("amplitude", 1.0)
+    if name == "mix":
+        left = iv
+        lf = args.get(prefix + "left.function")
+        if isinstance(lf, str):
+            left = _anima_call(lf, t, iv, args, prefix + "left.")
+        right = num("right.value")
+        if right is None:
+            return iv
+        rf = args.get(prefix + "right.function")
+        if isinstance(rf, str):
+            right = _anima_call(rf, t, right, args, prefix + "right.")
+        start, length = num("start_time"), num("length")
+        if start is None or length is None:
+            return iv
+        if t <= start:
+            return left
+        if t >= start + length:
+            return right
+        m = (t - start) / length
+        m = 6 * m ** 5 - 15 * m ** 4 + 10 * m ** 3
+        return left * (1 - m) + right * m
+    return iv
 
 
 class Layout:
@@ -153,61 +200,66 @@ def _translated(a, x, y):
     return [a[0], a[1], a[2], a[3], a[0] * x + a[2] * y + a[4], a[1] * x + a[3] * y + a[5]]
 
 
-def screen_space(flame, w, h):
+def screen_space(flame, w, h, t=0.0):
     aff = [1.0, 0.0, 0.0, 1.0, 0.0, 0.0]
     aff = _translated(aff, w / 2.0, h / 2.0)
-    aff = _scaled(aff, _anima(flame["scale"]) * h)
-    aff = _rotated(aff, _anima(flame["rotate"]))
-    return _translated(aff, -_anima(flame["center_x"]), -_anima(flame["center_y"]))
+    aff = _scaled(aff, _anima(flame["scale"], t) * h)
+    aff = _rotated(aff, _anima(flame["rotate"], t))
+    return _translated(aff, -_anima(flame["center_x"], t), -_anima(flame["center_y"], t))
 
 
-def plane_space(flame, w, h):
+def plane_space(flame, w, h, t=0.0):
     aff = [1.0, 0.0, 0.0, 1.0, 0.0, 0.0]
-    aff = _translated(aff, _anima(flame["center_x"]), _anima(flame["center_y"]))
-    aff = _rotated(aff, -_anima(flame["rotate"]))
-    aff = _scaled(aff, 1.0 / (_anima(flame["scale"]) * h))
+    aff = _translated(aff, _anima(flame["center_x"], t), _anima(flame["center_y"], t))
+    aff = _rotated(aff, -_anima(flame["rotate"], t))
+    aff = _scaled(aff, 1.0 / (_anima(flame["scale"], t) * h))
     return _translated(aff, w / -2.0, h / -2.0)
 
 
-def pack_sample(flame: dict, L: Layout, w: int, h: int) -> np.ndarray:
-    """one packed sample (static flames only: anima calls are not evaluated here)"""
+def pack_samples(flame: dict, L: Layout, w: int, h: int, start: float, offset: float, count: int) -> np.ndarray:
+    """flame::pack_samples (flame_types.h:707-712): samples at start + offset * i"""
+    return np.concatenate([pack_sample(flame, L, w, h, start + offset * i) for i in range(count)])
+
+
+def pack_sample(flame: dict, L: Layout, w: int, h: int, t: float = 0.0) -> np.ndarray:
+    """one packed sample at time t (flame_types.h:634-672)"""
     out = np.zeros(L.size_reals + 768)
-    out[0:6] = screen_space(flame, w, h)
-    out[6:12] = plane_space(flame, w, h)
+    out[0:6] = screen_space(flame, w, h, t)
+    out[6:12] = plane_space(flame, w, h, t)
     if L.chaos:
         n = L.n_xforms
         for i, row in enumerate(flame["chaos"]):
             base = L.chaos_base + i * (n + 1)
             for j, v in enumerate(row):
-                out[base + 1 + j] = _anima(v)
+                out[base + 1 + j] = _anima(v, t)
     xfs = flame["xforms"] + ([flame["final_xform"]] if L.has_final else [])
     for xf, xs in zip(xfs, L.xforms):
-        out[xs["weight"]] = _anima(xf["weight"])
-        out[xs["color"]] = _anima(xf["color"])
-        out[xs["color_speed"]] = _anima(xf["color_speed"])
-        out[xs["opacity"]] = _anima(xf["opacity"])
+        out[xs["weight"]] = _anima(xf["weight"], t)
+        out[xs["color"]] = _anima(xf["color"], t)
+        out[xs["color_speed"]] = _anima(xf["color_speed"], t)
+        out[xs["opacity"]] = _anima(xf["opacity"], t)
         for vl, vs in zip(xf["vchain"], xs["vlinks"]):
-            a = [_anima(v) for v in vl["transform"]]
-            a = _scaled(a, _anima(vl["mod_scale"]))
-            a = _rotated(a, _anima(vl["mod_rotate"]))
-            a = _translated(a, _anima(vl["mod_x"]), _anima(vl["mod_y"]))
+            a = [_anima(v, t) for v in vl["transform"]]
+            a = _scaled(a, _anima(vl["mod_scale"], t))
+            a = _rotated(a, _anima(vl["mod_rotate"], t))
+            a = _translated(a, _anima(vl["mod_x"], t), _anima(vl["mod_y"], t))
             out[vs["aff"]:vs["aff"] + 6] = a
             for slot in vs["vars"]:
                 val = vl["variations"][slot["name"]]
                 user, pre, dflt = L.vparams[slot["name"]]
                 if isinstance(val, dict) and "weight" in val:
-                    out[slot["weight"]] = _anima(val["weight"])
+                    out[slot["weight"]] = _anima(val["weight"], t)
                     given = val.get("parameters", {})
                 else:
-                    out[slot["weight"]] = _anima(val)
+                    out[slot["weight"]] = _anima(val, t)
                     given = {}
                 for pn in user:
-                    out[slot["params"][pn]] = _anima(given.get(pn, dflt.get(pn, 0.0)))
+                    out[slot["params"][pn]] = _anima(given.get(pn, dflt.get(pn, 0.0)), t)
     base = L.size_reals
     for j, (hh, ss, vv) in enumerate(flame["palette"]):
-        out[base + 3 * j] = math.fmod(hh + _anima(flame["mod_hue"]), 360.0)
-        out[base + 3 * j + 1] = min(max(ss + _anima(flame["mod_sat"]), 0.0), 1.0)
-        out[base + 3 * j + 2] = min(max(vv + _anima(flame["mod_val"]), 0.0), 1.0)
+        out[base + 3 * j] = math.fmod(hh + _anima(flame["mod_hue"], t), 360.0)
+        out[base + 3 * j + 1] = min(max(ss + _anima(flame["mod_sat"], t), 0.0), 1.0)
+        out[base + 3 * j + 2] = min(max(vv + _anima(flame["mod_val"], t), 0.0), 1.0)
     return out
 
 

@@ -45,15 +45,23 @@ double call_function(const std::string& name, double t, double iv, const std::ma
         if (auto it = args.find(prefix + "left.function"); it != args.end()) {
             if (auto s = std::get_if<std::string>(&it->second)) left = call_function(*s, t, iv, args, prefix + "left.");
         }
-        auto rv = args.find(prefix + "right.value");
-        if (rv == args.end()) return iv;
-        double right = std::holds_alternative<double>(rv->second) ? std::get<double>(rv->second) : iv;
+        // a missing or non-numeric right value, start time or length makes
+        // the reference log an error and return the initial value
+        auto number = [&](const std::string& key, double& out) {
+            auto it = args.find(prefix + key);
+            if (it == args.end()) return false;
+            if (auto d = std::get_if<double>(&it->second)) out = *d;
+            else if (auto i = std::get_if<int>(&it->second)) out = double(*i);
+            else return false;
+            return true;
+        };
+        double right = 0.0, start = 0.0, length = 0.0;
+        if (!number("right.value", right)) return iv;
         if (auto it = args.find(prefix + "right.function"); it != args.end()) {
             if (auto s = std::get_if<std::string>(&it->second))
                 right = call_function(*s, t, right, args, prefix + "right.");
         }
-        double start = arg_double(args, prefix + "start_time", 0.0);
-        double length = arg_double(args, prefix + "length", 0.0);
+        if (!number("start_time", start) || !number("length", length)) return iv;
         if (t <= start) return left;
         if (t >= start + length) return right;
         double m = (t - start) / length;

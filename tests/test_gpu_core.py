@@ -413,3 +413,36 @@ def test_density_estimate_paths_agree(small):
     keep = ~dense
     np.testing.assert_array_equal(outs[0][keep], outs[1][keep])
     assert (outs[0][keep] != 0).any()
+
+
+def test_animated_flame_json_matches_oracle(db, gpu):
+    """a flame JSON with increase / sine / mix animations (rotation, scale,
+    xform weight and colour, vlink rotation, affine term, variation weight and
+    parameter), packed at four times by the library: the per-block parameters
+    after warmup equal the oracle's packing, Catmull-Rom and precalc of the
+    same times, and one-step trajectories with those per-block parameters
+    keep the stated bound"""
+    from test_cpu_anima import animated_flame
+    f = animated_flame()
+    fo = db.flame(f)
+    k = db.compile(fo)
+    samples = fo.pack_samples(W, H, 0.0, 0.25, 4)
+    st = k.warmup(samples, W, H, seed=6, fuse=3, walkers=64 * 256)
+    inf = st.info()
+    js = fo.to_json()
+    L = oracle.Layout(js, oracle.load_variation_params(render.CONFIG_DIR))
+    osamp = oracle.pack_samples(f, L, W, H, 0.0, 0.25, 4)
+    np.testing.assert_allclose(samples, osamp, rtol=1e-12, atol=1e-12)
+    ss = L.size_reals + 768
+    fixed = oracle.fix_rotation(osamp.copy(), ss, L.size_reals, L.affine_indices, 4)
+    P, pals = oracle.eval_samples(oracle.catmull(fixed, ss), L, inf.blocks)
+    P = oracle.precalc(P, L, 6, 0)
+    n = L.size_reals
+    np.testing.assert_allclose(st.params()[:, :n], P[:, :n], rtol=2e-6, atol=2e-5)
+    assert np.array_equal(st.palettes(), pals)
+    assert not np.allclose(st.params()[0, 12:n], st.params()[-1, 12:n])   # blocks see different times
+    x0 = st.walkers()
+    tr, xid = st.trace(6)
+    r = _bound(js, st, x0, tr, xid, 3, 0, K_F32)
+    assert r["bad"] == 0 and r["wrong_reset"] == 0 and r["missed_reset"] == 0, r
+    assert r["knife_steps"] <= 0.01 * r["steps"], r
