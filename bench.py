@@ -101,7 +101,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--tpb", type=int, default=512)
+    ap.add_argument("--tpb", type=int, default=512, help="plain path walkers per workgroup (without autotuning)")
+    ap.add_argument("--den-tpb", type=int, default=512, help="denoised path walkers per workgroup (queue binning)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="plain path at --tpb / --binning / --walkers instead of the autotuned shape")
     ap.add_argument("--binning", choices=("queue", "atomic"), default="queue",
                     help="queue: records sorted by screen tile and summed in LDS; atomic: LDS bin cache + atomics")
     ap.add_argument("--tile", type=int, default=12, help="LDS bin cache slots (log2) of atomic binning, 0 = off")
@@ -122,6 +125,10 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only three 1 GiB device-to-device copies: the known-size traffic that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of the bin call into bytes")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter runs that measure the bin call's HBM traffic")
+    ap.add_argument("--tune-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py runs on one GPU; --gpus above 1 is not supported",
@@ -138,6 +145,20 @@ def main() -> int:
         print(json.dumps({"copy_calibration": {"bytes_per_copy": n, "copies": 3}}))
         return 0
 
+    # The bin call's launch shape (autotuned) and its HBM traffic (rocprofv3
+    # counters) come from child processes started before this process
+    # touches the GPU: the counter runs need the profiler around a process
+    # of their own.
+    tuned = traffic = None
+    if not (args.tune_only or args.traffic_probe):
+        if not args.no_autotune:
+            tuned = run_child(["--tune-only", "--seed", str(args.seed)])["autotune"]
+            sh = tuned["shape"]
+            args.binning, args.tpb, args.tile = sh["binning"], sh["threads_per_block"], sh["bin_tile_log2"]
+            args.atomic_width, args.walkers = sh["atomic_width"], sh["walkers"]
+            args.no_autotune = True
+        traffic = {"note": "not measured (--no-traffic)"} if args.no_traffic else live_traffic(args)
+
     import ctypes as C
 
     import numpy as np
@@ -152,12 +173,37 @@ def main() -> int:
     DE_R = 9.0
     f = fl.flame_1080(args.seed)
     db = render.VariationDB()
+    if args.traffic_probe:
+        return traffic_probe(args, db, f, W, H, ITERS)
+    if not args.no_autotune:
+        # the plain path's launch shape from the autotuner: one 2^28-iteration
+        # bin call of this flame per candidate, device time
+        from refraktor_amd import autotune
+        grid = ([autotune.Shape(tpb, 64, 0, w, 1) for tpb in (512, 1024) for w in (1 << 20, 1 << 21)] +
+                [autotune.Shape(1024, 64, 12, 1 << 20, 0)])
+        fo = db.flame(f)
+        t0 = time.perf_counter()
+        shape, results = autotune.tune(db, fo, fo.pack_samples(W, H), W, H, grid, 1 << 28, use_cache=False)
+        args.binning = "queue" if shape.bin_queue else "atomic"
+        args.tpb, args.tile, args.atomic_width, args.walkers = (shape.threads_per_block, shape.bin_tile_log2,
+                                                                shape.atomic_width, shape.walkers)
+        tuned = {"shape": {"binning": args.binning, "threads_per_block": args.tpb, "bin_tile_log2": args.tile,
+                           "atomic_width": args.atomic_width, "walkers": args.walkers},
+                 "candidates": [{"binning": "queue" if r["bin_queue"] else "atomic",
+                                 "threads_per_block": r["threads_per_block"], "bin_tile_log2": r["bin_tile_log2"],
+                                 "walkers": r["walkers"], "iters_per_s": r["iters_per_s"]} for r in results],
+                 "tune_s": time.perf_counter() - t0}
+        if args.tune_only:
+            print(json.dumps({"autotune": tuned}), flush=True)
+            return 0
     queue = args.binning == "queue"
     if queue:
         args.tile = 0
     k = db.compile(f, threads_per_block=args.tpb, atomic_width=args.atomic_width, bin_tile_log2=args.tile,
                    hist_bits=args.hist_bits, bin_queue=queue)
     k64 = db.compile(f, threads_per_block=args.tpb, bin_tile_log2=args.tile, hist_bits=64, bin_queue=queue)
+    # the denoised path's small budget: queue binning at its own shape
+    kd = db.compile(f, threads_per_block=args.den_tpb, atomic_width=64, hist_bits=32, bin_queue=True)
     samples = k.pack_samples(W, H)
     gamma, bright, vib = f["gamma"], f["brightness"], f["vibrancy"]
     lib = _lib.lib()
@@ -206,7 +252,7 @@ def main() -> int:
             check(lib.rfk_tonemap(st.handle, out.ptr, fmt, C.byref(ta)))  # the histogram itself
         return st, res, out
 
-    def denoised(lane: Lane, seed: int, kern=k, smp=samples, tone=(gamma, bright, vib)):
+    def denoised(lane: Lane, seed: int, kern=kd, smp=samples, tone=(gamma, bright, vib)):
         st, res, tm = plain(lane, seed, kern, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers,
                             args.low_de_radius, smp, tone)
         inf = st.info()
@@ -290,7 +336,11 @@ def main() -> int:
         peak_inc = v.value
     except Exception:
         pass
-    traffic = measured_traffic(args, queue)
+    if traffic and traffic.get("bytes_per_bin_call"):
+        # the counter run's bytes per draw times this run's draws per call
+        traffic_bytes = traffic["bytes_per_bin_call"] / traffic["draws"] * dr
+    else:
+        traffic_bytes = None
 
     cpu = None
     if not args.no_cpu:
@@ -333,6 +383,7 @@ def main() -> int:
                   "denoised": None if fps_den is None else
                   {"fps": fps_den, "ms": ms_den, "sequential_ms": seq_den, "iterations": args.low_iters,
                    "walkers": args.low_walkers, "de_radius": args.low_de_radius,
+                   "threads_per_block": args.den_tpb, "binning": "queue", "hist_bits": 32,
                    "psnr_db": psnr_den, "fallback_frames": fallbacks,
                    "fps_with_fallback": None if eff_ms_den is None else 1e3 / eff_ms_den,
                    "speedup_vs_plain": (1e3 / eff_ms_den if eff_ms_den else fps_den) / fps_plain,
@@ -344,12 +395,13 @@ def main() -> int:
         "psnr_db_vs_converged": psnr_den if den_ok else psnr_plain,
         "roofline_denoiser": unet,
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic_bytes, "traffic_measurement": traffic,
                      "kernel": "rfk_iterate" + (" + queue sort/bin kernels" if queue else ""),
                      "unit_of_work": "one bin call of 2^30 iterations", "bytes_per_draw": 16,
                      "draws_per_launch": dr, "launch_ms": kms,
                      "atomic_peak_increments_per_s": peak_inc},
         "cpu_baseline": cpu,
+        "autotune": tuned,
     }
     print(json.dumps(line), flush=True)
     return 0
@@ -391,8 +443,7 @@ def held_out_flames(args, db, lanes, plain, denoised, score, summarize) -> dict:
     frames = []
     for s in HELD_OUT_SEEDS:
         f = fl.flame_1080(s)
-        kern = db.compile(f, threads_per_block=args.tpb, atomic_width=args.atomic_width, bin_tile_log2=args.tile,
-                          hist_bits=args.hist_bits, bin_queue=queue)
+        kern = db.compile(f, threads_per_block=args.den_tpb, atomic_width=64, hist_bits=32, bin_queue=True)
         k64 = db.compile(f, threads_per_block=args.tpb, bin_tile_log2=args.tile, hist_bits=64, bin_queue=queue)
         smp = kern.pack_samples(W, H)
         tone = (f["gamma"], f["brightness"], f["vibrancy"])
@@ -415,22 +466,105 @@ def held_out_flames(args, db, lanes, plain, denoised, score, summarize) -> dict:
     return out
 
 
-def measured_traffic(args, queue: bool):
-    """HBM bytes of one 2^30-iteration bin call from the committed rocprofv3
-    counter summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated against
-    a plain device copy), when it was measured with this launch shape"""
-    path = ROOT / "profiles" / "traffic_1080p_bin.json"
-    try:
-        t = json.loads(path.read_text())
-    except (OSError, ValueError):
-        return None
-    shape = {"binning": args.binning, "threads_per_block": args.tpb, "walkers": args.walkers,
-             "hist_bits": args.hist_bits, "atomic_width": args.atomic_width}
-    if not queue:
-        shape["bin_tile_log2"] = args.tile
-    if any(t.get("shape", {}).get(k) != v for k, v in shape.items()):
-        return None
-    return t.get("bytes_per_bin_call")
+def shape_args(args) -> list[str]:
+    return ["--binning", args.binning, "--tpb", str(args.tpb), "--tile", str(args.tile), "--atomic-width",
+            str(args.atomic_width), "--walkers", str(args.walkers), "--hist-bits", str(args.hist_bits),
+            "--seed", str(args.seed)]
+
+
+def run_child(extra: list[str], timeout: float = 600.0) -> dict:
+    """python3 bench.py <extra> as a child process; its last stdout line as JSON"""
+    import subprocess
+    r = subprocess.run([sys.executable, "-u", str(ROOT / "bench.py"), *extra], capture_output=True, text=True,
+                       timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"bench child {extra} failed ({r.returncode}):\n{r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def traffic_probe(args, db, f, W, H, iters) -> int:
+    """under rocprofv3 --pmc: warmup, a 1 GiB copy (marker and calibration),
+    one bin call of the timed shape, two more copies"""
+    from refraktor_amd import _lib, render
+    lib = _lib.lib()
+    queue = args.binning == "queue"
+    k = db.compile(f, threads_per_block=args.tpb, atomic_width=args.atomic_width,
+                   bin_tile_log2=0 if queue else args.tile, hist_bits=args.hist_bits, bin_queue=queue)
+    st = k.warmup(k.pack_samples(W, H), W, H, seed=args.seed, fuse=20, walkers=args.walkers)
+    n = 1 << 30
+    src, dst = render.DeviceBuffer(n), render.DeviceBuffer(n)
+    src.zero()
+    _lib.check(lib.rfk_synchronize())
+    _lib.check(lib.rfk_memcpy_dtod(dst.ptr, src.ptr, n))
+    _lib.check(lib.rfk_synchronize())
+    res = st.bin(iters=iters)
+    for _ in range(2):
+        _lib.check(lib.rfk_memcpy_dtod(dst.ptr, src.ptr, n))
+    _lib.check(lib.rfk_synchronize())
+    print(json.dumps({"draws": res["total_draws"], "copy_bytes": n}), flush=True)
+    return 0
+
+
+def _counter_rows(path: Path) -> list[tuple[int, str, float]]:
+    """(dispatch, kernel, value) rows of a rocprofv3 counter-collection csv"""
+    import csv
+    rows = []
+    for fpath in path.rglob("*counter_collection.csv"):
+        with open(fpath, newline="") as fh:
+            for r in csv.DictReader(fh):
+                rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return rows
+
+
+def live_traffic(args) -> dict:
+    """HBM bytes of one bin call of the timed launch shape: FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) over the dispatches between the probe's
+    first and second 1 GiB copy, scaled by what the same counters read for
+    those copies (bytes per KiB)"""
+    import shutil
+    import subprocess
+    import tempfile
+    if not shutil.which("rocprofv3"):
+        return {"note": "rocprofv3 not found: traffic not measured"}
+    out = {"shape": {"binning": args.binning, "threads_per_block": args.tpb, "walkers": args.walkers,
+                     "hist_bits": args.hist_bits, "atomic_width": args.atomic_width}}
+    with tempfile.TemporaryDirectory() as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = Path(tmp) / counter
+            cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", str(d), "-o", "run", "--",
+                   sys.executable, str(ROOT / "bench.py"), "--traffic-probe", *shape_args(args)]
+            try:
+                r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+            except subprocess.TimeoutExpired:
+                return {**out, "note": f"rocprofv3 {counter} run timed out"}
+            lines = r.stdout.strip().splitlines()
+            probe = None
+            for line in reversed(lines):
+                if line.startswith("{"):
+                    probe = json.loads(line)
+                    break
+            rows = _counter_rows(d)
+            # rocprofv3 can exit with an error after a complete run: the
+            # probe's own last line and the counter file decide
+            if probe is None or not rows:
+                return {**out, "note": f"rocprofv3 {counter} run gave no counters (exit {r.returncode})"}
+            # the probe's 1 GiB copies (the bin call makes small copies of its own)
+            copies = [i for i, (_, name, v) in enumerate(rows) if "copyBuffer" in name and v > 65536.0]
+            if len(copies) != 3:
+                return {**out, "note": f"{counter}: the probe's copies were not found"}
+            kib_copy = [rows[i][2] for i in copies[:3]]
+            scale = probe["copy_bytes"] / (sum(kib_copy) / 3.0)
+            kib_bin = sum(v for _, name, v in rows[copies[0] + 1:copies[1]])
+            key = "read" if counter == "FETCH_SIZE" else "write"
+            out[f"{key}_bytes"] = kib_bin * scale
+            out[f"{key}_scale_bytes_per_kib"] = scale
+            out["draws"] = probe["draws"]
+    out["bytes_per_bin_call"] = out["read_bytes"] + out["write_bytes"]
+    out["bytes_per_draw"] = out["bytes_per_bin_call"] / out["draws"]
+    out["method"] = ("rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over the dispatches of one bin call of this "
+                     "shape in a child process, each scaled by the counter's reading of a 1 GiB device copy")
+    return out
 
 
 def cpu_baseline(f: dict, W: int, H: int, seed: int, it_per_frame: int) -> dict:

@@ -13,6 +13,7 @@ import math
 
 import numpy as np
 
+from . import autotune
 from . import flame as fl
 from . import render
 
@@ -55,6 +56,24 @@ class Animation:
         return self.batch.run(graph)
 
 
+def batch_grid(walkers: int) -> list:
+    """launch shapes a batch can take: queue binning at 256, 512 and 1024
+    walkers per workgroup, and atomic binning behind the LDS bin cache"""
+    return ([autotune.Shape(tpb, 64, 0, walkers, 1) for tpb in (256, 512, 1024)] +
+            [autotune.Shape(1024, 64, 12, walkers, 0)])
+
+
+def tune_shape(db: render.VariationDB, keyframes: list, width: int, height: int, walkers: int,
+               probe_iters: int = 1 << 26, use_cache: bool = True):
+    """the fastest batch shape for these keyframes, measured by the
+    autotuner on one frame of the aligned structure (its first segment) at
+    the batch's size and walkers per frame"""
+    aligned, samples = control_samples(db, keyframes, width, height)
+    ss = samples.size // (len(keyframes) + 2)
+    return autotune.tune(db, aligned[0], samples[:4 * ss], width, height, batch_grid(walkers), probe_iters,
+                         use_cache=use_cache)
+
+
 def main(argv=None) -> int:
     """The animation workload: 4 keyframes of 6-10 xforms, 120 frames at
     1280x720, budgets log-uniform in [2^26, 2^29], queue-binned chunk
@@ -76,16 +95,32 @@ def main(argv=None) -> int:
                     help="LDS bin cache slots (log2) of atomic binning, 0 = off (default 12)")
     ap.add_argument("--atomic-width", type=int, default=64)
     ap.add_argument("--replays", type=int, default=3)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use --binning / --tpb / --tile instead of the autotuned shape")
     args = ap.parse_args(argv)
+    db = render.VariationDB()
+    keyframes = keyframe_flames(args.keyframes, args.seed)
+    tuned = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        shape, results = tune_shape(db, keyframes, args.width, args.height, args.walkers, use_cache=False)
+        tuned = {"shape": {"binning": "queue" if shape.bin_queue else "atomic",
+                           "threads_per_block": shape.threads_per_block, "atomic_width": shape.atomic_width,
+                           "bin_tile_log2": shape.bin_tile_log2, "walkers": shape.walkers},
+                 "candidates": [{"binning": "queue" if r["bin_queue"] else "atomic",
+                                 "threads_per_block": r["threads_per_block"], "bin_tile_log2": r["bin_tile_log2"],
+                                 "iters_per_s": r["iters_per_s"], "status": r["status"]} for r in results],
+                 "tune_s": time.perf_counter() - t0}
+        args.binning = "queue" if shape.bin_queue else "atomic"
+        args.tpb, args.tile, args.atomic_width = shape.threads_per_block, shape.bin_tile_log2, shape.atomic_width
     queue = args.binning == "queue"
     if args.tpb is None:
         args.tpb = 512 if queue else 1024
     if args.tile is None:
         args.tile = 0 if queue else 12
-    db = render.VariationDB()
     budgets = frame_budgets(args.frames, args.seed)
     t0 = time.perf_counter()
-    anim = Animation(db, keyframe_flames(args.keyframes, args.seed), args.width, args.height, args.frames, budgets,
+    anim = Animation(db, keyframes, args.width, args.height, args.frames, budgets,
                      walkers=args.walkers, seed=args.seed, threads_per_block=args.tpb,
                      atomic_width=args.atomic_width, bin_tile_log2=args.tile, bin_queue=queue)
     setup_s = time.perf_counter() - t0
@@ -101,7 +136,8 @@ def main(argv=None) -> int:
                       "iterations_per_s": passes / (best * 1e-3),
                       "config": {"binning": args.binning, "walkers": args.walkers, "threads_per_block": args.tpb,
                                  "bin_tile_log2": args.tile, "grid": anim.batch.grid, "items": anim.batch.items,
-                                 "atomic_width": args.atomic_width}}), flush=True)
+                                 "atomic_width": args.atomic_width},
+                      "autotune": tuned}), flush=True)
     return 0
 
 

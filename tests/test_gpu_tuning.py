@@ -123,3 +123,26 @@ def test_autotune_picks_a_running_shape(gpu, db, tmp_path, monkeypatch):
     assert all(r["iters_per_s"] > 0 for r in results[:2])
     again, _ = autotune.tune(db, f, samples, W, H, grid=grid, probe_iters=1 << 22)
     assert again == shape  # served from the cache file
+
+
+def test_animation_autotuned_batch(gpu, db):
+    """the animation picks its launch shape with the autotuner (one frame of
+    the aligned keyframes at the batch's size and walkers per frame) and the
+    batch renders every frame at its budget with that shape"""
+    from refraktor_amd import animation
+    kf = animation.keyframe_flames(4, 3)
+    walkers = 1 << 14
+    shape, results = animation.tune_shape(db, kf, W, H, walkers, probe_iters=1 << 21, use_cache=False)
+    assert shape in animation.batch_grid(walkers)
+    assert all(r["status"] == 0 and r["iters_per_s"] > 0 for r in results)
+    assert sum(r["best"] for r in results) == 1
+    budgets = np.array([1 << 20, 3 << 18, 1 << 19, 5 << 18, 1 << 20, 1 << 18], dtype=np.uint64)
+    anim = animation.Animation(db, kf, W, H, len(budgets), budgets, walkers=walkers, seed=3,
+                               threads_per_block=shape.threads_per_block, atomic_width=shape.atomic_width,
+                               bin_tile_log2=shape.bin_tile_log2, bin_queue=bool(shape.bin_queue))
+    anim.render(graph=True)
+    for f in range(len(budgets)):
+        inf = anim.batch.frame(f)
+        assert int(budgets[f]) <= inf["total_passes"] < int(budgets[f]) + inf["walkers"]
+        h = anim.batch.histogram(f)
+        assert int(h[..., 3].sum(dtype=np.uint64)) == 255 * inf["total_draws"]
