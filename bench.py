@@ -127,6 +127,8 @@ def main() -> int:
                          "FETCH_SIZE / WRITE_SIZE counters of the bin call into bytes")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter runs that measure the bin call's HBM traffic")
+    ap.add_argument("--unet-only", action="store_true",
+                    help="time the U-Net forward alone (--steps forwards of one denoised frame's features)")
     ap.add_argument("--tune-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -150,7 +152,7 @@ def main() -> int:
     # touches the GPU: the counter runs need the profiler around a process
     # of their own.
     tuned = traffic = None
-    if not (args.tune_only or args.traffic_probe):
+    if not (args.tune_only or args.traffic_probe or args.unet_only):
         if not args.no_autotune:
             tuned = run_child(["--tune-only", "--seed", str(args.seed)])["autotune"]
             sh = tuned["shape"]
@@ -175,7 +177,7 @@ def main() -> int:
     db = render.VariationDB()
     if args.traffic_probe:
         return traffic_probe(args, db, f, W, H, ITERS)
-    if not args.no_autotune:
+    if not (args.no_autotune or args.unet_only):
         # the plain path's launch shape from the autotuner: one 2^28-iteration
         # bin call of this flame per candidate, device time
         from refraktor_amd import autotune
@@ -277,6 +279,14 @@ def main() -> int:
         lib.rfk_synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / steps
         return ms, float(np.mean(kms)), float(np.mean(draws)), float(np.mean(passes)), last
+
+    if args.unet_only:
+        if not use_den:
+            raise SystemExit(f"--unet-only: no trained weights at {weights}")
+        denoised(lanes[0], args.seed)
+        lib.rfk_synchronize()
+        print(json.dumps({"roofline_denoiser": unet_roofline(lanes[0], H, W, reps=args.steps)}), flush=True)
+        return 0
 
     ms_plain, kms, dr, ps, img_plain = timed(plain, args.steps, args.warmup)
     ms_den = img_den = None

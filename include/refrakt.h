@@ -385,6 +385,14 @@ int rfk_nn_conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
  * (x1_mode 1: x1 is H/2 x W/2) or a 2x2 average pool (2: x1 is 2H x 2W)  */
 int rfk_nn_conv3x3_forward_ex(const void* x1, int c1, int x1_mode, const void* x2, int c2, int N, int H, int W,
                               const void* wgt, const float* bias, int Cout, int act, void* y, float* yf, void* stream);
+/* as rfk_nn_conv3x3_forward_ex with BF16 y, and ypool = its 2x2 average
+ * pool (N x H/2 x W/2 x Cout BF16; even H and W, Cout a multiple of 8), as
+ * rfk_nn_avgpool2 computes it from y: the encoder's convolution + pool      */
+int rfk_nn_conv3x3_forward_pool(const void* x1, int c1, int x1_mode, const void* x2, int c2, int N, int H, int W,
+                                const void* wgt, const float* bias, int Cout, int act, void* y, void* ypool,
+                                void* stream);
+/* y = 2x2 average pool of x (N x H x W x C BF16, C a multiple of 8)       */
+int rfk_nn_avgpool2(const void* x, int N, int H, int W, int C, void* y, void* stream);
 int rfk_nn_conv3x3_wgrad(const void* x1, int c1, const void* x2, int c2, const void* dy, int Cout, int N, int H,
                          int W, float* dw, void* stream);
 int rfk_nn_conv3x3_dgrad(const void* dy, int Cout, int N, int H, int W, const void* wgt, int Cin, void* dx,

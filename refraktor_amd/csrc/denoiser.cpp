@@ -200,9 +200,9 @@ struct level_shape {
 
 // Forward pass; activations are kept under their names for the backward pass.
 // fused (inference): upsamples are read through by the next convolution
-// instead of being stored (the average pools stay separate: reading four
-// pixels per tap in the convolution's staging costs more than the pool);
-// training keeps every activation for the backward pass.
+// instead of being stored, and the average pools are written by the
+// convolutions that produce their inputs; training keeps every activation
+// for the backward pass.
 // With out_rgb (inference only), the output layer writes clamp(base + r)
 // directly (rout untouched) where the strip kernel takes its shape; the
 // return value 1 tells the caller that out_rgb is done.
@@ -214,11 +214,12 @@ int forward(rfk_denoiser* d, int N, int H, int W, const void* feat, float* rout,
         buf& b = act_buf(d, N, H, W, name, size_t(N) * L[lvl].H * L[lvl].W * C * 2, rc);
         return b.p;
     };
-    auto conv = [&](int li, const void* x1, const void* x2, int lvl, void* y, float* yf, int mode = 0) -> int {
+    auto conv = [&](int li, const void* x1, const void* x2, int lvl, void* y, float* yf, int mode = 0,
+                    void* ypool = nullptr) -> int {
         const conv_def& c = d->convs[size_t(li)];
         NN_HIP(rfk::nn::conv3x3_forward(x1, c.c1, x2, c.c2, N, L[lvl].H, L[lvl].W,
                                         static_cast<char*>(d->wbf) + c.w_off * 2, d->b + c.b_off, c.cout, c.act, 0.f,
-                                        y, yf, s, mode));
+                                        y, yf, s, mode, ypool));
         return RFK_OK;
     };
     const int* ch = d->ch;
@@ -231,15 +232,13 @@ int forward(rfk_denoiser* d, int N, int H, int W, const void* feat, float* rout,
         void *c1 = A("c1", 1, ch[1]), *d1 = A("d1", 1, ch[1]);
         void *c0 = A("c0", 0, ch[0]), *d0 = A("d0", 0, ch[0]);
         if (rc) return rc;
+        // the encoder's average pools come out of the convolutions' epilogues
         if ((rc = conv(0, feat, nullptr, 0, a0, nullptr))) return rc;
-        if ((rc = conv(1, a0, nullptr, 0, e0, nullptr))) return rc;
-        NN_HIP(rfk::nn::avgpool2_forward(e0, N, H, W, ch[0], p0, s));
+        if ((rc = conv(1, a0, nullptr, 0, e0, nullptr, 0, p0))) return rc;
         if ((rc = conv(2, p0, nullptr, 1, a1, nullptr))) return rc;
-        if ((rc = conv(3, a1, nullptr, 1, e1, nullptr))) return rc;
-        NN_HIP(rfk::nn::avgpool2_forward(e1, N, L[1].H, L[1].W, ch[1], p1, s));
+        if ((rc = conv(3, a1, nullptr, 1, e1, nullptr, 0, p1))) return rc;
         if ((rc = conv(4, p1, nullptr, 2, a2, nullptr))) return rc;
-        if ((rc = conv(5, a2, nullptr, 2, e2, nullptr))) return rc;
-        NN_HIP(rfk::nn::avgpool2_forward(e2, N, L[2].H, L[2].W, ch[2], p2, s));
+        if ((rc = conv(5, a2, nullptr, 2, e2, nullptr, 0, p2))) return rc;
         if ((rc = conv(6, p2, nullptr, 3, a3, nullptr))) return rc;
         if ((rc = conv(7, a3, nullptr, 3, bb, nullptr))) return rc;
         if ((rc = conv(8, bb, e2, 2, c2, nullptr, 1))) return rc;
@@ -593,6 +592,25 @@ int rfk_nn_conv3x3_forward_ex(const void* x1, int c1, int x1_mode, const void* x
     if (x1_mode < 0 || x1_mode > 2) return fail(RFK_ERR_INVALID, "x1_mode must be 0, 1 or 2");
     NN_HIP(rfk::nn::conv3x3_forward(x1, c1, x2, c2, N, H, W, wgt, bias, Cout, act, 0.f, y, yf,
                                     static_cast<hipStream_t>(stream), x1_mode));
+    return RFK_OK;
+}
+
+int rfk_nn_conv3x3_forward_pool(const void* x1, int c1, int x1_mode, const void* x2, int c2, int N, int H, int W,
+                                const void* wgt, const float* bias, int Cout, int act, void* y, void* ypool,
+                                void* stream) {
+    if (!x1 || !wgt || !y || !ypool) return fail(RFK_ERR_INVALID, "null argument");
+    if (x1_mode < 0 || x1_mode > 2) return fail(RFK_ERR_INVALID, "x1_mode must be 0, 1 or 2");
+    if ((H | W) & 1 || Cout % 8) return fail(RFK_ERR_INVALID, "pooling needs even H and W and Cout a multiple of 8");
+    NN_HIP(rfk::nn::conv3x3_forward(x1, c1, x2, c2, N, H, W, wgt, bias, Cout, act, 0.f, y, nullptr,
+                                    static_cast<hipStream_t>(stream), x1_mode, ypool));
+    return RFK_OK;
+}
+
+int rfk_nn_avgpool2(const void* x, int N, int H, int W, int C, void* y, void* stream) {
+    if (!x || !y) return fail(RFK_ERR_INVALID, "null argument");
+    if (N <= 0 || H < 2 || W < 2 || (H | W) & 1 || C <= 0 || C % 8)
+        return fail(RFK_ERR_INVALID, "avgpool2 needs even H and W and C a multiple of 8");
+    NN_HIP(rfk::nn::avgpool2_forward(x, N, H, W, C, y, static_cast<hipStream_t>(stream)));
     return RFK_OK;
 }
 

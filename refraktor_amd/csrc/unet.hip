@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 #include <unordered_map>
 
 #include "unet_kernels.hpp"
@@ -36,6 +37,26 @@ __device__ __forceinline__ bf16x8 zero8() {
 #pragma unroll
     for (int i = 0; i < 8; i++) z[i] = bf16(0.0f);
     return z;
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Raw buffer reads: a lane whose byte offset lies past the resource's size
+// reads zeros, so a masked-off lane passes kBufferOOB (beyond any resource,
+// all of which hold fewer than 2^31 bytes) instead of selecting zero after
+// the load, and the load stays in flight until its data is used.
+constexpr unsigned kBufferOOB = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(bytes), 0x00020000);
+}
+__device__ __forceinline__ bf16x8 buffer_load8(__amdgpu_buffer_rsrc_t r, unsigned byte_off) {
+    return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0));
+}
+
+// act_fwd with the activation's branch taken out of the lanes: relu (act 1)
+// gives 0 below zero, leaky (2) v * slope, none (0) v * 1
+__device__ __forceinline__ float act_select(float v, bool relu, float neg_slope) {
+    return v > 0.f ? v : (relu ? 0.f : v * neg_slope);
 }
 
 // ------------------------------------------------------------- forward
@@ -579,35 +600,62 @@ __device__ __forceinline__ bf16x8 load_px8(const bf16* __restrict__ x1n, int c1,
     return ok ? v : zero8();
 }
 
+// bf16 elements of one strip wave's LDS: the larger of a staged chunk (34
+// pixels x KC channels) and an output row (32 pixels x 32 channels), both
+// with rows padded by PAD
+constexpr int strip_wave_lds(int kc) { return 34 * (kc + PAD) > 32 * (32 + PAD) ? 34 * (kc + PAD) : 32 * (32 + PAD); }
+
+// compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 // Every wave walks its own 32-pixel column strip down `rows` output rows for
 // 32 output channels (grid.z = images x 32-channel tiles), input-stationary:
-// each staged input
-// row (one KC-channel chunk at a time, from registers loaded one step ahead)
-// feeds the three output rows it touches, held in three rotating
-// accumulators, so an input row is read from LDS once per (dx, k-step)
-// instead of three times, and waves never wait for each other after the
-// weights are staged. D = W (32 out channels x K) * X (K x 32 pixels): a lane
-// ends up with 16 channels of one pixel and stores them as 4-channel vectors.
-// MODE is x1's read (0 direct, 1 2x upsample, 2 2x2 average pool); element
-// offsets inside one image are 32-bit (the host checks they fit).
-// NW waves per workgroup share the layer's weights in LDS and never wait for
-// each other once the weights are staged; more of them per workgroup means
-// more resident waves per CU for the same LDS
-template <int KC, int MODE, int NW>
+// each staged input row (one KC-channel chunk at a time, NCH chunks) feeds
+// the three output rows it touches, held in three rotating accumulators, so
+// an input row is read from LDS once per (dx, k-step) instead of three
+// times, and waves never wait for each other after the weights are staged.
+// D = W (32 out channels x K) * X (K x 32 pixels): a lane ends up with 16
+// channels of one pixel.
+// Each chunk's pixels are fetched three chunks ahead with raw buffer loads
+// into one of three register sets: a lane outside the image or past the
+// channels passes an offset beyond the buffer and reads zeros, so nothing
+// waits on the data until it is written to LDS; a lane's column and channel
+// group are fixed, so its byte offsets are computed once and each chunk only
+// adds the row's and chunk's scalar part. Inside a chunk the MFMA operands
+// are read from LDS one (dx, k-step) ahead of their use. Whole 32-channel
+// BF16 output tiles leave through the wave's LDS rows as 16-byte stores.
+// MODE is x1's read (0 direct, 1 2x upsample, 2 2x2 average pool); byte
+// offsets inside one image are 32-bit (the host checks every image's tensors
+// hold fewer than 2^30 bytes). NW waves per workgroup share the layer's
+// weights in LDS.
+template <int KC, int NCH, int MODE, int NW>
 __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __restrict__ x1, int c1,
                                                             const bf16* __restrict__ x2, int c2, int H, int W,
                                                             const bf16* __restrict__ wgt,
                                                             const float* __restrict__ bias, int Cout, int act,
                                                             float slope, int rows, bf16* __restrict__ y,
-                                                            float* __restrict__ yf, const float* __restrict__ rbase,
+                                                            bf16* __restrict__ ypool, float* __restrict__ yf,
+                                                            const float* __restrict__ rbase,
                                                             float* __restrict__ rgb) {
-    constexpr int SPX = 34, XP = KC + PAD, G = KC / 8, NQ = (SPX * G + 63) / 64;
+    constexpr int SPX = 34, XP = KC + PAD, G = KC / 8, NQ = (SPX * G + 63) / 64, KS = KC / 16, NS = 3 * KS;
+    constexpr int WP = NCH * KC + PAD, GW = NCH * KC / 8;
+    // this wave's LDS: a staged chunk [SPX][XP], or an output row [32][OXP]
+    constexpr int OXP = 32 + PAD, WAVE_LDS = strip_wave_lds(KC);
+    static_assert(64 % G == 0, "a lane's channel group is the same in all its loads");
     extern __shared__ __attribute__((aligned(16))) unsigned char conv_smem[];
     const int Cin = c1 + c2;
-    const int nch = (Cin + KC - 1) / KC, CinP = nch * KC, WP = CinP + PAD, GW = CinP / 8;
     const int t = int(threadIdx.x), wv = t >> 6, l = t & 63, m = l & 31, kh = l >> 5;
-    bf16* s_w = reinterpret_cast<bf16*>(conv_smem);  // [9 taps][32 out][WP]
-    bf16* s_x = s_w + 9 * 32 * WP + wv * SPX * XP;   // this wave's staged chunk [SPX][XP]
+    // [32 out][9 taps][WP]: every tap and chunk of a lane's output channel
+    // lies within an immediate offset of one base address
+    bf16* s_w = reinterpret_cast<bf16*>(conv_smem);
+    float* s_bias = reinterpret_cast<float*>(s_w + 9 * 32 * WP);    // the tile's 32 biases
+    bf16* s_x = s_w + 9 * 32 * WP + 64 + wv * WAVE_LDS;
     const int x0 = int(blockIdx.x) * 32 * NW + wv * 32;  // the wave's first output pixel
     const int y0 = int(blockIdx.y) * rows;
     const int ntile = (Cout + 31) / 32;
@@ -615,44 +663,124 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
     const int H1 = MODE == 1 ? H / 2 : (MODE == 2 ? 2 * H : H), W1 = MODE == 1 ? W / 2 : (MODE == 2 ? 2 * W : W);
     const bf16* x1n = x1 + size_t(n) * H1 * W1 * c1;
     const bf16* x2n = x2 ? x2 + size_t(n) * H * W * c2 : nullptr;
+    const __amdgpu_buffer_rsrc_t r1 = buffer_rsrc(x1n, unsigned(H1 * W1 * c1) * 2u);
+    const __amdgpu_buffer_rsrc_t r2 = buffer_rsrc(x2n ? x2n : x1n, x2n ? unsigned(H * W * c2) * 2u : 0u);
 
     for (int e = t; e < 9 * 32 * GW; e += 64 * NW) {
-        const int g = e % GW, co = (e / GW) % 32, tap = e / (32 * GW);
+        const int g = e % GW, tap = (e / GW) % 9, co = e / (9 * GW);
         bf16x8 v = zero8();
         if (co0 + co < Cout && g * 8 < Cin) v = load8(wgt + unsigned(((co0 + co) * 9 + tap) * Cin + g * 8));
-        *reinterpret_cast<bf16x8*>(s_w + (tap * 32 + co) * WP + g * 8) = v;
+        *reinterpret_cast<bf16x8*>(s_w + (co * 9 + tap) * WP + g * 8) = v;
     }
-    // (input row jj of the strip = image row y0 - 1 + jj, chunk ch) -> registers
-    auto load_chunk = [&](int jj, int ch, bf16x8* v) {
+    if (t < 32) s_bias[t] = (bias && co0 + t < Cout) ? bias[co0 + t] : 0.f;
+
+    // the lane's part of a staged row: channel group g, column (l + 64 q) / G
+    const int g = l % G;
+    unsigned o1c[NQ], o2c[NQ];
+    bool colok[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int col = (l + 64 * q) / G, gx = x0 - 1 + col;
+        colok[q] = col < SPX && gx >= 0 && gx < W;
+        o1c[q] = unsigned((MODE == 1 ? (gx >> 1) : gx) * c1 + g * 8) * 2u;
+        o2c[q] = unsigned(gx * c2 + g * 8 - c1) * 2u;  // x2's channel is c - c1
+    }
+    // (input row jj of the strip = image row y0 - 1 + jj, chunk CH) -> registers
+    auto load_chunk = [&](int jj, auto chc, bf16x8* v) __attribute__((always_inline)) {
+        constexpr int CH = decltype(chc)::value, CB = CH * KC;
         const int gy = y0 - 1 + jj;
         const bool row_ok = gy >= 0 && gy < H;
+        const bool ch_ok = g * 8 < Cin - CB;
+        if (MODE == 2) {
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int col = (l + 64 * q) / G;
+                v[q] = load_px8<2>(x1n, c1, x2n, c2, H, W, gy, x0 - 1 + col, CB + g * 8, row_ok && colok[q] && ch_ok);
+            }
+            return;
+        }
+        // every chunk lies wholly inside x1 or x2 (the host sends layers
+        // with c2 > 0 here only when c1 is a multiple of KC): one resource,
+        // chosen per wave
+        const bool in2 = c2 > 0 && CB >= c1;
+        const unsigned srow = in2 ? unsigned(gy * W * c2 + CB) * 2u
+                                  : unsigned((MODE == 1 ? (gy >> 1) * W1 : gy * W) * c1 + CB) * 2u;
+        const __amdgpu_buffer_rsrc_t r = in2 ? r2 : r1;
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
-            const int e = l + q * 64;
-            const int g = e % G, col = e / G;
-            const int gx = x0 - 1 + col, c = ch * KC + g * 8;
-            v[q] = load_px8<MODE>(x1n, c1, x2n, c2, H, W, gy, gx, c,
-                                  row_ok && col < SPX && gx >= 0 && gx < W && c < Cin);
+            const bool ok = row_ok && colok[q] && ch_ok;
+            v[q] = buffer_load8(r, ok ? (in2 ? o2c[q] : o1c[q]) + srow : kBufferOOB);
         }
     };
-    float bv[16];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int co = co0 + 8 * i + 4 * kh + j;
-            bv[i * 4 + j] = (bias && co < Cout) ? bias[co] : 0.f;
-        }
+    // biases of accumulator elements 4i..4i+3 (channels co0 + 8i + 4kh + 0..3)
+    auto bias4 = [&](int i) __attribute__((always_inline)) { return *reinterpret_cast<const float4*>(s_bias + 8 * i + 4 * kh); };
+    const bool relu = act == 1;
+    const float neg_slope = act == 2 ? slope : 1.f;
     const int px = x0 + m;
-    auto store_row = [&](int yy, const f32x16& acc) {
+    const bool via_lds = y && !yf && !rgb && (Cout & 31) == 0;
+    // with ypool (even rows per strip, even H and W), the 2x2 average pool of
+    // the BF16 output as well, summed in the pooling kernel's order: the even
+    // row's pixel pair, then the odd row's
+    bf16x8 prev[2];
+    auto store_row = [&](int yy, const f32x16& acc) __attribute__((always_inline)) {
+        if (via_lds) {
+            // the wave's reads of s_x for this row are issued before
+            // these writes, its next chunk's writes after these reads
+            __asm__ volatile("" ::: "memory");
+            typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float4 bb = bias4(i);
+                const float bj[4] = {bb.x, bb.y, bb.z, bb.w};
+                bf16x4 b;
+#pragma unroll
+                for (int j = 0; j < 4; j++) b[j] = bf16(act_select(acc[i * 4 + j] + bj[j], relu, neg_slope));
+                *reinterpret_cast<bf16x4*>(s_x + m * OXP + 8 * i + 4 * kh) = b;
+            }
+            __asm__ volatile("" ::: "memory");
+            const size_t o = (size_t(n) * H + yy) * W;
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const int p = r * 16 + (l >> 2), cg = l & 3;
+                const bf16x8 v = *reinterpret_cast<const bf16x8*>(s_x + p * OXP + cg * 8);
+                if (x0 + p < W) *reinterpret_cast<bf16x8*>(y + (o + x0 + p) * Cout + co0 + cg * 8) = v;
+                if (ypool) {
+                    if ((yy & 1) == 0) {
+                        prev[r] = v;
+                    } else {
+                        // pixel p + 1 of both rows sits four lanes up
+                        const u32x4 pa = __builtin_bit_cast(u32x4, prev[r]), pv = __builtin_bit_cast(u32x4, v);
+                        u32x4 na, nv;
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            na[j] = unsigned(__shfl_down(int(pa[j]), 4));
+                            nv[j] = unsigned(__shfl_down(int(pv[j]), 4));
+                        }
+                        const bf16x8 b = __builtin_bit_cast(bf16x8, na), d = __builtin_bit_cast(bf16x8, nv);
+                        bf16x8 pool;
+#pragma unroll
+                        for (int j = 0; j < 8; j++)
+                            pool[j] = bf16(0.25f * (float(prev[r][j]) + float(b[j]) + float(v[j]) + float(d[j])));
+                        if ((p & 1) == 0 && x0 + p < W) {
+                            const size_t po = (size_t(n) * (H / 2) + yy / 2) * (W / 2) + (x0 + p) / 2;
+                            *reinterpret_cast<bf16x8*>(ypool + po * Cout + co0 + cg * 8) = pool;
+                        }
+                    }
+                }
+            }
+            __asm__ volatile("" ::: "memory");
+            return;
+        }
         if (px >= W) return;
         if (rgb) {
             // the network's output layer: rgb = clamp(base + first three channels)
             if (co0 == 0 && kh == 0) {
                 const size_t q = ((size_t(n) * H + yy) * W + px) * 3;
+                const float4 bb = bias4(0);
+                const float bj[3] = {bb.x, bb.y, bb.z};
 #pragma unroll
                 for (int j = 0; j < 3; j++)
-                    rgb[q + j] = fminf(fmaxf(rbase[q + j] + act_fwd(acc[j] + bv[j], act, slope), 0.f), 1.f);
+                    rgb[q + j] = fminf(fmaxf(rbase[q + j] + act_fwd(acc[j] + bj[j], act, slope), 0.f), 1.f);
             }
             return;
         }
@@ -661,9 +789,11 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
         for (int i = 0; i < 4; i++) {
             const int co = co0 + 8 * i + 4 * kh;
             if (co >= Cout) continue;
+            const float4 bb = bias4(i);
+            const float bj[4] = {bb.x, bb.y, bb.z, bb.w};
             float v[4];
 #pragma unroll
-            for (int j = 0; j < 4; j++) v[j] = act_fwd(acc[i * 4 + j] + bv[i * 4 + j], act, slope);
+            for (int j = 0; j < 4; j++) v[j] = act_select(acc[i * 4 + j] + bj[j], relu, neg_slope);
             if ((Cout & 3) == 0) {
                 if (y) {
                     typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
@@ -684,59 +814,23 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
         }
     };
 
-    bf16x8 nxt[NQ];
-    load_chunk(0, 0, nxt);
-    __syncthreads();  // weights staged; from here on every wave runs on its own
+    // Steps (input row jj, chunk ch) run jj-major; step s's chunk is loaded
+    // at step s - 3 into register set s % 3. Rows are unrolled three at a
+    // time, so both the set of a step and the roles of the accumulators are
+    // fixed at compile time.
+    bf16x8 pa[NQ], pb[NQ], pc[NQ];
+    static_for<0, 3>([&](auto sc) __attribute__((always_inline)) {
+        constexpr int S = decltype(sc)::value;
+        auto& buf = [&]() __attribute__((always_inline)) -> bf16x8 (&)[NQ] {
+            if constexpr (S == 0) return pa;
+            else if constexpr (S == 1) return pb;
+            else return pc;
+        }();
+        load_chunk(S / NCH, std::integral_constant<int, S % NCH>{}, buf);
+    });
+    __syncthreads();  // weights and biases staged; from here on every wave runs on its own
+    if (x0 >= W) return;  // a wave wholly right of the image (the last column of workgroups)
 
-    // one input row: aA/aB/aC accumulate output rows jj, jj - 1, jj - 2 (taps
-    // dy = 0, 1, 2); afterwards row jj - 2 is complete
-    auto step = [&](int jj, f32x16& aA, f32x16& aB, f32x16& aC) {
-        if (jj >= rows + 2) return;
-        const int gy = y0 - 1 + jj;
-        const bool in_img = gy >= 0 && gy < H;
-        // all three MFMAs run unconditionally (rows outside the strip or the
-        // image collect junk that is never stored and is cleared on reuse):
-        // no branch between an operand read and its MFMA
-        const bool vC = jj >= 2 && y0 + jj - 2 < H;
-        for (int ch = 0; ch < nch; ch++) {
-            // the wave's previous reads of s_x are issued before these writes
-            // (LDS executes one wave's instructions in order)
-            __asm__ volatile("" ::: "memory");
-#pragma unroll
-            for (int q = 0; q < NQ; q++) {
-                const int e = l + q * 64;
-                const int g = e % G, col = e / G;
-                if (col < SPX) *reinterpret_cast<bf16x8*>(s_x + col * XP + g * 8) = nxt[q];
-            }
-            __asm__ volatile("" ::: "memory");
-            {
-                int nj = jj, nc = ch + 1;
-                if (nc == nch) {
-                    nc = 0;
-                    nj++;
-                }
-                if (nj < rows + 2) load_chunk(nj, nc, nxt);
-            }
-            if (!in_img) continue;
-#pragma unroll
-            for (int dx = 0; dx < 3; dx++) {
-#pragma unroll
-                for (int ks = 0; ks < KC / 16; ks++) {
-                    const bf16x8 xb = *reinterpret_cast<const bf16x8*>(s_x + (m + dx) * XP + ks * 16 + kh * 8);
-                    const int k = ch * KC + ks * 16 + kh * 8;
-                    const bf16x8 w0 = *reinterpret_cast<const bf16x8*>(s_w + ((0 + dx) * 32 + m) * WP + k);
-                    const bf16x8 w1 = *reinterpret_cast<const bf16x8*>(s_w + ((3 + dx) * 32 + m) * WP + k);
-                    const bf16x8 w2 = *reinterpret_cast<const bf16x8*>(s_w + ((6 + dx) * 32 + m) * WP + k);
-                    aA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, xb, aA, 0, 0, 0);
-                    aB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, xb, aB, 0, 0, 0);
-                    aC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2, xb, aC, 0, 0, 0);
-                }
-            }
-        }
-        if (vC) store_row(y0 + jj - 2, aC);
-#pragma unroll
-        for (int i = 0; i < 16; i++) aC[i] = 0.f;
-    };
     f32x16 a0, a1, a2;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
@@ -744,10 +838,76 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
         a1[i] = 0.f;
         a2[i] = 0.f;
     }
-    for (int jj = 0; jj < rows + 2; jj += 3) {
-        step(jj, a0, a2, a1);
-        step(jj + 1, a1, a0, a2);
-        step(jj + 2, a2, a1, a0);
+    // one input row: aA/aB/aC accumulate output rows jj, jj - 1, jj - 2 (taps
+    // dy = 0, 1, 2); afterwards row jj - 2 is complete
+    auto row = [&](auto rc, int jj, f32x16& aA, f32x16& aB, f32x16& aC) __attribute__((always_inline)) {
+        constexpr int R = decltype(rc)::value;
+        const int gy = y0 - 1 + jj;
+        const bool in_img = gy >= 0 && gy < H;
+        static_for<0, NCH>([&](auto chc) __attribute__((always_inline)) {
+            constexpr int CH = decltype(chc)::value, S = R * NCH + CH;
+            auto& buf = [&]() __attribute__((always_inline)) -> bf16x8 (&)[NQ] {
+                if constexpr (S % 3 == 0) return pa;
+                else if constexpr (S % 3 == 1) return pb;
+                else return pc;
+            }();
+            // the wave's previous reads of s_x are issued before these writes
+            // (LDS executes one wave's instructions in order)
+            __asm__ volatile("" ::: "memory");
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int col = (l + 64 * q) / G;
+                if (col < SPX) *reinterpret_cast<bf16x8*>(s_x + col * XP + g * 8) = buf[q];
+            }
+            __asm__ volatile("" ::: "memory");
+            // step s + 3
+            constexpr int NJ = (CH + 3) / NCH, NC = (CH + 3) % NCH;
+            if (jj + NJ < rows + 2) load_chunk(jj + NJ, std::integral_constant<int, NC>{}, buf);
+            if (!in_img) return;  // rows outside the image add nothing
+            const bf16* sx = s_x + m * XP + kh * 8;
+            const bf16* sw = s_w + m * 9 * WP + CH * KC + kh * 8;
+            // operands of step i = (dx = i / KS, ks = i % KS): the pixels are
+            // read one step ahead, each weight fragment right after the MFMA
+            // that used the previous one, three MFMAs before its own
+            auto rx = [&](int i) __attribute__((always_inline)) {
+                return *reinterpret_cast<const bf16x8*>(sx + (i / KS) * XP + (i % KS) * 16);
+            };
+            auto rw = [&](int i, int dy) __attribute__((always_inline)) {
+                return *reinterpret_cast<const bf16x8*>(sw + (3 * dy + i / KS) * WP + (i % KS) * 16);
+            };
+            bf16x8 xc = rx(0), w0 = rw(0, 0), w1 = rw(0, 1), w2 = rw(0, 2);
+#pragma unroll
+            for (int i = 0; i < NS; i++) {
+                const bool more = i + 1 < NS;
+                const bf16x8 xn = more ? rx(i + 1) : xc;
+                __builtin_amdgcn_sched_barrier(0);
+                aA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, xc, aA, 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (more) w0 = rw(i + 1, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                aB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, xc, aB, 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (more) w1 = rw(i + 1, 1);
+                __builtin_amdgcn_sched_barrier(0);
+                aC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2, xc, aC, 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (more) w2 = rw(i + 1, 2);
+                xc = xn;
+            }
+        });
+        if (jj >= 2 && y0 + jj - 2 < H) store_row(y0 + jj - 2, aC);
+#pragma unroll
+        for (int i = 0; i < 16; i++) aC[i] = 0.f;
+    };
+    for (int jb = 0; jb < rows + 2; jb += 3) {
+        static_for<0, 3>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int R = decltype(rc)::value;
+            const int jj = jb + R;
+            if (jj >= rows + 2) return;
+            if constexpr (R == 0) row(rc, jj, a0, a2, a1);
+            else if constexpr (R == 1) row(rc, jj, a1, a0, a2);
+            else row(rc, jj, a2, a1, a0);
+        });
     }
 }
 
@@ -1052,20 +1212,24 @@ static size_t strip_weight_bytes(int Cin, int KC) {
     return size_t(9) * 32 * (CinP + PAD) * sizeof(bf16);
 }
 
-// waves per strip workgroup by x1's read mode (measured per layer at 1080p:
-// direct reads gain up to 10 % from 16 waves over 4, the upsampling read is
-// fastest with 8 (96->32 at 1080p 166 -> 131 us), the pooling read with 4)
-static int strip_waves(int mode) { return mode == 1 ? 8 : mode == 2 ? 4 : 16; }
-
 static size_t strip_lds_bytes(int Cin, int KC, int nw) {
     const int CinP = (Cin + KC - 1) / KC * KC;
-    return (size_t(9) * 32 * (CinP + PAD) + size_t(nw) * 34 * (KC + PAD)) * sizeof(bf16);
+    return (size_t(9) * 32 * (CinP + PAD) + 64 + size_t(nw) * strip_wave_lds(KC)) * sizeof(bf16);
+}
+
+// waves per strip workgroup (one workgroup per CU either way, its LDS being
+// the layer's weights): direct reads 8, the upsampling read 12 where the
+// weights leave room (96->32 at 1080p 153 -> 143 us), the pooling read 4
+static int strip_waves(int mode, int Cin, int KC) {
+    if (mode == 1) return strip_lds_bytes(Cin, KC, 12) <= 160 * 1024 ? 12 : 8;
+    return mode == 2 ? 4 : 8;
 }
 
 // Rows per strip: a workgroup's time grows with rows + 2 (the halo rows), and
 // a launch takes ceil(workgroups / resident workgroups) rounds, so pick the
-// row count that minimises rounds x (rows + 2).
-static int strip_rows(const void* fn, size_t lds, int nw, int zdim, int H, int W) {
+// row count that minimises rounds x (rows + 2); even counts only when the
+// kernel also pools.
+static int strip_rows(const void* fn, size_t lds, int nw, int zdim, int H, int W, bool even) {
     thread_local std::unordered_map<size_t, int> resident_cache;
     int resident = 0;
     const size_t key = reinterpret_cast<size_t>(fn) * 1000003u + lds;
@@ -1083,7 +1247,7 @@ static int strip_rows(const void* fn, size_t lds, int nw, int zdim, int H, int W
     const long cols = (W + 32 * nw - 1) / (32 * nw);
     int best = 16;
     long best_cost = -1;
-    for (int r = 4; r <= 64; r++) {
+    for (int r = 4; r <= 64; r += even ? 2 : 1) {
         const long blocks = cols * ((H + r - 1) / r) * zdim;
         const long cost = (blocks + resident - 1) / resident * (r + 2);
         if (best_cost < 0 || cost < best_cost) {
@@ -1094,34 +1258,64 @@ static int strip_rows(const void* fn, size_t lds, int nw, int zdim, int H, int W
     return best;
 }
 
-template <int KC, int M, int NW>
-static hipError_t launch_strip_mode(const void* x1, int c1, const void* x2, int c2, int N, int H, int W,
-                                    const void* wgt, const float* bias, int Cout, int act, float slope, void* y,
-                                    float* yf, size_t lds, hipStream_t s, const float* rbase, float* rgb) {
-    const void* fn = reinterpret_cast<const void*>(&conv3x3_strip_kernel<KC, M, NW>);
+// one strip convolution's operands (see conv3x3_strip_kernel)
+struct strip_args {
+    const void* x1;
+    int c1;
+    const void* x2;
+    int c2, N, H, W;
+    const void* wgt;
+    const float* bias;
+    int Cout, act;
+    float slope;
+    void* y;
+    void* ypool;
+    float* yf;
+    const float* rbase;
+    float* rgb;
+};
+
+template <int KC, int NCH, int M, int NW>
+static hipError_t launch_strip_nch(const strip_args& a, size_t lds, hipStream_t s) {
+    const void* fn = reinterpret_cast<const void*>(&conv3x3_strip_kernel<KC, NCH, M, NW>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
-    const int rows = strip_rows(fn, lds, NW, N * ((Cout + 31) / 32), H, W);
-    dim3 grid((W + 32 * NW - 1) / (32 * NW), (H + rows - 1) / rows, N * ((Cout + 31) / 32));
-    conv3x3_strip_kernel<KC, M, NW><<<grid, 64 * NW, lds, s>>>((const bf16*)x1, c1, (const bf16*)x2, c2, H, W,
-                                                                (const bf16*)wgt, bias, Cout, act, slope, rows,
-                                                                (bf16*)y, yf, rbase, rgb);
+    const int zdim = a.N * ((a.Cout + 31) / 32);
+    const int rows = strip_rows(fn, lds, NW, zdim, a.H, a.W, a.ypool != nullptr);
+    dim3 grid((a.W + 32 * NW - 1) / (32 * NW), (a.H + rows - 1) / rows, zdim);
+    conv3x3_strip_kernel<KC, NCH, M, NW><<<grid, 64 * NW, lds, s>>>(
+        (const bf16*)a.x1, a.c1, (const bf16*)a.x2, a.c2, a.H, a.W, (const bf16*)a.wgt, a.bias, a.Cout, a.act,
+        a.slope, rows, (bf16*)a.y, (bf16*)a.ypool, a.yf, a.rbase, a.rgb);
     return hipGetLastError();
 }
 
-// lds = strip_lds_bytes(Cin, KC, strip_waves(mode))
+// the kernel instance for the layer's chunk count (KC 16: one chunk; KC 32:
+// up to seven, 224 input channels, the most whose weights fit in LDS)
+template <int KC, int M, int NW>
+static hipError_t launch_strip_mode(const strip_args& a, size_t lds, hipStream_t s) {
+    const int nch = (a.c1 + a.c2 + KC - 1) / KC;
+    if constexpr (KC == 16) {
+        if (nch == 1) return launch_strip_nch<KC, 1, M, NW>(a, lds, s);
+    } else {
+        switch (nch) {
+            case 1: return launch_strip_nch<KC, 1, M, NW>(a, lds, s);
+            case 2: return launch_strip_nch<KC, 2, M, NW>(a, lds, s);
+            case 3: return launch_strip_nch<KC, 3, M, NW>(a, lds, s);
+            case 4: return launch_strip_nch<KC, 4, M, NW>(a, lds, s);
+            case 5: return launch_strip_nch<KC, 5, M, NW>(a, lds, s);
+            case 6: return launch_strip_nch<KC, 6, M, NW>(a, lds, s);
+            case 7: return launch_strip_nch<KC, 7, M, NW>(a, lds, s);
+            default: break;
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
 template <int KC>
-static hipError_t launch_strip(const void* x1, int c1, int mode, const void* x2, int c2, int N, int H, int W,
-                               const void* wgt, const float* bias, int Cout, int act, float slope, void* y, float* yf,
-                               size_t lds, hipStream_t s, const float* rbase = nullptr, float* rgb = nullptr) {
-    if (mode == 1)
-        return launch_strip_mode<KC, 1, 8>(x1, c1, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s, rbase,
-                                           rgb);
-    if (mode == 2)
-        return launch_strip_mode<KC, 2, 4>(x1, c1, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s, rbase,
-                                           rgb);
-    return launch_strip_mode<KC, 0, 16>(x1, c1, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s, rbase,
-                                        rgb);
+static hipError_t launch_strip(const strip_args& a, int mode, int nw, size_t lds, hipStream_t s) {
+    if (mode == 1) return nw == 12 ? launch_strip_mode<KC, 1, 12>(a, lds, s) : launch_strip_mode<KC, 1, 8>(a, lds, s);
+    if (mode == 2) return launch_strip_mode<KC, 2, 4>(a, lds, s);
+    return launch_strip_mode<KC, 0, 8>(a, lds, s);
 }
 
 template <int R, int C, int KCH>
@@ -1135,32 +1329,39 @@ static void launch_conv_fwd(const void* x1, int c1, int mode1, const void* x2, i
 
 hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N, int H, int W, const void* wgt,
                            const float* bias, int Cout, int act, float slope, void* y, float* yf, hipStream_t s,
-                           int x1_mode) {
+                           int x1_mode, void* ypool) {
     if ((c1 % 8) || (c2 % 8) || c1 <= 0 || c2 < 0 || Cout <= 0) return hipErrorInvalidValue;
     if (x1_mode < 0 || x1_mode > 2 || (x1_mode == 1 && ((H | W) & 1))) return hipErrorInvalidValue;
+    if (ypool && (!y || (Cout % 8) || ((H | W) & 1))) return hipErrorInvalidValue;
     const int Cin = c1 + c2;
-    // 32-bit element offsets inside one image (the strip kernels' indexing)
+    // 32-bit offsets inside one image (the strip kernels' indexing): the
+    // strip kernel's buffer reads take byte offsets below 2^31, the other
+    // kernels element offsets
     const int H1 = x1_mode == 2 ? 2 * H : H, W1 = x1_mode == 2 ? 2 * W : W;
-    const bool offsets32 = size_t(H1) * W1 * c1 < (size_t(1) << 31) && size_t(H) * W * std::max(c2, Cout) < (size_t(1) << 31) &&
+    const bool offsets32 = size_t(H1) * W1 * c1 < (size_t(1) << 30) && size_t(H) * W * std::max(c2, Cout) < (size_t(1) << 30) &&
                            size_t(Cout) * 9 * Cin < (size_t(1) << 31);
     // the input-stationary strip kernel wherever the layer's weights fit in
-    // 140 KB of LDS next to the staging rows (inputs up to 240 channels) and
-    // the output has at most three 32-channel tiles: measured fastest there
-    // (96->32 at 1080p 131 us against 192 us weight-stationary, 160->64 at
-    // 540x960 116 against 150 us, 224->96 at 270x480 82 against 94 us)
+    // 140 KB of LDS next to the staging rows (inputs up to 224 channels):
+    // measured fastest there (96->32 at 1080p 143 us against 192 us
+    // weight-stationary, 128->128 at 135x240 against the tiled kernel)
     const int KC = Cin <= 16 ? 16 : 32;
-    const size_t lds = strip_lds_bytes(Cin, KC, strip_waves(x1_mode));
-    if (Cout <= 96 && strip_weight_bytes(Cin, KC) <= 140 * 1024 && lds <= 160 * 1024 && offsets32) {
-        if (KC == 16) return launch_strip<16>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s);
-        return launch_strip<32>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, lds, s);
+    const int nw = strip_waves(x1_mode, Cin, KC);
+    const size_t lds = strip_lds_bytes(Cin, KC, nw);
+    // (the strip kernel reads every KC-channel chunk from one of x1, x2)
+    const bool chunks_split = c2 == 0 || c1 % KC == 0;
+    if (strip_weight_bytes(Cin, KC) <= 140 * 1024 && lds <= 160 * 1024 && offsets32 && chunks_split) {
+        // the pool comes out of the store epilogue of whole 32-channel BF16 tiles
+        const bool pool_fused = ypool && !yf && (Cout % 32) == 0;
+        const strip_args a{x1, c1, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, pool_fused ? ypool : nullptr, yf,
+                           nullptr, nullptr};
+        hipError_t e = KC == 16 ? launch_strip<16>(a, x1_mode, nw, lds, s) : launch_strip<32>(a, x1_mode, nw, lds, s);
+        if (e != hipSuccess || !ypool || pool_fused) return e;
+        return avgpool2_forward(y, N, H, W, Cout, ypool, s);
     }
-    // wider inputs: the weight-stationary strip; the 128-channel outputs run
-    // on the tiled kernel (128->128 at 135x240 39 us against 44 us strip)
+    // wider inputs: the weight-stationary strip; the tiled kernel for the rest
     if (Cin >= 96 && Cout <= 96 && offsets32) {
         launch_wstat<4>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
-        return hipGetLastError();
-    }
-    if (Cout <= 32) {
+    } else if (Cout <= 32) {
         if (Cin <= 16)
             launch_conv_fwd<1, 1, 16>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
         else
@@ -1172,7 +1373,9 @@ hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
     } else {
         launch_conv_fwd<1, 4, 16>(x1, c1, x1_mode, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, yf, s);
     }
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !ypool) return e;
+    return avgpool2_forward(y, N, H, W, Cout, ypool, s);
 }
 
 hipError_t conv3x3_forward_rgb(const void* x, int cin, int N, int H, int W, const void* wgt, const float* bias,
@@ -1187,22 +1390,20 @@ hipError_t conv3x3_forward_rgb(const void* x, int cin, int N, int H, int W, cons
         const void* fn = reinterpret_cast<const void*>(&conv3x3_rgb8_kernel<NW>);
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
         if (e != hipSuccess) return e;
-        const int rows = strip_rows(fn, lds, NW, N, H, W);
+        const int rows = strip_rows(fn, lds, NW, N, H, W, false);
         dim3 grid((W + 32 * NW - 1) / (32 * NW), (H + rows - 1) / rows, N);
         conv3x3_rgb8_kernel<NW><<<grid, 64 * NW, lds, s>>>((const bf16*)x, cin, H, W, (const bf16*)wgt, bias, Cout, act,
                                                           rows, base, out_rgb);
         return hipGetLastError();
     }
     const int KC = cin <= 16 ? 16 : 32;
-    const size_t lds = strip_lds_bytes(cin, KC, strip_waves(0));
+    const int nw = strip_waves(0, cin, KC);
+    const size_t lds = strip_lds_bytes(cin, KC, nw);
     if (strip_weight_bytes(cin, KC) > 140 * 1024 || lds > 160 * 1024 ||
-        size_t(H) * W * std::max(cin, 3) >= (size_t(1) << 31))
+        size_t(H) * W * std::max(cin, 3) >= (size_t(1) << 30))
         return hipErrorInvalidValue;
-    if (KC == 16)
-        return launch_strip<16>(x, cin, 0, nullptr, 0, N, H, W, wgt, bias, Cout, act, 0.f, nullptr, nullptr, lds, s, base,
-                                out_rgb);
-    return launch_strip<32>(x, cin, 0, nullptr, 0, N, H, W, wgt, bias, Cout, act, 0.f, nullptr, nullptr, lds, s, base,
-                            out_rgb);
+    const strip_args a{x, cin, nullptr, 0, N, H, W, wgt, bias, Cout, act, 0.f, nullptr, nullptr, nullptr, base, out_rgb};
+    return KC == 16 ? launch_strip<16>(a, 0, nw, lds, s) : launch_strip<32>(a, 0, nw, lds, s);
 }
 
 size_t conv3x3_wgrad_workspace(int c1, int c2, int Cout, int N, int H, int W, int* groups, int* tiles_per_group) {

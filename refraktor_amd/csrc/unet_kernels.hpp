@@ -12,10 +12,12 @@ namespace rfk::nn {
 // y = act(conv3x3(concat(x1', x2)) + bias); c1, c2 multiples of 8 (c2 may be 0).
 // act: 0 none, 1 ReLU, 2 leaky ReLU(slope). y (BF16) and/or yf (FP32) may be null.
 // x1_mode: 0 x1' = x1 (H x W), 1 x1' = nearest 2x upsample of x1 (H/2 x W/2),
-// 2 x1' = 2x2 average pool of x1 (2H x 2W).
+// 2 x1' = 2x2 average pool of x1 (2H x 2W). ypool (needs y, even H and W,
+// Cout a multiple of 8): also the 2x2 average pool of y, N x H/2 x W/2 x Cout
+// BF16, as avgpool2_forward computes it.
 hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N, int H, int W, const void* wgt,
                            const float* bias, int Cout, int act, float slope, void* y, float* yf, hipStream_t s,
-                           int x1_mode = 0);
+                           int x1_mode = 0, void* ypool = nullptr);
 // The inference network's output layer with the residual fused:
 // out_rgb[p][c] = clamp(base[p][c] + conv(x)[p][c], 0, 1) for c < 3 (FP32,
 // N*H*W*3). hipErrorInvalidValue when the strip kernel does not take the shape.

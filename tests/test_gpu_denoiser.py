@@ -35,7 +35,9 @@ CASES = [  # (N, H, W, c1, c2, Cout): every tile shape of the forward kernel
     (2, 11, 140, 16, 0, 24),
     (1, 10, 150, 64, 32, 32),
     (1, 9, 131, 48, 0, 16),
-    (1, 37, 45, 16, 8, 3),  # strip kernel: channel count not a multiple of 4, three row groups
+    (1, 37, 45, 16, 8, 3),  # x1 ends inside a 32-channel chunk: the tiled kernel
+    (1, 37, 45, 32, 8, 3),  # strip kernel: channel count not a multiple of 4, three row groups
+    (1, 23, 70, 32, 32, 64),  # strip kernel, two 32-channel tiles, x2 in its own chunk
 ]
 
 
@@ -82,6 +84,71 @@ def test_conv_fused_upsample_and_pool(gpu, mode, Co, c1, c2):
                                                    None, yf.ptr, None))
     want = nn.conv3x3(np.concatenate([x1_eff, x2], axis=-1), w, b)
     np.testing.assert_allclose(yf.download(np.float32, (N, H, W, Co)), want, rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("N,H,W,c1,c2,Co,mode", [(1, 19, 300, 32, 0, 32, 0), (2, 7, 45, 16, 0, 32, 0),
+                                                  (1, 12, 70, 64, 32, 32, 1), (1, 9, 100, 64, 0, 64, 0),
+                                                  (1, 10, 68, 96, 64, 96, 1), (1, 5, 33, 8, 0, 32, 0)])
+def test_conv_bf16_output(gpu, N, H, W, c1, c2, Co, mode):
+    """BF16 outputs (the inference network's activations, written through
+    LDS as whole 32-channel rows) against the oracle rounded to BF16: at most
+    one BF16 step apart; widths that end in a partial 32-pixel strip, batches
+    of two, the upsampling read"""
+    rng = np.random.default_rng(7 + N + H + W + Co + mode)
+    h1, w1 = (H // 2, W // 2) if mode == 1 else (H, W)
+    x1 = nn.to_bf16(rng.normal(size=(N, h1, w1, c1)).astype(np.float32))
+    x2 = nn.to_bf16(rng.normal(size=(N, H, W, c2)).astype(np.float32)) if c2 else None
+    w = nn.to_bf16((rng.normal(size=(Co, 9, c1 + c2)) * 0.1).astype(np.float32))
+    b = rng.normal(size=Co).astype(np.float32)
+    y = render.DeviceBuffer(N * H * W * Co * 2)
+    d1, d2, dw, db_ = _bf16_dev(x1), (_bf16_dev(x2) if c2 else None), _bf16_dev(w), _dev(b)
+    _lib.check(_lib.lib().rfk_nn_conv3x3_forward_ex(d1.ptr, c1, mode, d2.ptr if d2 else None, c2, N, H, W, dw.ptr,
+                                                   db_.ptr, Co, 1, y.ptr, None, None))
+    _lib.check(_lib.lib().rfk_synchronize())
+    bits = y.download(np.uint16, (N, H, W, Co)).astype(np.uint32) << 16
+    got = bits.view(np.float32)
+    x = nn.up(x1) if mode == 1 else x1
+    if c2:
+        x = np.concatenate([x, x2], axis=-1)
+    want = np.maximum(nn.conv3x3(x, w, b), 0)
+    # one BF16 step (8 significant bits) of the larger magnitude, or 1e-6
+    step = np.maximum(np.abs(want), np.abs(got)) * 2.0 ** -7 + 1e-6
+    assert np.all(np.abs(got - want) <= step), np.max(np.abs(got - want) / step)
+
+
+@pytest.mark.parametrize("N,H,W,c1,Co", [(1, 20, 300, 32, 32), (2, 14, 66, 8, 32), (1, 12, 100, 64, 64),
+                                         (1, 10, 68, 96, 96), (1, 8, 40, 32, 24), (1, 6, 50, 128, 128)])
+def test_conv_pool_epilogue(gpu, N, H, W, c1, Co):
+    """the encoder's convolution + 2x2 average pool in one pass equals the
+    convolution followed by the pooling kernel, bit for bit (whole 32-channel
+    tiles pool in the convolution's epilogue, other widths through the
+    pooling kernel); the pool itself against the oracle's"""
+    rng = np.random.default_rng(3 + N + H + W + c1 + Co)
+    x1 = nn.to_bf16(rng.normal(size=(N, H, W, c1)).astype(np.float32))
+    w = nn.to_bf16((rng.normal(size=(Co, 9, c1)) * 0.1).astype(np.float32))
+    b = rng.normal(size=Co).astype(np.float32)
+    d1, dw, db_ = _bf16_dev(x1), _bf16_dev(w), _dev(b)
+    lib = _lib.lib()
+    y = render.DeviceBuffer(N * H * W * Co * 2)
+    yp = render.DeviceBuffer(N * (H // 2) * (W // 2) * Co * 2)
+    _lib.check(lib.rfk_nn_conv3x3_forward_pool(d1.ptr, c1, 0, None, 0, N, H, W, dw.ptr, db_.ptr, Co, 1, y.ptr,
+                                               yp.ptr, None))
+    y2 = render.DeviceBuffer(N * H * W * Co * 2)
+    _lib.check(lib.rfk_nn_conv3x3_forward_ex(d1.ptr, c1, 0, None, 0, N, H, W, dw.ptr, db_.ptr, Co, 1, y2.ptr, None,
+                                             None))
+    yp2 = render.DeviceBuffer(N * (H // 2) * (W // 2) * Co * 2)
+    _lib.check(lib.rfk_nn_avgpool2(y2.ptr, N, H, W, Co, yp2.ptr, None))
+    _lib.check(lib.rfk_synchronize())
+    ybits = y.download(np.uint16, (N, H, W, Co))
+    pbits = yp.download(np.uint16, (N, H // 2, W // 2, Co))
+    assert np.array_equal(ybits, y2.download(np.uint16, (N, H, W, Co)))
+    assert np.array_equal(pbits, yp2.download(np.uint16, (N, H // 2, W // 2, Co)))
+    yv = (ybits.astype(np.uint32) << 16).view(np.float32)
+    pv = (pbits.astype(np.uint32) << 16).view(np.float32)
+    want = nn.pool(yv.astype(np.float64))
+    assert np.all(np.abs(pv - want) <= np.abs(want) * 2.0 ** -8 + 1e-30)  # BF16 rounding of the exact mean
+    assert lib.rfk_nn_conv3x3_forward_pool(d1.ptr, c1, 0, None, 0, N, H - 1, W, dw.ptr, db_.ptr, Co, 1, y.ptr,
+                                           yp.ptr, None) == _lib.RFK_ERR_INVALID
 
 
 @pytest.mark.parametrize("N,H,W", [(1, 24, 40), (2, 16, 72), (1, 40, 536)])
