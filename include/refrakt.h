@@ -391,6 +391,14 @@ int rfk_nn_conv3x3_forward_ex(const void* x1, int c1, int x1_mode, const void* x
 int rfk_nn_conv3x3_forward_pool(const void* x1, int c1, int x1_mode, const void* x2, int c2, int N, int H, int W,
                                 const void* wgt, const float* bias, int Cout, int act, void* y, void* ypool,
                                 void* stream);
+/* two full-resolution layers in one pass, A = act_a(conv3x3(x) + ba) (x:
+ * cin <= 32 channels, A: 32, never stored), then y = act_b(conv3x3(A) + bb)
+ * (BF16, cout_b = 32, ypool its 2x2 average pool or null) or, with out_rgb
+ * instead of y, out_rgb = clamp(base + act_b(conv3x3(A) + bb)[:3], 0, 1)
+ * (3 <= cout_b <= 8): equal to the layers run one by one                  */
+int rfk_nn_conv3x3_pair_forward(const void* x, int cin, int N, int H, int W, const void* wa, const float* ba,
+                                int act_a, const void* wb, const float* bb, int cout_b, int act_b, void* y,
+                                void* ypool, const float* base, float* out_rgb, void* stream);
 /* y = 2x2 average pool of x (N x H x W x C BF16, C a multiple of 8)       */
 int rfk_nn_avgpool2(const void* x, int N, int H, int W, int C, void* y, void* stream);
 int rfk_nn_conv3x3_wgrad(const void* x1, int c1, const void* x2, int c2, const void* dy, int Cout, int N, int H,

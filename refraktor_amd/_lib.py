@@ -157,6 +157,8 @@ _SIGS = [
     ("rfk_nn_conv3x3_forward_pool", C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P,
                                               C.c_int, C.c_int, _P, _P, _P]),
     ("rfk_nn_avgpool2", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P]),
+    ("rfk_nn_conv3x3_pair_forward", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P,
+                                              C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     ("rfk_nn_conv3x3_wgrad", C.c_int, [_P, C.c_int, _P, C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P]),
     ("rfk_nn_conv3x3_dgrad", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, C.c_int, _P, _P]),
     ("rfk_nn_gather_crops", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, C.c_int, C.c_int, C.c_int, _P,

@@ -233,8 +233,23 @@ int forward(rfk_denoiser* d, int N, int H, int W, const void* feat, float* rout,
         void *c0 = A("c0", 0, ch[0]), *d0 = A("d0", 0, ch[0]);
         if (rc) return rc;
         // the encoder's average pools come out of the convolutions' epilogues
-        if ((rc = conv(0, feat, nullptr, 0, a0, nullptr))) return rc;
-        if ((rc = conv(1, a0, nullptr, 0, e0, nullptr, 0, p0))) return rc;
+        // features -> c0 -> c1 (+ pool) in one pass where the shapes allow
+        {
+            const conv_def &ca = d->convs[0], &cb = d->convs[1];
+            const hipError_t e = ca.c2 == 0 && cb.c2 == 0 && cb.c1 == ca.cout
+                                     ? rfk::nn::conv3x3_pair_forward(
+                                           feat, ca.c1, N, H, W, static_cast<char*>(d->wbf) + ca.w_off * 2,
+                                           d->b + ca.b_off, ca.act, static_cast<char*>(d->wbf) + cb.w_off * 2,
+                                           d->b + cb.b_off, cb.cout, cb.act, e0, p0, nullptr, nullptr, s)
+                                     : hipErrorInvalidValue;
+            if (e == hipErrorInvalidValue) {
+                (void)hipGetLastError();
+                if ((rc = conv(0, feat, nullptr, 0, a0, nullptr))) return rc;
+                if ((rc = conv(1, a0, nullptr, 0, e0, nullptr, 0, p0))) return rc;
+            } else if (e != hipSuccess) {
+                return fail(RFK_ERR_HIP, std::string("first level: ") + hipGetErrorString(e));
+            }
+        }
         if ((rc = conv(2, p0, nullptr, 1, a1, nullptr))) return rc;
         if ((rc = conv(3, a1, nullptr, 1, e1, nullptr, 0, p1))) return rc;
         if ((rc = conv(4, p1, nullptr, 2, a2, nullptr))) return rc;
@@ -246,6 +261,22 @@ int forward(rfk_denoiser* d, int N, int H, int W, const void* feat, float* rout,
         if ((rc = conv(10, d2, e1, 1, c1, nullptr, 1))) return rc;
         if ((rc = conv(11, c1, nullptr, 1, d1, nullptr))) return rc;
         if ((rc = conv(12, d1, e0, 0, c0, nullptr, 1))) return rc;
+        if (out_rgb) {
+            // c13 -> output layer -> rgb in one pass where the shapes allow
+            const conv_def &ca = d->convs[13], &cb = d->convs[14];
+            const hipError_t e = ca.c2 == 0 && cb.c2 == 0 && cb.c1 == ca.cout
+                                     ? rfk::nn::conv3x3_pair_forward(
+                                           c0, ca.c1, N, H, W, static_cast<char*>(d->wbf) + ca.w_off * 2,
+                                           d->b + ca.b_off, ca.act, static_cast<char*>(d->wbf) + cb.w_off * 2,
+                                           d->b + cb.b_off, cb.cout, cb.act, nullptr, nullptr, base, out_rgb, s)
+                                     : hipErrorInvalidValue;
+            if (e == hipSuccess) {
+                if (rgb_done) *rgb_done = true;
+                return RFK_OK;
+            }
+            if (e != hipErrorInvalidValue) return fail(RFK_ERR_HIP, std::string("last level: ") + hipGetErrorString(e));
+            (void)hipGetLastError();
+        }
         if ((rc = conv(13, c0, nullptr, 0, d0, nullptr))) return rc;
         if (out_rgb) {
             const conv_def& c = d->convs[14];
@@ -603,6 +634,16 @@ int rfk_nn_conv3x3_forward_pool(const void* x1, int c1, int x1_mode, const void*
     if ((H | W) & 1 || Cout % 8) return fail(RFK_ERR_INVALID, "pooling needs even H and W and Cout a multiple of 8");
     NN_HIP(rfk::nn::conv3x3_forward(x1, c1, x2, c2, N, H, W, wgt, bias, Cout, act, 0.f, y, nullptr,
                                     static_cast<hipStream_t>(stream), x1_mode, ypool));
+    return RFK_OK;
+}
+
+int rfk_nn_conv3x3_pair_forward(const void* x, int cin, int N, int H, int W, const void* wa, const float* ba,
+                                int act_a, const void* wb, const float* bb, int cout_b, int act_b, void* y,
+                                void* ypool, const float* base, float* out_rgb, void* stream) {
+    const hipError_t e = rfk::nn::conv3x3_pair_forward(x, cin, N, H, W, wa, ba, act_a, wb, bb, cout_b, act_b, y, ypool,
+                                                       base, out_rgb, static_cast<hipStream_t>(stream));
+    if (e == hipErrorInvalidValue) return fail(RFK_ERR_INVALID, "conv3x3_pair_forward: unsupported shape or arguments");
+    if (e != hipSuccess) return fail(RFK_ERR_HIP, std::string("conv3x3_pair_forward: ") + hipGetErrorString(e));
     return RFK_OK;
 }
 

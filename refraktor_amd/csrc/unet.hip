@@ -911,6 +911,329 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
     }
 }
 
+// The two full-resolution convolutions at either end of the network in one
+// pass, so the 32-channel activation between them never leaves the CU:
+// A (Cin <= KA input channels -> 32, act_a) and then
+//   BKIND 0: B (32 -> 32, act_b) written as BF16, with its 2x2 average pool
+//            (the encoder's first level: features -> c0 -> c1 -> pool);
+//   BKIND 1: the output layer (32 -> at most 8 channels, taps stacked as in
+//            conv3x3_rgb8_kernel), rgb = clamp(base + act_b(first three))
+//            (the decoder's end: c13 -> c14 -> rgb).
+// Every wave owns a 32-column strip of A's output (image columns xs - 1 ..
+// xs + 30, xs a multiple of 30) and produces B's output for the 30 columns
+// xs .. xs + 29; it walks down its rows as the strip kernel does: an input
+// row (fetched three rows ahead by buffer loads) feeds A's three rotating
+// accumulators, A's completed row goes through LDS (zero outside the image,
+// B's padding) into B's accumulators, and B's completed row is stored. The
+// MFMA sequences are the strip kernel's (and the stacked output kernel's),
+// so the results equal the two separate layers bit for bit.
+template <int KA, int BKIND, int NW>
+__global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __restrict__ x, int Cin, int H, int W,
+                                                           const bf16* __restrict__ wa, const float* __restrict__ ba,
+                                                           int act_a, const bf16* __restrict__ wb,
+                                                           const float* __restrict__ bb, int cout_b, int act_b,
+                                                           int rows, bf16* __restrict__ y, bf16* __restrict__ ypool,
+                                                           const float* __restrict__ rbase, float* __restrict__ rgb) {
+    constexpr int SPX = 34, XP = KA + PAD, G = KA / 8, NQ = (SPX * G + 63) / 64, KSA = KA / 16, NSA = 3 * KSA;
+    constexpr int WPA = KA + PAD, MP = 32 + PAD, OXP = 32 + PAD, XS = strip_wave_lds(KA);
+    constexpr int WB_ROWS = BKIND == 0 ? 9 : 3;  // B's weight rows per output row of the MFMA tile
+    static_assert(64 % G == 0, "a lane's channel group is the same in all its loads");
+    extern __shared__ __attribute__((aligned(16))) unsigned char conv_smem[];
+    const int t = int(threadIdx.x), wv = t >> 6, l = t & 63, m = l & 31, kh = l >> 5;
+    // [32 out][9 taps][WPA]; B: [32 out][9 taps][MP] or [32 rows 8 dy + co][3 dx][MP]
+    bf16* s_wa = reinterpret_cast<bf16*>(conv_smem);
+    bf16* s_wb = s_wa + 32 * 9 * WPA;
+    float* s_ba = reinterpret_cast<float*>(s_wb + 32 * WB_ROWS * MP);
+    float* s_bb = s_ba + 32;
+    bf16* s_x = reinterpret_cast<bf16*>(s_bb + 32) + wv * (XS + SPX * MP);  // staged input row / output row
+    bf16* s_m = s_x + XS;                                                  // A's row [34][MP]
+    const int xs = (int(blockIdx.x) * NW + wv) * 30;
+    const int y0 = int(blockIdx.y) * rows;
+    const int n = int(blockIdx.z);
+    const bf16* xn = x + size_t(n) * H * W * Cin;
+    const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(xn, unsigned(H * W * Cin) * 2u);
+
+    for (int e = t; e < 32 * 9 * (WPA / 8); e += 64 * NW) {
+        const int g = e % (WPA / 8), tap = (e / (WPA / 8)) % 9, co = e / (9 * (WPA / 8));
+        bf16x8 v = zero8();
+        if (g * 8 < Cin) v = load8(wa + unsigned((co * 9 + tap) * Cin + g * 8));
+        *reinterpret_cast<bf16x8*>(s_wa + (co * 9 + tap) * WPA + g * 8) = v;
+    }
+    for (int e = t; e < 32 * WB_ROWS * (MP / 8); e += 64 * NW) {
+        const int g = e % (MP / 8), j = (e / (MP / 8)) % WB_ROWS, r = e / (WB_ROWS * (MP / 8));
+        bf16x8 v = zero8();
+        if (g < 4) {
+            if (BKIND == 0) {
+                if (r < cout_b) v = load8(wb + unsigned((r * 9 + j) * 32 + g * 8));
+            } else {
+                const int dy = r >> 3, co = r & 7;
+                if (dy < 3 && co < cout_b) v = load8(wb + unsigned((co * 9 + dy * 3 + j) * 32 + g * 8));
+            }
+        }
+        *reinterpret_cast<bf16x8*>(s_wb + (r * WB_ROWS + j) * MP + g * 8) = v;
+    }
+    if (t < 32) {
+        s_ba[t] = ba ? ba[t] : 0.f;
+        s_bb[t] = (bb && t < cout_b) ? bb[t] : 0.f;
+    }
+    // A's row columns 32 and 33 are B's operands only for the two unused lanes
+    if (l < 2 * (MP / 8)) *reinterpret_cast<bf16x8*>(s_m + (32 + l / (MP / 8)) * MP + (l % (MP / 8)) * 8) = zero8();
+
+    // the lane's part of a staged input row: channel group g, columns (l + 64 q) / G
+    const int g = l % G;
+    unsigned oc[NQ];
+    bool colok[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int col = (l + 64 * q) / G, gx = xs - 2 + col;
+        colok[q] = col < SPX && gx >= 0 && gx < W;
+        oc[q] = unsigned(gx * Cin + g * 8) * 2u;
+    }
+    const bool ch_ok = g * 8 < Cin;
+    // input row jj = image row y0 - 2 + jj
+    auto load_row = [&](int jj, bf16x8* v) __attribute__((always_inline)) {
+        const int gy = y0 - 2 + jj;
+        const bool row_ok = gy >= 0 && gy < H;
+        const unsigned srow = unsigned(gy * W * Cin) * 2u;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) v[q] = buffer_load8(rx, row_ok && colok[q] && ch_ok ? oc[q] + srow : kBufferOOB);
+    };
+    const int NJ = rows + 4;
+    bf16x8 pa[NQ], pb[NQ], pc[NQ];
+    load_row(0, pa);
+    load_row(1, pb);
+    load_row(2, pc);
+    __syncthreads();  // weights and biases staged; from here on every wave runs on its own
+    if (xs >= W) return;
+
+    const bool relu_a = act_a == 1, relu_b = act_b == 1;
+    const float neg_a = act_a == 2 ? 0.f : 1.f, neg_b = act_b == 2 ? 0.f : 1.f;
+    const int mx = xs - 1 + m;  // image column of A's output in this lane
+    f32x16 a0, a1, a2, b0, b1, b2;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        a0[i] = 0.f;
+        a1[i] = 0.f;
+        a2[i] = 0.f;
+        b0[i] = 0.f;
+        b1[i] = 0.f;
+        b2[i] = 0.f;
+    }
+    // the output layer: group 0 of A's two previous rows, group 1 of the previous one
+    float q0a[3] = {0.f, 0.f, 0.f}, q0b[3] = {0.f, 0.f, 0.f}, q1a[3] = {0.f, 0.f, 0.f};
+    bf16x8 prev[2];
+
+    // B's output row yy (lane-transposed through s_x, as the strip kernel stores)
+    auto store_b = [&](int yy, const f32x16& acc) __attribute__((always_inline)) {
+        __asm__ volatile("" ::: "memory");
+        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float4 bq = *reinterpret_cast<const float4*>(s_bb + 8 * i + 4 * kh);
+            const float bj[4] = {bq.x, bq.y, bq.z, bq.w};
+            bf16x4 b;
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[j] = bf16(act_select(acc[i * 4 + j] + bj[j], relu_b, neg_b));
+            *reinterpret_cast<bf16x4*>(s_x + m * OXP + 8 * i + 4 * kh) = b;
+        }
+        __asm__ volatile("" ::: "memory");
+        const size_t o = (size_t(n) * H + yy) * W;
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const int p = r * 16 + (l >> 2), cg = l & 3;
+            const bf16x8 v = *reinterpret_cast<const bf16x8*>(s_x + p * OXP + cg * 8);
+            const bool keep = p < 30 && xs + p < W;
+            if (keep) *reinterpret_cast<bf16x8*>(y + (o + xs + p) * 32 + cg * 8) = v;
+            if (ypool) {
+                if ((yy & 1) == 0) {
+                    prev[r] = v;
+                } else {
+                    const u32x4 pu = __builtin_bit_cast(u32x4, prev[r]), vu = __builtin_bit_cast(u32x4, v);
+                    u32x4 nu, nv;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        nu[j] = unsigned(__shfl_down(int(pu[j]), 4));
+                        nv[j] = unsigned(__shfl_down(int(vu[j]), 4));
+                    }
+                    const bf16x8 b = __builtin_bit_cast(bf16x8, nu), d = __builtin_bit_cast(bf16x8, nv);
+                    bf16x8 pool;
+#pragma unroll
+                    for (int j = 0; j < 8; j++)
+                        pool[j] = bf16(0.25f * (float(prev[r][j]) + float(b[j]) + float(v[j]) + float(d[j])));
+                    if (keep && (p & 1) == 0) {
+                        const size_t po = (size_t(n) * (H / 2) + yy / 2) * (W / 2) + (xs + p) / 2;
+                        *reinterpret_cast<bf16x8*>(ypool + po * 32 + cg * 8) = pool;
+                    }
+                }
+            }
+        }
+        __asm__ volatile("" ::: "memory");
+    };
+
+    // input row jj: A's accumulators hold A rows jj, jj - 1, jj - 2 (aA, aB,
+    // aC); B's hold B rows jj - 2, jj - 3, jj - 4 (bA, bB, bC), in the
+    // relative numbering where A row q is image row y0 - 1 + q and B row o
+    // image row y0 + o
+    auto row = [&](auto rc, int jj, f32x16& aA, f32x16& aB, f32x16& aC, f32x16& bA, f32x16& bB, f32x16& bC)
+        __attribute__((always_inline)) {
+        constexpr int R = decltype(rc)::value;
+        auto& buf = [&]() __attribute__((always_inline)) -> bf16x8 (&)[NQ] {
+            if constexpr (R == 0) return pa;
+            else if constexpr (R == 1) return pb;
+            else return pc;
+        }();
+        __asm__ volatile("" ::: "memory");
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const int col = (l + 64 * q) / G;
+            if (col < SPX) *reinterpret_cast<bf16x8*>(s_x + col * XP + g * 8) = buf[q];
+        }
+        __asm__ volatile("" ::: "memory");
+        if (jj + 3 < NJ) load_row(jj + 3, buf);
+        const int gy = y0 - 2 + jj;
+        if (gy >= 0 && gy < H) {
+            const bf16* sx = s_x + m * XP + kh * 8;
+            const bf16* sw = s_wa + m * 9 * WPA + kh * 8;
+            auto rxa = [&](int i) __attribute__((always_inline)) {
+                return *reinterpret_cast<const bf16x8*>(sx + (i / KSA) * XP + (i % KSA) * 16);
+            };
+            auto rwa = [&](int i, int dy) __attribute__((always_inline)) {
+                return *reinterpret_cast<const bf16x8*>(sw + (3 * dy + i / KSA) * WPA + (i % KSA) * 16);
+            };
+            bf16x8 xc = rxa(0), w0 = rwa(0, 0), w1 = rwa(0, 1), w2 = rwa(0, 2);
+#pragma unroll
+            for (int i = 0; i < NSA; i++) {
+                const bool more = i + 1 < NSA;
+                const bf16x8 xn2 = more ? rxa(i + 1) : xc;
+                __builtin_amdgcn_sched_barrier(0);
+                aA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, xc, aA, 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (more) w0 = rwa(i + 1, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                aB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, xc, aB, 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (more) w1 = rwa(i + 1, 1);
+                __builtin_amdgcn_sched_barrier(0);
+                aC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2, xc, aC, 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (more) w2 = rwa(i + 1, 2);
+                xc = xn2;
+            }
+        }
+        if (jj >= 2) {
+            // A row q = jj - 2 (image row y0 - 3 + jj) is complete: activation,
+            // BF16, zero outside the image, into s_m for B
+            const int qy = y0 - 3 + jj;
+            const bool ok = qy >= 0 && qy < H && mx >= 0 && mx < W;
+            typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+            __asm__ volatile("" ::: "memory");
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float4 bq = *reinterpret_cast<const float4*>(s_ba + 8 * i + 4 * kh);
+                const float bj[4] = {bq.x, bq.y, bq.z, bq.w};
+                bf16x4 b;
+#pragma unroll
+                for (int j = 0; j < 4; j++) b[j] = bf16(ok ? act_select(aC[i * 4 + j] + bj[j], relu_a, neg_a) : 0.f);
+                *reinterpret_cast<bf16x4*>(s_m + m * MP + 8 * i + 4 * kh) = b;
+            }
+            __asm__ volatile("" ::: "memory");
+            const bool q_in = qy >= 0 && qy < H;
+            const bf16* sm = s_m + m * MP + kh * 8;
+            auto rxb = [&](int i) __attribute__((always_inline)) {
+                return *reinterpret_cast<const bf16x8*>(sm + (i / 2) * MP + (i % 2) * 16);
+            };
+            if constexpr (BKIND == 0) {
+                if (q_in) {
+                    const bf16* sw = s_wb + m * 9 * MP + kh * 8;
+                    auto rwb = [&](int i, int dy) __attribute__((always_inline)) {
+                        return *reinterpret_cast<const bf16x8*>(sw + (3 * dy + i / 2) * MP + (i % 2) * 16);
+                    };
+                    bf16x8 xc = rxb(0), w0 = rwb(0, 0), w1 = rwb(0, 1), w2 = rwb(0, 2);
+#pragma unroll
+                    for (int i = 0; i < 6; i++) {
+                        const bool more = i + 1 < 6;
+                        const bf16x8 xn2 = more ? rxb(i + 1) : xc;
+                        __builtin_amdgcn_sched_barrier(0);
+                        bA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, xc, bA, 0, 0, 0);
+                        __builtin_amdgcn_sched_barrier(0);
+                        if (more) w0 = rwb(i + 1, 0);
+                        __builtin_amdgcn_sched_barrier(0);
+                        bB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, xc, bB, 0, 0, 0);
+                        __builtin_amdgcn_sched_barrier(0);
+                        if (more) w1 = rwb(i + 1, 1);
+                        __builtin_amdgcn_sched_barrier(0);
+                        bC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2, xc, bC, 0, 0, 0);
+                        __builtin_amdgcn_sched_barrier(0);
+                        if (more) w2 = rwb(i + 1, 2);
+                        xc = xn2;
+                    }
+                }
+                // B row o = jj - 4 (image row y0 + jj - 4) is complete
+                const int oy = y0 + jj - 4;
+                if (jj >= 4 && oy < H) store_b(oy, bC);
+#pragma unroll
+                for (int i = 0; i < 16; i++) bC[i] = 0.f;
+            } else {
+                // the stacked output layer: one accumulator per dx
+                f32x16 acc[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++)
+#pragma unroll
+                    for (int i = 0; i < 16; i++) acc[d][i] = 0.f;
+                if (q_in) {
+                    const bf16* sw = s_wb + m * 3 * MP + kh * 8;
+#pragma unroll
+                    for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+                        for (int ks = 0; ks < 2; ks++) {
+                            const bf16x8 xb = rxb(dx * 2 + ks);
+                            const bf16x8 wv8 = *reinterpret_cast<const bf16x8*>(sw + dx * MP + ks * 16);
+                            acc[dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv8, xb, acc[dx], 0, 0, 0);
+                        }
+                }
+                float g0[3], g1[3], g2[3];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    g0[c] = (acc[0][c] + acc[1][c]) + acc[2][c];
+                    g1[c] = (acc[0][4 + c] + acc[1][4 + c]) + acc[2][4 + c];
+                    g2[c] = (acc[0][8 + c] + acc[1][8 + c]) + acc[2][8 + c];
+                }
+                // output row of A row q - 1 ... : image row qy - 1 = y0 + jj - 4
+                const int oy = y0 + jj - 4;
+                const int px = xs + m;
+                if (jj >= 4 && oy < H && kh == 0 && m < 30 && px < W) {
+                    const size_t qo = ((size_t(n) * H + oy) * W + px) * 3;
+                    const float v0 = q0b[0] + q1a[0] + g2[0], v1 = q0b[1] + q1a[1] + g2[1],
+                                v2 = q0b[2] + q1a[2] + g2[2];
+                    rgb[qo + 0] = fminf(fmaxf(rbase[qo + 0] + act_fwd(v0 + s_bb[0], act_b, 0.f), 0.f), 1.f);
+                    rgb[qo + 1] = fminf(fmaxf(rbase[qo + 1] + act_fwd(v1 + s_bb[1], act_b, 0.f), 0.f), 1.f);
+                    rgb[qo + 2] = fminf(fmaxf(rbase[qo + 2] + act_fwd(v2 + s_bb[2], act_b, 0.f), 0.f), 1.f);
+                }
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    q0b[c] = q0a[c];
+                    q0a[c] = g0[c];
+                    q1a[c] = g1[c];
+                }
+            }
+        }
+        // A row jj - 2 is done (or, before row 2, outside the strip): aC
+        // starts over as row jj + 1
+#pragma unroll
+        for (int i = 0; i < 16; i++) aC[i] = 0.f;
+    };
+    for (int jb = 0; jb < NJ; jb += 3) {
+        static_for<0, 3>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int R = decltype(rc)::value;
+            const int jj = jb + R;
+            if (jj >= NJ) return;
+            if constexpr (R == 0) row(rc, jj, a0, a2, a1, b0, b2, b1);
+            else if constexpr (R == 1) row(rc, jj, a1, a0, a2, b1, b0, b2);
+            else row(rc, jj, a2, a1, a0, b2, b1, b0);
+        });
+    }
+}
+
 // The network's output layer (at most 8 channels, rgb = clamp(base + act(
 // channels 0..2 + bias))): the three row taps' weights are stacked into one
 // 32-row MFMA tile, row 8 dy + co (rows 24..31 zero), so one MFMA per
@@ -1229,7 +1552,7 @@ static int strip_waves(int mode, int Cin, int KC) {
 // a launch takes ceil(workgroups / resident workgroups) rounds, so pick the
 // row count that minimises rounds x (rows + 2); even counts only when the
 // kernel also pools.
-static int strip_rows(const void* fn, size_t lds, int nw, int zdim, int H, int W, bool even) {
+static int strip_rows(const void* fn, size_t lds, int nw, int zdim, int H, int W, bool even, int px = 32) {
     thread_local std::unordered_map<size_t, int> resident_cache;
     int resident = 0;
     const size_t key = reinterpret_cast<size_t>(fn) * 1000003u + lds;
@@ -1244,7 +1567,7 @@ static int strip_rows(const void* fn, size_t lds, int nw, int zdim, int H, int W
             return 16;
         resident = resident_cache[key] = std::max(1, cus * per_cu);
     }
-    const long cols = (W + 32 * nw - 1) / (32 * nw);
+    const long cols = (W + px * nw - 1) / (px * nw);
     int best = 16;
     long best_cost = -1;
     for (int r = 4; r <= 64; r += even ? 2 : 1) {
@@ -1404,6 +1727,45 @@ hipError_t conv3x3_forward_rgb(const void* x, int cin, int N, int H, int W, cons
         return hipErrorInvalidValue;
     const strip_args a{x, cin, nullptr, 0, N, H, W, wgt, bias, Cout, act, 0.f, nullptr, nullptr, nullptr, base, out_rgb};
     return KC == 16 ? launch_strip<16>(a, 0, nw, lds, s) : launch_strip<32>(a, 0, nw, lds, s);
+}
+
+template <int KA, int BKIND>
+static hipError_t launch_pair(const void* x, int cin, int N, int H, int W, const void* wa, const float* ba, int act_a,
+                              const void* wb, const float* bb, int cout_b, int act_b, void* y, void* ypool,
+                              const float* base, float* out_rgb, hipStream_t s) {
+    constexpr int NW = 8, MP = 32 + PAD;
+    const size_t lds = (size_t(32) * 9 * (KA + PAD) + size_t(32) * (BKIND == 0 ? 9 : 3) * MP + 128 +
+                        size_t(NW) * (strip_wave_lds(KA) + 34 * MP)) *
+                       sizeof(bf16);
+    const void* fn = reinterpret_cast<const void*>(&conv3x3_pair_kernel<KA, BKIND, NW>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+    const int rows = strip_rows(fn, lds, NW, N, H, W, ypool != nullptr, 30);
+    dim3 grid((W + 30 * NW - 1) / (30 * NW), (H + rows - 1) / rows, N);
+    conv3x3_pair_kernel<KA, BKIND, NW><<<grid, 64 * NW, lds, s>>>((const bf16*)x, cin, H, W, (const bf16*)wa, ba,
+                                                                   act_a, (const bf16*)wb, bb, cout_b, act_b, rows,
+                                                                   (bf16*)y, (bf16*)ypool, base, out_rgb);
+    return hipGetLastError();
+}
+
+hipError_t conv3x3_pair_forward(const void* x, int cin, int N, int H, int W, const void* wa, const float* ba,
+                                int act_a, const void* wb, const float* bb, int cout_b, int act_b, void* y,
+                                void* ypool, const float* base, float* out_rgb, hipStream_t s) {
+    if (!x || !wa || !wb || cin <= 0 || cin > 32 || (cin % 8) || N <= 0 || H <= 0 || W <= 0) return hipErrorInvalidValue;
+    if (size_t(H) * W * 32 * 2 >= (size_t(1) << 30)) return hipErrorInvalidValue;
+    const bool rgb_out = out_rgb != nullptr;
+    if (rgb_out == (y != nullptr)) return hipErrorInvalidValue;
+    if (rgb_out && (!base || cout_b < 3 || cout_b > 8 || ypool)) return hipErrorInvalidValue;
+    if (!rgb_out && (cout_b != 32 || (ypool && ((H | W) & 1)))) return hipErrorInvalidValue;
+    if (cin <= 16)
+        return rgb_out ? launch_pair<16, 1>(x, cin, N, H, W, wa, ba, act_a, wb, bb, cout_b, act_b, y, ypool, base,
+                                            out_rgb, s)
+                       : launch_pair<16, 0>(x, cin, N, H, W, wa, ba, act_a, wb, bb, cout_b, act_b, y, ypool, base,
+                                            out_rgb, s);
+    return rgb_out ? launch_pair<32, 1>(x, cin, N, H, W, wa, ba, act_a, wb, bb, cout_b, act_b, y, ypool, base, out_rgb,
+                                        s)
+                   : launch_pair<32, 0>(x, cin, N, H, W, wa, ba, act_a, wb, bb, cout_b, act_b, y, ypool, base, out_rgb,
+                                        s);
 }
 
 size_t conv3x3_wgrad_workspace(int c1, int c2, int Cout, int N, int H, int W, int* groups, int* tiles_per_group) {

@@ -23,6 +23,16 @@ hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
 // N*H*W*3). hipErrorInvalidValue when the strip kernel does not take the shape.
 hipError_t conv3x3_forward_rgb(const void* x, int cin, int N, int H, int W, const void* wgt, const float* bias,
                                int Cout, int act, const float* base, float* out_rgb, hipStream_t s);
+// Two full-resolution layers in one pass: A = act_a(conv3x3(x) + ba), x with
+// cin <= 32 channels, A 32 channels (never stored), then either
+//   y (BF16, 32 channels) = act_b(conv3x3(A) + bb), cout_b = 32, and with
+//   ypool its 2x2 average pool (even H and W); or
+//   out_rgb = clamp(base + act_b(conv3x3(A) + bb)[:3], 0, 1), 3 <= cout_b <= 8.
+// Equal bit for bit to conv3x3_forward twice (and conv3x3_forward_rgb);
+// hipErrorInvalidValue for other shapes.
+hipError_t conv3x3_pair_forward(const void* x, int cin, int N, int H, int W, const void* wa, const float* ba,
+                                int act_a, const void* wb, const float* bb, int cout_b, int act_b, void* y,
+                                void* ypool, const float* base, float* out_rgb, hipStream_t s);
 // bytes of FP32 workspace conv3x3_wgrad needs
 size_t conv3x3_wgrad_workspace(int c1, int c2, int Cout, int N, int H, int W, int* groups, int* tiles_per_group);
 // dw = beta * dw + dL/dW (FP32, [Cout][9][Cin])

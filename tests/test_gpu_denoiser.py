@@ -151,6 +151,70 @@ def test_conv_pool_epilogue(gpu, N, H, W, c1, Co):
                                            yp.ptr, None) == _lib.RFK_ERR_INVALID
 
 
+@pytest.mark.parametrize("N,H,W,cin", [(1, 20, 300, 8), (2, 14, 66, 16), (1, 12, 100, 32), (1, 30, 250, 24),
+                                       (1, 2, 34, 8)])
+def test_conv_pair_equals_layers(gpu, N, H, W, cin):
+    """two full-resolution layers in one pass (the first level: features ->
+    32 -> 32 + pool) equal the layers run one by one, bit for bit: strips of
+    30 columns against the single layers' 32, partial strips at the right
+    edge, batches of two, images of two rows"""
+    rng = np.random.default_rng(11 + N + H + W + cin)
+    x = nn.to_bf16(rng.normal(size=(N, H, W, cin)).astype(np.float32))
+    wa = nn.to_bf16((rng.normal(size=(32, 9, cin)) * 0.2).astype(np.float32))
+    wb = nn.to_bf16((rng.normal(size=(32, 9, 32)) * 0.1).astype(np.float32))
+    ba, bb = rng.normal(size=32).astype(np.float32), rng.normal(size=32).astype(np.float32)
+    dx, dwa, dwb, dba, dbb = _bf16_dev(x), _bf16_dev(wa), _bf16_dev(wb), _dev(ba), _dev(bb)
+    lib = _lib.lib()
+    y, yp = render.DeviceBuffer(N * H * W * 64), render.DeviceBuffer(N * H * W * 16)
+    _lib.check(lib.rfk_nn_conv3x3_pair_forward(dx.ptr, cin, N, H, W, dwa.ptr, dba.ptr, 1, dwb.ptr, dbb.ptr, 32, 1,
+                                               y.ptr, yp.ptr, None, None, None))
+    a, y2, yp2 = (render.DeviceBuffer(N * H * W * 64), render.DeviceBuffer(N * H * W * 64),
+                  render.DeviceBuffer(N * H * W * 16))
+    _lib.check(lib.rfk_nn_conv3x3_forward_ex(dx.ptr, cin, 0, None, 0, N, H, W, dwa.ptr, dba.ptr, 32, 1, a.ptr, None,
+                                             None))
+    _lib.check(lib.rfk_nn_conv3x3_forward_pool(a.ptr, 32, 0, None, 0, N, H, W, dwb.ptr, dbb.ptr, 32, 1, y2.ptr,
+                                               yp2.ptr, None))
+    _lib.check(lib.rfk_synchronize())
+    got = y.download(np.uint16, (N, H, W, 32))
+    assert np.array_equal(got, y2.download(np.uint16, (N, H, W, 32)))
+    assert np.array_equal(yp.download(np.uint16, (N, H // 2, W // 2, 32)),
+                          yp2.download(np.uint16, (N, H // 2, W // 2, 32)))
+    assert np.count_nonzero(got) > got.size // 4
+
+
+@pytest.mark.parametrize("N,H,W,cout", [(1, 19, 300, 8), (2, 9, 45, 3), (1, 40, 536, 8)])
+def test_conv_pair_output_layer(gpu, N, H, W, cout):
+    """the last level in one pass (32 -> 32 -> the output layer, rgb =
+    clamp(base + first three)) against the layers run one by one with FP32
+    outputs (the output layer sums its taps in another order: 1e-5)"""
+    rng = np.random.default_rng(5 + N + H + W + cout)
+    x = nn.to_bf16(rng.normal(size=(N, H, W, 32)).astype(np.float32))
+    wa = nn.to_bf16((rng.normal(size=(32, 9, 32)) * 0.1).astype(np.float32))
+    wb = nn.to_bf16((rng.normal(size=(cout, 9, 32)) * 0.1).astype(np.float32))
+    ba, bb = rng.normal(size=32).astype(np.float32), (rng.normal(size=cout) * 0.1).astype(np.float32)
+    base = rng.uniform(size=(N, H, W, 3)).astype(np.float32)
+    dx, dwa, dwb, dba, dbb, dbase = _bf16_dev(x), _bf16_dev(wa), _bf16_dev(wb), _dev(ba), _dev(bb), _dev(base)
+    lib = _lib.lib()
+    out = render.DeviceBuffer(N * H * W * 12)
+    _lib.check(lib.rfk_nn_conv3x3_pair_forward(dx.ptr, 32, N, H, W, dwa.ptr, dba.ptr, 1, dwb.ptr, dbb.ptr, cout, 0,
+                                               None, None, dbase.ptr, out.ptr, None))
+    a, r = render.DeviceBuffer(N * H * W * 64), render.DeviceBuffer(N * H * W * cout * 4)
+    _lib.check(lib.rfk_nn_conv3x3_forward_ex(dx.ptr, 32, 0, None, 0, N, H, W, dwa.ptr, dba.ptr, 32, 1, a.ptr, None,
+                                             None))
+    _lib.check(lib.rfk_nn_conv3x3_forward_ex(a.ptr, 32, 0, None, 0, N, H, W, dwb.ptr, dbb.ptr, cout, 0, None, r.ptr,
+                                             None))
+    _lib.check(lib.rfk_synchronize())
+    want = np.clip(base + r.download(np.float32, (N, H, W, cout))[..., :3], 0, 1)
+    got = out.download(np.float32, (N, H, W, 3))
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-5)
+    assert 0.05 < np.mean((got > 0) & (got < 1))
+    # one of y / out_rgb, never both; out_rgb needs base
+    assert lib.rfk_nn_conv3x3_pair_forward(dx.ptr, 32, N, H, W, dwa.ptr, dba.ptr, 1, dwb.ptr, dbb.ptr, cout, 0,
+                                           a.ptr, None, dbase.ptr, out.ptr, None) == _lib.RFK_ERR_INVALID
+    assert lib.rfk_nn_conv3x3_pair_forward(dx.ptr, 32, N, H, W, dwa.ptr, dba.ptr, 1, dwb.ptr, dbb.ptr, cout, 0,
+                                           None, None, None, out.ptr, None) == _lib.RFK_ERR_INVALID
+
+
 @pytest.mark.parametrize("N,H,W", [(1, 24, 40), (2, 16, 72), (1, 40, 536)])
 def test_unet_inference_matches_oracle(gpu, N, H, W):
     """the fused inference forward of the whole network vs the float64 network;
