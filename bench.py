@@ -102,7 +102,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tpb", type=int, default=512, help="plain path walkers per workgroup (without autotuning)")
-    ap.add_argument("--den-tpb", type=int, default=512, help="denoised path walkers per workgroup (queue binning)")
+    ap.add_argument("--den-tpb", type=int, default=512, help="denoised path walkers per workgroup (without autotuning)")
+    ap.add_argument("--den-binning", choices=("queue", "atomic"), default="queue",
+                    help="denoised path binning (without autotuning)")
+    ap.add_argument("--den-tile", type=int, default=12, help="denoised path LDS bin cache slots (log2), atomic binning")
     ap.add_argument("--no-autotune", action="store_true",
                     help="plain path at --tpb / --binning / --walkers instead of the autotuned shape")
     ap.add_argument("--binning", choices=("queue", "atomic"), default="queue",
@@ -154,7 +157,12 @@ def main() -> int:
     tuned = traffic = None
     if not (args.tune_only or args.traffic_probe or args.unet_only):
         if not args.no_autotune:
-            tuned = run_child(["--tune-only", "--seed", str(args.seed)])["autotune"]
+            child = run_child(["--tune-only", "--seed", str(args.seed), "--low-walkers", str(args.low_walkers),
+                               "--low-iters", str(args.low_iters)])
+            tuned = child["autotune"]
+            tuned["denoised"] = child["autotune_denoised"]
+            dsh = tuned["denoised"]["shape"]
+            args.den_binning, args.den_tpb, args.den_tile = dsh["binning"], dsh["threads_per_block"], dsh["bin_tile_log2"]
             sh = tuned["shape"]
             args.binning, args.tpb, args.tile = sh["binning"], sh["threads_per_block"], sh["bin_tile_log2"]
             args.atomic_width, args.walkers = sh["atomic_width"], sh["walkers"]
@@ -196,7 +204,22 @@ def main() -> int:
                                  "walkers": r["walkers"], "iters_per_s": r["iters_per_s"]} for r in results],
                  "tune_s": time.perf_counter() - t0}
         if args.tune_only:
-            print(json.dumps({"autotune": tuned}), flush=True)
+            # the denoised path's shape: one bin call of its budget per
+            # candidate at its walker count (queue or atomic binning; the
+            # histogram is the same either way)
+            dgrid = ([autotune.Shape(tpb, 64, 0, args.low_walkers, 1) for tpb in (256, 512, 1024)] +
+                     [autotune.Shape(tpb, 64, t, args.low_walkers, 0) for tpb in (512, 1024) for t in (11, 12)])
+            t0 = time.perf_counter()
+            dshape, dres = autotune.tune(db, fo, fo.pack_samples(W, H), W, H, dgrid, args.low_iters, use_cache=False)
+            den = {"shape": {"binning": "queue" if dshape.bin_queue else "atomic",
+                             "threads_per_block": dshape.threads_per_block,
+                             "bin_tile_log2": 0 if dshape.bin_queue else dshape.bin_tile_log2,
+                             "walkers": dshape.walkers},
+                   "candidates": [{"binning": "queue" if r["bin_queue"] else "atomic",
+                                   "threads_per_block": r["threads_per_block"], "bin_tile_log2": r["bin_tile_log2"],
+                                   "walkers": r["walkers"], "iters_per_s": r["iters_per_s"]} for r in dres],
+                   "tune_s": time.perf_counter() - t0}
+            print(json.dumps({"autotune": tuned, "autotune_denoised": den}), flush=True)
             return 0
     queue = args.binning == "queue"
     if queue:
@@ -204,8 +227,11 @@ def main() -> int:
     k = db.compile(f, threads_per_block=args.tpb, atomic_width=args.atomic_width, bin_tile_log2=args.tile,
                    hist_bits=args.hist_bits, bin_queue=queue)
     k64 = db.compile(f, threads_per_block=args.tpb, bin_tile_log2=args.tile, hist_bits=64, bin_queue=queue)
-    # the denoised path's small budget: queue binning at its own shape
-    kd = db.compile(f, threads_per_block=args.den_tpb, atomic_width=64, hist_bits=32, bin_queue=True)
+    # the denoised path's small budget at its own shape
+    den_queue = args.den_binning == "queue"
+    den_tile = 0 if den_queue else args.den_tile
+    kd = db.compile(f, threads_per_block=args.den_tpb, atomic_width=64, bin_tile_log2=den_tile, hist_bits=32,
+                    bin_queue=den_queue)
     samples = k.pack_samples(W, H)
     gamma, bright, vib = f["gamma"], f["brightness"], f["vibrancy"]
     lib = _lib.lib()
@@ -393,7 +419,8 @@ def main() -> int:
                   "denoised": None if fps_den is None else
                   {"fps": fps_den, "ms": ms_den, "sequential_ms": seq_den, "iterations": args.low_iters,
                    "walkers": args.low_walkers, "de_radius": args.low_de_radius,
-                   "threads_per_block": args.den_tpb, "binning": "queue", "hist_bits": 32,
+                   "threads_per_block": args.den_tpb, "binning": args.den_binning, "bin_tile_log2": den_tile,
+                   "hist_bits": 32,
                    "psnr_db": psnr_den, "fallback_frames": fallbacks,
                    "fps_with_fallback": None if eff_ms_den is None else 1e3 / eff_ms_den,
                    "speedup_vs_plain": (1e3 / eff_ms_den if eff_ms_den else fps_den) / fps_plain,
@@ -453,7 +480,9 @@ def held_out_flames(args, db, lanes, plain, denoised, score, summarize) -> dict:
     frames = []
     for s in HELD_OUT_SEEDS:
         f = fl.flame_1080(s)
-        kern = db.compile(f, threads_per_block=args.den_tpb, atomic_width=64, hist_bits=32, bin_queue=True)
+        kern = db.compile(f, threads_per_block=args.den_tpb, atomic_width=64, hist_bits=32,
+                          bin_queue=args.den_binning == "queue",
+                          bin_tile_log2=0 if args.den_binning == "queue" else args.den_tile)
         k64 = db.compile(f, threads_per_block=args.tpb, bin_tile_log2=args.tile, hist_bits=64, bin_queue=queue)
         smp = kern.pack_samples(W, H)
         tone = (f["gamma"], f["brightness"], f["vibrancy"])

@@ -334,13 +334,15 @@ __global__ void tonemap_kernel(const void* __restrict__ in, size_t npix, void* _
         col.y *= factor;
         col.z *= factor;
         col.w *= factor;
-        const double inv_gamma = 1.0 / double(gamma);
-        const double z = pow(double(col.w), inv_gamma);
-        const double gamma_factor = z / double(col.w);
-        const float ig = float(inv_gamma);
-        o.x = sat(lerpf(powf(col.x, ig), float(double(col.x) * gamma_factor), vibrancy));
-        o.y = sat(lerpf(powf(col.y, ig), float(double(col.y) * gamma_factor), vibrancy));
-        o.z = sat(lerpf(powf(col.z, ig), float(double(col.z) * gamma_factor), vibrancy));
+        // tonemap.cu:88-94 takes z = w^(1/gamma) and the colour products in
+        // double; FP32 here (a few ulp of the FP32 result, far inside the
+        // tonemap's tolerance against the double oracle) keeps the kernel
+        // off the FP64 pipe
+        const float ig = float(1.0 / double(gamma));
+        const float gamma_factor = powf(col.w, ig) / col.w;
+        o.x = sat(lerpf(powf(col.x, ig), col.x * gamma_factor, vibrancy));
+        o.y = sat(lerpf(powf(col.y, ig), col.y * gamma_factor, vibrancy));
+        o.z = sat(lerpf(powf(col.z, ig), col.z * gamma_factor, vibrancy));
         o.w = sat(logf(input_density) * (4.0f / logf(2.0f)) / 96.0f);
     }
     if (Fmt == 0) {

@@ -59,6 +59,35 @@ __device__ __forceinline__ float act_select(float v, bool relu, float neg_slope)
     return v > 0.f ? v : (relu ? 0.f : v * neg_slope);
 }
 
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// one lane's 16 accumulator values (channels 8i + 4kh + j) plus their
+// biases (bias + 8i + 4kh + j), activated and rounded to BF16; the ReLU of
+// the hidden layers takes one max per value, the other activations go
+// through act_select
+__device__ __forceinline__ void act16_bf16(const f32x16& acc, const float* bias, int kh, bool relu, float neg_slope,
+                                           bf16x4 out[4]) {
+    if (relu) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float4 b = *reinterpret_cast<const float4*>(bias + 8 * i + 4 * kh);
+            out[i][0] = bf16(fmaxf(acc[i * 4 + 0] + b.x, 0.f));
+            out[i][1] = bf16(fmaxf(acc[i * 4 + 1] + b.y, 0.f));
+            out[i][2] = bf16(fmaxf(acc[i * 4 + 2] + b.z, 0.f));
+            out[i][3] = bf16(fmaxf(acc[i * 4 + 3] + b.w, 0.f));
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float4 b = *reinterpret_cast<const float4*>(bias + 8 * i + 4 * kh);
+            out[i][0] = bf16(act_select(acc[i * 4 + 0] + b.x, false, neg_slope));
+            out[i][1] = bf16(act_select(acc[i * 4 + 1] + b.y, false, neg_slope));
+            out[i][2] = bf16(act_select(acc[i * 4 + 2] + b.z, false, neg_slope));
+            out[i][3] = bf16(act_select(acc[i * 4 + 3] + b.w, false, neg_slope));
+        }
+    }
+}
+
 // ------------------------------------------------------------- forward
 // Output tile of a workgroup: 128*R consecutive pixels of one row x 32*C
 // output channels; wave w computes pixels [32R*w, +32R) x all 32C channels as
@@ -727,16 +756,10 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
             // the wave's reads of s_x for this row are issued before
             // these writes, its next chunk's writes after these reads
             __asm__ volatile("" ::: "memory");
-            typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+            bf16x4 b[4];
+            act16_bf16(acc, s_bias, kh, relu, neg_slope, b);
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const float4 bb = bias4(i);
-                const float bj[4] = {bb.x, bb.y, bb.z, bb.w};
-                bf16x4 b;
-#pragma unroll
-                for (int j = 0; j < 4; j++) b[j] = bf16(act_select(acc[i * 4 + j] + bj[j], relu, neg_slope));
-                *reinterpret_cast<bf16x4*>(s_x + m * OXP + 8 * i + 4 * kh) = b;
-            }
+            for (int i = 0; i < 4; i++) *reinterpret_cast<bf16x4*>(s_x + m * OXP + 8 * i + 4 * kh) = b[i];
             __asm__ volatile("" ::: "memory");
             const size_t o = (size_t(n) * H + yy) * W;
 #pragma unroll
@@ -1026,16 +1049,10 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
     // B's output row yy (lane-transposed through s_x, as the strip kernel stores)
     auto store_b = [&](int yy, const f32x16& acc) __attribute__((always_inline)) {
         __asm__ volatile("" ::: "memory");
-        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        bf16x4 b[4];
+        act16_bf16(acc, s_bb, kh, relu_b, neg_b, b);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const float4 bq = *reinterpret_cast<const float4*>(s_bb + 8 * i + 4 * kh);
-            const float bj[4] = {bq.x, bq.y, bq.z, bq.w};
-            bf16x4 b;
-#pragma unroll
-            for (int j = 0; j < 4; j++) b[j] = bf16(act_select(acc[i * 4 + j] + bj[j], relu_b, neg_b));
-            *reinterpret_cast<bf16x4*>(s_x + m * OXP + 8 * i + 4 * kh) = b;
-        }
+        for (int i = 0; i < 4; i++) *reinterpret_cast<bf16x4*>(s_x + m * OXP + 8 * i + 4 * kh) = b[i];
         __asm__ volatile("" ::: "memory");
         const size_t o = (size_t(n) * H + yy) * W;
 #pragma unroll
@@ -1082,6 +1099,17 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
             else if constexpr (R == 1) return pb;
             else return pc;
         }();
+        // the output layer's base pixels of this row's output, fetched now
+        // and used after the row's MFMAs
+        const int oy = y0 + jj - 4, px = xs + m;
+        const bool out_px = BKIND == 1 && jj >= 4 && oy < H && kh == 0 && m < 30 && px < W;
+        float rb[3] = {0.f, 0.f, 0.f};
+        if (out_px) {
+            const size_t qo = ((size_t(n) * H + oy) * W + px) * 3;
+            rb[0] = rbase[qo + 0];
+            rb[1] = rbase[qo + 1];
+            rb[2] = rbase[qo + 2];
+        }
         __asm__ volatile("" ::: "memory");
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
@@ -1125,16 +1153,13 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
             // BF16, zero outside the image, into s_m for B
             const int qy = y0 - 3 + jj;
             const bool ok = qy >= 0 && qy < H && mx >= 0 && mx < W;
-            typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
             __asm__ volatile("" ::: "memory");
+            bf16x4 b[4];
+            act16_bf16(aC, s_ba, kh, relu_a, neg_a, b);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const float4 bq = *reinterpret_cast<const float4*>(s_ba + 8 * i + 4 * kh);
-                const float bj[4] = {bq.x, bq.y, bq.z, bq.w};
-                bf16x4 b;
-#pragma unroll
-                for (int j = 0; j < 4; j++) b[j] = bf16(ok ? act_select(aC[i * 4 + j] + bj[j], relu_a, neg_a) : 0.f);
-                *reinterpret_cast<bf16x4*>(s_m + m * MP + 8 * i + 4 * kh) = b;
+                const bf16x4 v = ok ? b[i] : bf16x4{bf16(0.f), bf16(0.f), bf16(0.f), bf16(0.f)};
+                *reinterpret_cast<bf16x4*>(s_m + m * MP + 8 * i + 4 * kh) = v;
             }
             __asm__ volatile("" ::: "memory");
             const bool q_in = qy >= 0 && qy < H;
@@ -1168,8 +1193,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                         xc = xn2;
                     }
                 }
-                // B row o = jj - 4 (image row y0 + jj - 4) is complete
-                const int oy = y0 + jj - 4;
+                // B row o = jj - 4 (image row oy = y0 + jj - 4) is complete
                 if (jj >= 4 && oy < H) store_b(oy, bC);
 #pragma unroll
                 for (int i = 0; i < 16; i++) bC[i] = 0.f;
@@ -1198,16 +1222,14 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                     g1[c] = (acc[0][4 + c] + acc[1][4 + c]) + acc[2][4 + c];
                     g2[c] = (acc[0][8 + c] + acc[1][8 + c]) + acc[2][8 + c];
                 }
-                // output row of A row q - 1 ... : image row qy - 1 = y0 + jj - 4
-                const int oy = y0 + jj - 4;
-                const int px = xs + m;
-                if (jj >= 4 && oy < H && kh == 0 && m < 30 && px < W) {
+                // B's output row image qy - 1 = y0 + jj - 4 is complete
+                if (out_px) {
                     const size_t qo = ((size_t(n) * H + oy) * W + px) * 3;
                     const float v0 = q0b[0] + q1a[0] + g2[0], v1 = q0b[1] + q1a[1] + g2[1],
                                 v2 = q0b[2] + q1a[2] + g2[2];
-                    rgb[qo + 0] = fminf(fmaxf(rbase[qo + 0] + act_fwd(v0 + s_bb[0], act_b, 0.f), 0.f), 1.f);
-                    rgb[qo + 1] = fminf(fmaxf(rbase[qo + 1] + act_fwd(v1 + s_bb[1], act_b, 0.f), 0.f), 1.f);
-                    rgb[qo + 2] = fminf(fmaxf(rbase[qo + 2] + act_fwd(v2 + s_bb[2], act_b, 0.f), 0.f), 1.f);
+                    rgb[qo + 0] = fminf(fmaxf(rb[0] + act_fwd(v0 + s_bb[0], act_b, 0.f), 0.f), 1.f);
+                    rgb[qo + 1] = fminf(fmaxf(rb[1] + act_fwd(v1 + s_bb[1], act_b, 0.f), 0.f), 1.f);
+                    rgb[qo + 2] = fminf(fmaxf(rb[2] + act_fwd(v2 + s_bb[2], act_b, 0.f), 0.f), 1.f);
                 }
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
