@@ -114,9 +114,10 @@ def main() -> int:
     ap.add_argument("--atomic-width", type=int, default=64)
     ap.add_argument("--hist-bits", type=int, default=32)
     ap.add_argument("--walkers", type=int, default=1 << 20, help="walkers of the plain path")
-    ap.add_argument("--low-walkers", type=int, default=1 << 19,
+    ap.add_argument("--low-walkers", type=int, default=1 << 18,
                     help="walkers of the denoised path (its small budget pays less warmup with fewer)")
     ap.add_argument("--low-iters", type=int, default=1 << 25, help="budget of the denoised path")
+    ap.add_argument("--low-fuse", type=int, default=20, help="fuse iterations per walker of the denoised path")
     ap.add_argument("--low-de-radius", type=float, default=0.0,
                     help="density estimation radius of the denoised path (0: tonemap the histogram itself)")
     ap.add_argument("--weights", type=str, default=None)
@@ -256,12 +257,12 @@ def main() -> int:
     lanes = [Lane() for _ in range(max(1, args.pipeline))]
 
     def plain(lane: Lane, seed: int, kern=k, iters=ITERS, fmt=_lib.PIXEL_HALF4, walkers=args.walkers, de_r=DE_R,
-              smp=samples, tone=(gamma, bright, vib)):
+              smp=samples, tone=(gamma, bright, vib), fuse=20):
         st = lane.states.get((id(kern), walkers))
         if st is None:
-            st = lane.states[(id(kern), walkers)] = kern.warmup(smp, W, H, seed=seed, fuse=20, walkers=walkers)
+            st = lane.states[(id(kern), walkers)] = kern.warmup(smp, W, H, seed=seed, fuse=fuse, walkers=walkers)
         else:
-            st.rewarm(smp, seed=seed, fuse=20)
+            st.rewarm(smp, seed=seed, fuse=fuse)
         # density estimation is enqueued behind the bin call; the host only
         # waits for the call's counters (the tonemap needs its quality)
         st.bin_async(iters=iters)
@@ -282,7 +283,7 @@ def main() -> int:
 
     def denoised(lane: Lane, seed: int, kern=kd, smp=samples, tone=(gamma, bright, vib)):
         st, res, tm = plain(lane, seed, kern, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers,
-                            args.low_de_radius, smp, tone)
+                            args.low_de_radius, smp, tone, args.low_fuse)
         inf = st.info()
         check(lib.rfk_denoiser_features(inf.histogram, inf.hist_bits, tm.ptr, npx, lane.feat.ptr, lane.base.ptr,
                                         inf.stream))
@@ -418,7 +419,7 @@ def main() -> int:
                             "psnr_db": psnr_plain},
                   "denoised": None if fps_den is None else
                   {"fps": fps_den, "ms": ms_den, "sequential_ms": seq_den, "iterations": args.low_iters,
-                   "walkers": args.low_walkers, "de_radius": args.low_de_radius,
+                   "walkers": args.low_walkers, "fuse": args.low_fuse, "de_radius": args.low_de_radius,
                    "threads_per_block": args.den_tpb, "binning": args.den_binning, "bin_tile_log2": den_tile,
                    "hist_bits": 32,
                    "psnr_db": psnr_den, "fallback_frames": fallbacks,
@@ -491,7 +492,8 @@ def held_out_flames(args, db, lanes, plain, denoised, score, summarize) -> dict:
         _, _, img = denoised(lanes[0], s, kern, smp, tone)
         lib.rfk_synchronize()
         sc = score(img.download(np.float32, (H, W, 3)), ref)
-        _, _, lo = plain(lanes[0], s, kern, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers, 9.0, smp, tone)
+        _, _, lo = plain(lanes[0], s, kern, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers, 9.0, smp, tone,
+                         args.low_fuse)
         lib.rfk_synchronize()
         sc["plain_same_budget_psnr_db"] = score(lo.download(np.float32, (H, W, 4)), ref)["psnr_db"]
         sc["seed"] = s
