@@ -1592,7 +1592,7 @@ static int strip_rows(const void* fn, size_t lds, int nw, int zdim, int H, int W
     const long cols = (W + px * nw - 1) / (px * nw);
     int best = 16;
     long best_cost = -1;
-    for (int r = 4; r <= 64; r += even ? 2 : 1) {
+    for (int r = 2; r <= 64; r += even ? 2 : 1) {
         const long blocks = cols * ((H + r - 1) / r) * zdim;
         const long cost = (blocks + resident - 1) / resident * (r + 2);
         if (best_cost < 0 || cost < best_cost) {
