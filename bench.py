@@ -114,8 +114,8 @@ def main() -> int:
     ap.add_argument("--atomic-width", type=int, default=64)
     ap.add_argument("--hist-bits", type=int, default=32)
     ap.add_argument("--walkers", type=int, default=1 << 20, help="walkers of the plain path")
-    ap.add_argument("--low-walkers", type=int, default=1 << 18,
-                    help="walkers of the denoised path (its small budget pays less warmup with fewer)")
+    ap.add_argument("--low-walkers", type=int, default=1 << 19,
+                    help="walkers of the denoised path")
     ap.add_argument("--low-iters", type=int, default=1 << 25, help="budget of the denoised path")
     ap.add_argument("--low-fuse", type=int, default=20, help="fuse iterations per walker of the denoised path")
     ap.add_argument("--low-de-radius", type=float, default=0.0,
@@ -257,7 +257,7 @@ def main() -> int:
     lanes = [Lane() for _ in range(max(1, args.pipeline))]
 
     def plain(lane: Lane, seed: int, kern=k, iters=ITERS, fmt=_lib.PIXEL_HALF4, walkers=args.walkers, de_r=DE_R,
-              smp=samples, tone=(gamma, bright, vib), fuse=20):
+              smp=samples, tone=(gamma, bright, vib), fuse=20, features=False):
         st = lane.states.get((id(kern), walkers))
         if st is None:
             st = lane.states[(id(kern), walkers)] = kern.warmup(smp, W, H, seed=seed, fuse=fuse, walkers=walkers)
@@ -274,6 +274,10 @@ def main() -> int:
             out = lane.images[fmt] = render.DeviceBuffer(npx * _lib.PIXEL_BYTES[fmt])
         inf = st.info()
         ta = _lib.TonemapArgs(inf.quality, tone[0], tone[1], tone[2], 0.0)
+        if features:
+            # the denoiser's input straight from the bins (tonemap + features in one pass)
+            check(lib.rfk_tonemap_features(st.handle, C.byref(ta), lane.feat.ptr, lane.base.ptr))
+            return st, res, None
         # everything of a frame on the state's stream: the next re-warm waits for it
         if de_r > 0:
             check(lib.rfk_tonemap_image(lane.de.ptr, W, H, out.ptr, fmt, C.byref(ta), inf.stream))
@@ -282,11 +286,13 @@ def main() -> int:
         return st, res, out
 
     def denoised(lane: Lane, seed: int, kern=kd, smp=samples, tone=(gamma, bright, vib)):
+        direct = args.low_de_radius <= 0
         st, res, tm = plain(lane, seed, kern, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers,
-                            args.low_de_radius, smp, tone, args.low_fuse)
+                            args.low_de_radius, smp, tone, args.low_fuse, features=direct)
         inf = st.info()
-        check(lib.rfk_denoiser_features(inf.histogram, inf.hist_bits, tm.ptr, npx, lane.feat.ptr, lane.base.ptr,
-                                        inf.stream))
+        if not direct:
+            check(lib.rfk_denoiser_features(inf.histogram, inf.hist_bits, tm.ptr, npx, lane.feat.ptr, lane.base.ptr,
+                                            inf.stream))
         lane.den.forward(lane.feat, lane.base, 1, H, W, lane.rgb, stream=inf.stream)
         return st, res, lane.rgb
 

@@ -360,6 +360,10 @@ int rfk_denoiser_get_grads(const rfk_denoiser* d, float* gw /* host */, float* g
  * and its float4 tonemap */
 int rfk_denoiser_features(const uint32_t* hist, int hist_bits, const float* tonemapped_rgba, size_t npix, void* feat,
                           float* base, void* stream);
+/* rfk_tonemap into float4 followed by rfk_denoiser_features, in one pass
+ * on the state's stream: the same feat (8 BF16 channels per pixel) and base
+ * bit for bit, without the tonemapped image                               */
+int rfk_tonemap_features(rfk_state* s, const rfk_tonemap_args* a, void* feat, float* base);
 /* out_rgb = clamp(base + U-Net(feat), 0, 1) */
 int rfk_denoiser_forward(rfk_denoiser* d, const void* feat, const float* base, int N, int H, int W, float* out_rgb,
                          void* stream);

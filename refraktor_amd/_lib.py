@@ -146,6 +146,7 @@ _SIGS = [
     ("rfk_denoiser_get_params", C.c_int, [_P, _P, _P]),
     ("rfk_denoiser_set_params", C.c_int, [_P, _P, _P]),
     ("rfk_denoiser_get_grads", C.c_int, [_P, _P, _P]),
+    ("rfk_tonemap_features", C.c_int, [_P, C.POINTER(TonemapArgs), _P, _P]),
     ("rfk_denoiser_features", C.c_int, [_P, C.c_int, _P, C.c_size_t, _P, _P, _P]),
     ("rfk_denoiser_forward", C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
     ("rfk_denoiser_train_step", C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_float,

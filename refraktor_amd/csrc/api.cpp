@@ -1309,6 +1309,16 @@ int rfk_tonemap(rfk_state* s, void* out, int pixel_format, const rfk_tonemap_arg
                           s->stream);
 }
 
+int rfk_tonemap_features(rfk_state* s, const rfk_tonemap_args* a, void* feat, float* base) {
+    if (!s || !a || !feat || !base) return fail(RFK_ERR_INVALID, "null argument");
+    const double q = a->quality > 0 ? a->quality : s->quality;
+    if (!(q > 0)) return fail(RFK_ERR_INVALID, "quality must be positive (render first)");
+    const float k = std::pow(10.0f, -std::log10(float(q)) - 0.5f);  // as tonemap_common
+    RFK_HIP(rfk::sk::tonemap_features(s->hist, s->hist64, size_t(s->width) * s->height, k, float(a->brightness),
+                                      float(a->gamma), float(a->vibrancy), feat, base, s->stream));
+    return RFK_OK;
+}
+
 int rfk_tonemap_image(const float* in_rgba, uint32_t width, uint32_t height, void* out, int pixel_format,
                       const rfk_tonemap_args* a, void* stream) {
     return tonemap_common(in_rgba, 0, size_t(width) * height, out, pixel_format, a, 0.0,

@@ -316,16 +316,10 @@ struct half4_t {
 __device__ __forceinline__ float sat(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return a + t * (b - a); }
 
-// In: 0 float4 draw units, 1 u32 bins, 2 u64 bins
-template <int In, int Fmt>
-__global__ void tonemap_kernel(const void* __restrict__ in, size_t npix, void* __restrict__ out, float scale_constant,
-                               float brightness, float gamma, float vibrancy) {
-    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g >= npix) return;
-    float4 col;
-    if (In == 0) col = reinterpret_cast<const float4*>(in)[g];
-    else if (In == 1) col = bin_units(reinterpret_cast<const uint4*>(in)[g]);
-    else col = bin_units(reinterpret_cast<const bin64*>(in)[g]);
+// One pixel of the log-density tonemap (tonemap.cu:50-105): draw units in,
+// RGBA out (zero for an empty bin)
+__device__ __forceinline__ float4 tonemap_pixel(float4 col, float scale_constant, float brightness, float gamma,
+                                                float vibrancy) {
     float4 o = {0.f, 0.f, 0.f, 0.f};
     if (col.w != 0.0f) {
         const float input_density = col.w;
@@ -345,6 +339,20 @@ __global__ void tonemap_kernel(const void* __restrict__ in, size_t npix, void* _
         o.z = sat(lerpf(powf(col.z, ig), col.z * gamma_factor, vibrancy));
         o.w = sat(logf(input_density) * (4.0f / logf(2.0f)) / 96.0f);
     }
+    return o;
+}
+
+// In: 0 float4 draw units, 1 u32 bins, 2 u64 bins
+template <int In, int Fmt>
+__global__ void tonemap_kernel(const void* __restrict__ in, size_t npix, void* __restrict__ out, float scale_constant,
+                               float brightness, float gamma, float vibrancy) {
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= npix) return;
+    float4 col;
+    if (In == 0) col = reinterpret_cast<const float4*>(in)[g];
+    else if (In == 1) col = bin_units(reinterpret_cast<const uint4*>(in)[g]);
+    else col = bin_units(reinterpret_cast<const bin64*>(in)[g]);
+    const float4 o = tonemap_pixel(col, scale_constant, brightness, gamma, vibrancy);
     if (Fmt == 0) {
         reinterpret_cast<float4*>(out)[g] = o;
     } else if (Fmt == 1) {
@@ -360,6 +368,36 @@ __global__ void tonemap_kernel(const void* __restrict__ in, size_t npix, void* _
         auto q = [](float v) { return uint32_t(fminf(fmaxf(v * 255.0f + 0.5f, 0.0f), 255.0f)); };
         reinterpret_cast<uint32_t*>(out)[g] = q(o.x) | (q(o.y) << 8) | (q(o.z) << 16) | (q(o.w) << 24);
     }
+}
+
+// The denoiser's input straight from the histogram, without the tonemapped
+// image in between: the tonemap of tonemap_kernel and the features of the
+// denoiser's features kernel from it ([R, G, B, alpha] of the tonemap,
+// log2(1 + density) / 16, coverage, two zero channels; BF16) and the FP32
+// base image (the tonemap's RGB)
+template <class BinT>
+__global__ void tonemap_features_kernel(const BinT* __restrict__ hist, size_t npix, float scale_constant,
+                                        float brightness, float gamma, float vibrancy, __bf16* __restrict__ feat,
+                                        float* __restrict__ base) {
+    typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+    const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= npix) return;
+    const float4 col = bin_units(hist[g]);
+    const float dens = col.w;
+    const float4 o = tonemap_pixel(col, scale_constant, brightness, gamma, vibrancy);
+    bf16x8 f;
+    f[0] = __bf16(o.x);
+    f[1] = __bf16(o.y);
+    f[2] = __bf16(o.z);
+    f[3] = __bf16(o.w);
+    f[4] = __bf16(log2f(1.0f + dens) * (1.0f / 16.0f));
+    f[5] = __bf16(dens > 0.0f ? 1.0f : 0.0f);
+    f[6] = __bf16(0.0f);
+    f[7] = __bf16(0.0f);
+    *reinterpret_cast<bf16x8*>(feat + 8 * g) = f;
+    base[3 * g + 0] = o.x;
+    base[3 * g + 1] = o.y;
+    base[3 * g + 2] = o.z;
 }
 
 // -------------------------------------------------------------- density HDR
@@ -536,6 +574,20 @@ hipError_t density_estimate(const uint32_t* hist, bool hist64, uint32_t width, u
                                   inv_z, nlevels, radius, level_scratch, out_rgba, s);
     return density_estimate_t(reinterpret_cast<const uint4*>(hist), width, height, max_radius, min_radius, curve,
                               inv_z, nlevels, radius, level_scratch, out_rgba, s);
+}
+
+hipError_t tonemap_features(const void* hist, bool hist64, size_t npix, float scale_constant, float brightness,
+                            float gamma, float vibrancy, void* feat, float* base, hipStream_t s) {
+    if (!npix) return hipSuccess;
+    if (hist64)
+        tonemap_features_kernel<<<nblocks(npix), kBlock, 0, s>>>(reinterpret_cast<const bin64*>(hist), npix,
+                                                                 scale_constant, brightness, gamma, vibrancy,
+                                                                 reinterpret_cast<__bf16*>(feat), base);
+    else
+        tonemap_features_kernel<<<nblocks(npix), kBlock, 0, s>>>(reinterpret_cast<const uint4*>(hist), npix,
+                                                                 scale_constant, brightness, gamma, vibrancy,
+                                                                 reinterpret_cast<__bf16*>(feat), base);
+    return hipGetLastError();
 }
 
 hipError_t tonemap(const void* in, int in_kind, size_t npix, void* out, int format, float scale_constant,

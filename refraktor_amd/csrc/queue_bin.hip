@@ -8,7 +8,7 @@
 //                 (in place) and the tile's total
 //   2. qseg       exclusive scan of the tile totals -> tile starts, and the
 //                 (tile, segment of <= 2^18 records) work items of step 4
-//   3. qscatter   counting sort by tile: each block's records, 8192 at a time,
+//   3. qscatter   counting sort by tile: each block's records, 4096 at a time,
 //                 ranked in LDS and written as tile runs at their offsets,
 //                 colour (u32) and pixel in tile (u16) in separate arrays
 //   4. qbin       one workgroup per work item sums its records in a 64x64
@@ -39,7 +39,10 @@ constexpr uint32_t kTileLog2 = 6;  // 64x64-pixel tiles
 constexpr uint32_t kTilePix = 1u << (2 * kTileLog2);
 constexpr uint32_t kNoTile = 0xFFFFFu;
 constexpr uint32_t kThreads = 1024;
-constexpr uint32_t kPiece = 8192;        // records sorted in LDS at a time
+// the counting sort: 512 threads rank 4096 records at a time, so that
+// several sorting workgroups share a CU and one's barriers overlap another's
+// memory traffic (2.5 % off a 2^30-iteration bin call against 1024 / 8192)
+constexpr uint32_t kScatterThreads = 512, kScatterPiece = 4096;
 constexpr uint32_t kSegment = 1u << 18;  // records per bin work item (u32 LDS sums cannot wrap)
 
 __device__ __forceinline__ uint32_t rec_tile(u64 r) { return uint32_t(r >> 44); }
@@ -67,7 +70,8 @@ __device__ __forceinline__ uint32_t wave_count(uint32_t* counters, uint32_t key,
     return rank;
 }
 
-// Block-wide exclusive scan of one value per thread (blockDim.x = 1024).
+// Block-wide exclusive scan of one value per thread (blockDim.x = T).
+template <uint32_t T = kThreads>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t* total) {
     uint32_t incl = v;
     for (uint32_t d = 1; d < 64; d <<= 1) {
@@ -77,7 +81,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     if (__lane_id() == 63) s_wave[threadIdx.x >> 6] = incl;
     __syncthreads();
     uint32_t base = 0, all = 0;
-    for (uint32_t w = 0; w < kThreads / 64; w++) {
+    for (uint32_t w = 0; w < T / 64; w++) {
         const uint32_t s = s_wave[w];
         if (w < (threadIdx.x >> 6)) base += s;
         all += s;
@@ -145,8 +149,10 @@ __global__ void __launch_bounds__(kThreads)
 // grid = nparts: partition p = records [p * per_part, (p + 1) * per_part),
 // column p % ncols of the counts of group p / ncols, whose ntiles tiles (the
 // records' tile fields) are chunk tiles from (p / ncols) * ntiles on.
-// LDS: cursor[ntiles], lcount[ntiles], lstart[ntiles], piece[kPiece].
-__global__ void __launch_bounds__(kThreads)
+// LDS: cursor[ntiles], lcount[ntiles], lstart[ntiles], piece[PIECE]. T
+// threads sort PIECE records at a time.
+template <uint32_t T, uint32_t PIECE>
+__global__ void __launch_bounds__(T)
     qscatter_kernel(const u64* __restrict__ rec, size_t per_part, uint32_t ntiles, uint32_t ncols,
                     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ tile_start,
                     uint32_t* __restrict__ sorted_col, uint16_t* __restrict__ sorted_loc) {
@@ -155,14 +161,14 @@ __global__ void __launch_bounds__(kThreads)
     uint32_t* lcount = cursor + ntiles;
     uint32_t* lstart = lcount + ntiles;
     u64* piece = reinterpret_cast<u64*>(lstart + ntiles + (ntiles & 1u));
-    __shared__ uint32_t s_wave[kThreads / 64];
+    __shared__ uint32_t s_wave[T / 64];
     const uint32_t group = blockIdx.x / ncols, col = blockIdx.x - group * ncols;
     const uint32_t tile0 = group * ntiles;
-    for (uint32_t t = threadIdx.x; t < ntiles; t += kThreads)
+    for (uint32_t t = threadIdx.x; t < ntiles; t += T)
         cursor[t] = tile_start[tile0 + t] + offsets[size_t(tile0 + t) * ncols + col];
     const size_t b0 = size_t(blockIdx.x) * per_part, b1 = b0 + per_part;
-    constexpr uint32_t kPer = kPiece / kThreads;
-    const uint32_t per = (ntiles + kThreads - 1) / kThreads;
+    constexpr uint32_t kPer = PIECE / T;
+    const uint32_t per = (ntiles + T - 1) / T;
     const uint32_t lo = threadIdx.x * per, hi = std::min(ntiles, lo + per);
     // all loads of a piece are issued before any of them is used, and the
     // next piece's loads are in flight while this one is ranked and written
@@ -170,17 +176,17 @@ __global__ void __launch_bounds__(kThreads)
     auto load_piece = [&](size_t base) {
 #pragma unroll
         for (uint32_t j = 0; j < kPer; j++) {
-            const size_t i = base + size_t(j) * kThreads + threadIdx.x;
+            const size_t i = base + size_t(j) * T + threadIdx.x;
             nxt[j] = i < b1 ? rec[i] : ~0ull;
         }
     };
     load_piece(b0);
-    for (size_t base = b0; base < b1; base += kPiece) {
-        for (uint32_t t = threadIdx.x; t < ntiles; t += kThreads) lcount[t] = 0;
+    for (size_t base = b0; base < b1; base += PIECE) {
+        for (uint32_t t = threadIdx.x; t < ntiles; t += T) lcount[t] = 0;
         u64 r[kPer];
 #pragma unroll
         for (uint32_t j = 0; j < kPer; j++) r[j] = nxt[j];
-        if (base + kPiece < b1) load_piece(base + kPiece);
+        if (base + PIECE < b1) load_piece(base + PIECE);
         __syncthreads();
         uint32_t rank[kPer];
 #pragma unroll
@@ -192,7 +198,7 @@ __global__ void __launch_bounds__(kThreads)
         uint32_t sum = 0;
         for (uint32_t t = lo; t < hi; t++) sum += lcount[t];
         uint32_t used = 0;
-        uint32_t run = block_exclusive_scan(sum, s_wave, &used);
+        uint32_t run = block_exclusive_scan<T>(sum, s_wave, &used);
         for (uint32_t t = lo; t < hi; t++) {
             lstart[t] = run;
             run += lcount[t];
@@ -205,7 +211,7 @@ __global__ void __launch_bounds__(kThreads)
         }
         __syncthreads();
         // sorted order: consecutive threads write consecutive addresses of a run
-        for (uint32_t k = threadIdx.x; k < used; k += kThreads) {
+        for (uint32_t k = threadIdx.x; k < used; k += T) {
             const u64 v = piece[k];
             const uint32_t tile = rec_tile(v);
             const size_t pos = size_t(cursor[tile]) + (k - lstart[tile]);
@@ -213,7 +219,7 @@ __global__ void __launch_bounds__(kThreads)
             sorted_loc[pos] = uint16_t((v >> 32) & (kTilePix - 1u));
         }
         __syncthreads();
-        for (uint32_t t = threadIdx.x; t < ntiles; t += kThreads) cursor[t] += lcount[t];
+        for (uint32_t t = threadIdx.x; t < ntiles; t += T) cursor[t] += lcount[t];
     }
 }
 
@@ -320,9 +326,9 @@ hipError_t queue_bin_groups(const unsigned long long* records, size_t per_part, 
     qscan_col_kernel<<<uint32_t(ntiles), kThreads, 0, s>>>(counts, ncols, tile_total);
     qseg_kernel<<<1, kThreads, 0, s>>>(tile_total, uint32_t(ntiles), tile_start, reinterpret_cast<uint4*>(segments),
                                        max_segments, nseg);
-    const size_t lds = size_t(3 * tpf + 1) * sizeof(uint32_t) + kPiece * sizeof(u64) + 8;
+    const size_t lds = size_t(3 * tpf + 1) * sizeof(uint32_t) + kScatterPiece * sizeof(u64) + 8;
     // more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per workgroup)
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&qscatter_kernel),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&qscatter_kernel<kScatterThreads, kScatterPiece>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
     // sorted records, colour and pixel-in-tile apart (the tile is the work
@@ -330,8 +336,8 @@ hipError_t queue_bin_groups(const unsigned long long* records, size_t per_part, 
     const size_t n = per_part * nparts;
     uint32_t* sorted_col = reinterpret_cast<uint32_t*>(sorted);
     uint16_t* sorted_loc = reinterpret_cast<uint16_t*>(sorted_col + n);
-    qscatter_kernel<<<nparts, kThreads, lds, s>>>(records, per_part, tpf, ncols, counts, tile_start, sorted_col,
-                                                  sorted_loc);
+    qscatter_kernel<kScatterThreads, kScatterPiece><<<nparts, kScatterThreads, lds, s>>>(
+        records, per_part, tpf, ncols, counts, tile_start, sorted_col, sorted_loc);
     if (hist64)
         qbin_kernel<<<max_segments, kThreads, 0, s>>>(sorted_col, sorted_loc, reinterpret_cast<const uint4*>(segments),
                                                       nseg, width, height, tiles_x, tpf, group_frame, frame_elems,

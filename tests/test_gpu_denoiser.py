@@ -215,6 +215,33 @@ def test_conv_pair_output_layer(gpu, N, H, W, cout):
                                            None, None, None, out.ptr, None) == _lib.RFK_ERR_INVALID
 
 
+@pytest.mark.parametrize("hist_bits", [32, 64])
+def test_tonemap_features_one_pass(db, gpu, hist_bits):
+    """rfk_tonemap_features equals rfk_tonemap (float4) then
+    rfk_denoiser_features bit for bit, feat and base, from 32- and 64-bit bins"""
+    W, H = 320, 180
+    f = fl.flame_small(2)
+    k = db.compile(f, hist_bits=hist_bits)
+    st = k.warmup(k.pack_samples(W, H), W, H, seed=4, fuse=10, walkers=8192)
+    st.bin(iters=8192 * 256)
+    inf = st.info()
+    lib = _lib.lib()
+    args = _lib.TonemapArgs(0.0, f["gamma"], f["brightness"], f["vibrancy"], 0.0)
+    npx = W * H
+    feat, base = render.DeviceBuffer(npx * 16), render.DeviceBuffer(npx * 12)
+    _lib.check(lib.rfk_tonemap_features(st.handle, C.byref(args), feat.ptr, base.ptr))
+    tm = render.DeviceBuffer(npx * 16)
+    _lib.check(lib.rfk_tonemap(st.handle, tm.ptr, _lib.PIXEL_FLOAT4, C.byref(args)))
+    feat2, base2 = render.DeviceBuffer(npx * 16), render.DeviceBuffer(npx * 12)
+    _lib.check(lib.rfk_denoiser_features(inf.histogram, inf.hist_bits, tm.ptr, npx, feat2.ptr, base2.ptr, inf.stream))
+    _lib.check(lib.rfk_synchronize())
+    a, b = feat.download(np.uint16, (npx, 8)), feat2.download(np.uint16, (npx, 8))
+    assert np.array_equal(a, b)
+    assert np.array_equal(base.download(np.uint32, (npx, 3)), base2.download(np.uint32, (npx, 3)))
+    assert np.count_nonzero(a[:, 5]) > 500  # covered pixels
+    assert lib.rfk_tonemap_features(st.handle, None, feat.ptr, base.ptr) == _lib.RFK_ERR_INVALID
+
+
 @pytest.mark.parametrize("N,H,W", [(1, 24, 40), (2, 16, 72), (1, 40, 536)])
 def test_unet_inference_matches_oracle(gpu, N, H, W):
     """the fused inference forward of the whole network vs the float64 network;
