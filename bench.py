@@ -285,7 +285,10 @@ def main() -> int:
             check(lib.rfk_tonemap(st.handle, out.ptr, fmt, C.byref(ta)))  # the histogram itself
         return st, res, out
 
-    def denoised(lane: Lane, seed: int, kern=kd, smp=samples, tone=(gamma, bright, vib)):
+    den_kernel = {"k": kd}
+
+    def denoised(lane: Lane, seed: int, kern=None, smp=samples, tone=(gamma, bright, vib)):
+        kern = kern or den_kernel["k"]
         direct = args.low_de_radius <= 0
         st, res, tm = plain(lane, seed, kern, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers,
                             args.low_de_radius, smp, tone, args.low_fuse, features=direct)
@@ -320,6 +323,22 @@ def main() -> int:
         lib.rfk_synchronize()
         print(json.dumps({"roofline_denoiser": unet_roofline(lanes[0], H, W, reps=args.steps)}), flush=True)
         return 0
+
+    if use_den and tuned is not None and "denoised" in tuned:
+        # the denoised path's workgroup size by frames in flight: a smaller
+        # iterate workgroup fits beside another frame's convolutions on a CU,
+        # which the bin call's own rate cannot show
+        frame_ms = {}
+        for tpb in sorted({c["threads_per_block"] for c in tuned["denoised"]["candidates"] if c["binning"] == "queue"}):
+            kk = db.compile(f, threads_per_block=tpb, atomic_width=64, hist_bits=32, bin_queue=True)
+            frame_ms[tpb] = timed(lambda lane, seed, kk=kk: denoised(lane, seed, kk), 12, 3)[0]
+            for lane in lanes:  # free the candidate's states (their record buffers)
+                lane.states = {key: st for key, st in lane.states.items() if key[0] != id(kk)}
+        best = min(frame_ms, key=frame_ms.get)
+        args.den_binning, args.den_tpb, den_tile = "queue", best, 0
+        kd = den_kernel["k"] = db.compile(f, threads_per_block=best, atomic_width=64, hist_bits=32, bin_queue=True)
+        tuned["denoised"]["frames_in_flight_ms"] = {str(k): v for k, v in frame_ms.items()}
+        tuned["denoised"]["shape"].update({"binning": "queue", "threads_per_block": best, "bin_tile_log2": 0})
 
     ms_plain, kms, dr, ps, img_plain = timed(plain, args.steps, args.warmup)
     ms_den = img_den = None
