@@ -314,6 +314,13 @@ struct half4_t {
 };
 
 __device__ __forceinline__ float sat(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+// x^y for x >= 0 from the hardware's base-2 exponential and logarithm
+// (v_exp_f32, v_log_f32: about 1 ulp each): the reference builds its
+// tonemapper with fast math (tonemapper.cpp:20), where powf is this; the
+// library powf's exact rounding made the tonemap VALU-bound
+__device__ __forceinline__ float fast_pow(float x, float y) {
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return a + t * (b - a); }
 
 // One pixel of the log-density tonemap (tonemap.cu:50-105): draw units in,
@@ -330,13 +337,12 @@ __device__ __forceinline__ float4 tonemap_pixel(float4 col, float scale_constant
         col.w *= factor;
         // tonemap.cu:88-94 takes z = w^(1/gamma) and the colour products in
         // double; FP32 here (a few ulp of the FP32 result, far inside the
-        // tonemap's tolerance against the double oracle) keeps the kernel
-        // off the FP64 pipe
+        // tonemap's tolerance against the double oracle)
         const float ig = float(1.0 / double(gamma));
-        const float gamma_factor = powf(col.w, ig) / col.w;
-        o.x = sat(lerpf(powf(col.x, ig), col.x * gamma_factor, vibrancy));
-        o.y = sat(lerpf(powf(col.y, ig), col.y * gamma_factor, vibrancy));
-        o.z = sat(lerpf(powf(col.z, ig), col.z * gamma_factor, vibrancy));
+        const float gamma_factor = fast_pow(col.w, ig) / col.w;
+        o.x = sat(lerpf(fast_pow(col.x, ig), col.x * gamma_factor, vibrancy));
+        o.y = sat(lerpf(fast_pow(col.y, ig), col.y * gamma_factor, vibrancy));
+        o.z = sat(lerpf(fast_pow(col.z, ig), col.z * gamma_factor, vibrancy));
         o.w = sat(logf(input_density) * (4.0f / logf(2.0f)) / 96.0f);
     }
     return o;
