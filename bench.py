@@ -490,12 +490,12 @@ def summarize(frames: list) -> dict:
             "rel_l1_max": max(r), "rel_l1_mean": sum(r) / len(r), "below_40db": sum(1 for v in p if v < 40.0)}
 
 
-HELD_OUT_SEEDS = range(10001, 10009)
+HELD_OUT_SEEDS = range(10001, 10017)  # the denoiser's test flames
 
 
 def held_out_flames(args, db, lanes, plain, denoised, score, summarize) -> dict:
     """the denoised path on flames the network never saw (1080p flames of
-    seeds 10001-10008, the bench's launch shape), each judged against its own
+    seeds 10001-10016, the bench's launch shape), each judged against its own
     2^32-iteration plain render"""
     import numpy as np
     from refraktor_amd import _lib

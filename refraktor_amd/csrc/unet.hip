@@ -97,8 +97,8 @@ __device__ __forceinline__ void act16_bf16(const f32x16& acc, const float* bias,
 // through a 2x upsample (x1 stored at H/2 x W/2, nearest) or a 2x2 average
 // pool (x1 stored at 2H x 2W), so the decoder and encoder skip the
 // intermediate tensors. It serves the layers the strip kernel below does not
-// take (more than 96 output channels, or inputs too wide for the strip
-// kernel's weights in LDS). Shapes: (R, C, KCH) = (1,1,32) and (1,1,16) for
+// take (inputs too wide for the strip kernel's weights in LDS, x1 ending
+// inside a 32-channel chunk, tensors of 2^30 bytes or more). Shapes: (R, C, KCH) = (1,1,32) and (1,1,16) for
 // Cout <= 32, (1,2,32) for <= 64, (1,3,16) for <= 96, (1,4,16) above.
 __device__ __forceinline__ bf16x8 pool8(const bf16* p00, const bf16* p01, const bf16* p10, const bf16* p11) {
     const bf16x8 a = load8(p00), b = load8(p01), c = load8(p10), d = load8(p11);
