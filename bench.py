@@ -253,11 +253,19 @@ def main() -> int:
             self.feat = render.DeviceBuffer(npx * denoise.FEAT_C * 2)
             self.base = render.DeviceBuffer(npx * 12)
             self.rgb = render.DeviceBuffer(npx * 12)
+            self.pending = []  # states whose bin call nobody has waited for yet
 
     lanes = [Lane() for _ in range(max(1, args.pipeline))]
 
+    def settle(lane: Lane) -> list:
+        """wait for the lane's outstanding bin calls (their results)"""
+        out = [st.bin_wait() for st in lane.pending]
+        lane.pending.clear()
+        return out
+
     def plain(lane: Lane, seed: int, kern=k, iters=ITERS, fmt=_lib.PIXEL_HALF4, walkers=args.walkers, de_r=DE_R,
-              smp=samples, tone=(gamma, bright, vib), fuse=20, features=False):
+              smp=samples, tone=(gamma, bright, vib), fuse=20):
+        settle(lane)
         st = lane.states.get((id(kern), walkers))
         if st is None:
             st = lane.states[(id(kern), walkers)] = kern.warmup(smp, W, H, seed=seed, fuse=fuse, walkers=walkers)
@@ -274,10 +282,6 @@ def main() -> int:
             out = lane.images[fmt] = render.DeviceBuffer(npx * _lib.PIXEL_BYTES[fmt])
         inf = st.info()
         ta = _lib.TonemapArgs(inf.quality, tone[0], tone[1], tone[2], 0.0)
-        if features:
-            # the denoiser's input straight from the bins (tonemap + features in one pass)
-            check(lib.rfk_tonemap_features(st.handle, C.byref(ta), lane.feat.ptr, lane.base.ptr))
-            return st, res, None
         # everything of a frame on the state's stream: the next re-warm waits for it
         if de_r > 0:
             check(lib.rfk_tonemap_image(lane.de.ptr, W, H, out.ptr, fmt, C.byref(ta), inf.stream))
@@ -289,15 +293,32 @@ def main() -> int:
 
     def denoised(lane: Lane, seed: int, kern=None, smp=samples, tone=(gamma, bright, vib)):
         kern = kern or den_kernel["k"]
-        direct = args.low_de_radius <= 0
-        st, res, tm = plain(lane, seed, kern, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers,
-                            args.low_de_radius, smp, tone, args.low_fuse, features=direct)
-        inf = st.info()
-        if not direct:
+        if args.low_de_radius > 0:
+            st, res, tm = plain(lane, seed, kern, args.low_iters, _lib.PIXEL_FLOAT4, args.low_walkers,
+                                args.low_de_radius, smp, tone, args.low_fuse)
+            inf = st.info()
             check(lib.rfk_denoiser_features(inf.histogram, inf.hist_bits, tm.ptr, npx, lane.feat.ptr, lane.base.ptr,
                                             inf.stream))
-        lane.den.forward(lane.feat, lane.base, 1, H, W, lane.rgb, stream=inf.stream)
-        return st, res, lane.rgb
+            lane.den.forward(lane.feat, lane.base, 1, H, W, lane.rgb, stream=inf.stream)
+            return st, res, lane.rgb
+        # the whole frame is enqueued without waiting for its bin call: the
+        # tonemap takes the call's quality from its counters on the device,
+        # and the host collects the call's result when the lane comes round
+        # again, so the next frames' binning is queued behind this one's
+        # network on their own streams
+        settle(lane)
+        key = (id(kern), args.low_walkers)
+        st = lane.states.get(key)
+        if st is None:
+            st = lane.states[key] = kern.warmup(smp, W, H, seed=seed, fuse=args.low_fuse, walkers=args.low_walkers)
+        else:
+            st.rewarm(smp, seed=seed, fuse=args.low_fuse)
+        st.bin_async(iters=args.low_iters)
+        ta = _lib.TonemapArgs(0.0, tone[0], tone[1], tone[2], 0.0)
+        check(lib.rfk_tonemap_features(st.handle, C.byref(ta), lane.feat.ptr, lane.base.ptr))
+        lane.den.forward(lane.feat, lane.base, 1, H, W, lane.rgb, stream=st.info().stream)
+        lane.pending.append(st)
+        return st, None, lane.rgb
 
     def timed(fn, steps: int, warm: int, nl: int = len(lanes)):
         """`steps` frames round-robin over the first `nl` lanes"""
@@ -308,12 +329,18 @@ def main() -> int:
         kms, draws, passes, last = [], [], [], None
         for i in range(steps):
             st, res, img = fn(lanes[i % nl], args.seed + i)
-            kms.append(res["kernel_ms"])
-            draws.append(res["total_draws"])
-            passes.append(res["total_passes"])
+            if res is not None:
+                kms.append(res["kernel_ms"])
+                draws.append(res["total_draws"])
+                passes.append(res["total_passes"])
             last = img
         lib.rfk_synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / steps
+        for lane in lanes:
+            for res in settle(lane):
+                kms.append(res["kernel_ms"])
+                draws.append(res["total_draws"])
+                passes.append(res["total_passes"])
         return ms, float(np.mean(kms)), float(np.mean(draws)), float(np.mean(passes)), last
 
     if args.unet_only:

@@ -362,7 +362,9 @@ int rfk_denoiser_features(const uint32_t* hist, int hist_bits, const float* tone
                           float* base, void* stream);
 /* rfk_tonemap into float4 followed by rfk_denoiser_features, in one pass
  * on the state's stream: the same feat (8 BF16 channels per pixel) and base
- * bit for bit, without the tonemapped image                               */
+ * bit for bit, without the tonemapped image. a->quality <= 0: the quality of
+ * the last bin call enqueued on the state, read from its counters on the
+ * device, so the call may follow rfk_bin_async without rfk_bin_wait      */
 int rfk_tonemap_features(rfk_state* s, const rfk_tonemap_args* a, void* feat, float* base);
 /* out_rgb = clamp(base + U-Net(feat), 0, 1) */
 int rfk_denoiser_forward(rfk_denoiser* d, const void* feat, const float* base, int N, int H, int W, float* out_rgb,

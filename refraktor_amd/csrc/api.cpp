@@ -1311,11 +1311,18 @@ int rfk_tonemap(rfk_state* s, void* out, int pixel_format, const rfk_tonemap_arg
 
 int rfk_tonemap_features(rfk_state* s, const rfk_tonemap_args* a, void* feat, float* base) {
     if (!s || !a || !feat || !base) return fail(RFK_ERR_INVALID, "null argument");
-    const double q = a->quality > 0 ? a->quality : s->quality;
-    if (!(q > 0)) return fail(RFK_ERR_INVALID, "quality must be positive (render first)");
-    const float k = std::pow(10.0f, -std::log10(float(q)) - 0.5f);  // as tonemap_common
-    RFK_HIP(rfk::sk::tonemap_features(s->hist, s->hist64, size_t(s->width) * s->height, k, float(a->brightness),
-                                      float(a->gamma), float(a->vibrancy), feat, base, s->stream));
+    const size_t npix = size_t(s->width) * s->height;
+    if (a->quality > 0) {
+        const float k = std::pow(10.0f, -std::log10(float(a->quality)) - 0.5f);  // as tonemap_common
+        RFK_HIP(rfk::sk::tonemap_features(s->hist, s->hist64, npix, k, nullptr, float(a->brightness),
+                                          float(a->gamma), float(a->vibrancy), feat, base, s->stream));
+    } else {
+        // the quality of the last bin call enqueued on the state, read on
+        // the device: no wait for the call
+        RFK_HIP(rfk::sk::tonemap_features(s->hist, s->hist64, npix, 0.0f, s->counters + RFK_CNT_HITS,
+                                          float(a->brightness), float(a->gamma), float(a->vibrancy), feat, base,
+                                          s->stream));
+    }
     return RFK_OK;
 }
 

@@ -381,13 +381,20 @@ __global__ void tonemap_kernel(const void* __restrict__ in, size_t npix, void* _
 // denoiser's features kernel from it ([R, G, B, alpha] of the tonemap,
 // log2(1 + density) / 16, coverage, two zero channels; BF16) and the FP32
 // base image (the tonemap's RGB)
+// With `hits` (the bin call's hit counter in device memory) the scale
+// constant is computed here from the call's quality, hits / (pixels x 255),
+// as the host computes it, so the caller need not wait for the call.
 template <class BinT>
 __global__ void tonemap_features_kernel(const BinT* __restrict__ hist, size_t npix, float scale_constant,
-                                        float brightness, float gamma, float vibrancy, __bf16* __restrict__ feat,
-                                        float* __restrict__ base) {
+                                        const unsigned long long* __restrict__ hits, float brightness, float gamma,
+                                        float vibrancy, __bf16* __restrict__ feat, float* __restrict__ base) {
     typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
     const size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= npix) return;
+    if (hits) {
+        const double q = double(*hits) / (double(npix) * 255.0);
+        scale_constant = powf(10.0f, -log10f(float(q)) - 0.5f);
+    }
     const float4 col = bin_units(hist[g]);
     const float dens = col.w;
     const float4 o = tonemap_pixel(col, scale_constant, brightness, gamma, vibrancy);
@@ -582,16 +589,17 @@ hipError_t density_estimate(const uint32_t* hist, bool hist64, uint32_t width, u
                               inv_z, nlevels, radius, level_scratch, out_rgba, s);
 }
 
-hipError_t tonemap_features(const void* hist, bool hist64, size_t npix, float scale_constant, float brightness,
-                            float gamma, float vibrancy, void* feat, float* base, hipStream_t s) {
+hipError_t tonemap_features(const void* hist, bool hist64, size_t npix, float scale_constant,
+                            const unsigned long long* hits, float brightness, float gamma, float vibrancy, void* feat,
+                            float* base, hipStream_t s) {
     if (!npix) return hipSuccess;
     if (hist64)
         tonemap_features_kernel<<<nblocks(npix), kBlock, 0, s>>>(reinterpret_cast<const bin64*>(hist), npix,
-                                                                 scale_constant, brightness, gamma, vibrancy,
+                                                                 scale_constant, hits, brightness, gamma, vibrancy,
                                                                  reinterpret_cast<__bf16*>(feat), base);
     else
         tonemap_features_kernel<<<nblocks(npix), kBlock, 0, s>>>(reinterpret_cast<const uint4*>(hist), npix,
-                                                                 scale_constant, brightness, gamma, vibrancy,
+                                                                 scale_constant, hits, brightness, gamma, vibrancy,
                                                                  reinterpret_cast<__bf16*>(feat), base);
     return hipGetLastError();
 }

@@ -39,9 +39,11 @@ hipError_t density_estimate(const uint32_t* hist, bool hist64, uint32_t width, u
 // Tonemap (tonemap.cu:50-105). in_kind: 0 float4 draw units, 1 u32 RGBA
 // fixed point, 2 u64 RGBA fixed point.
 // the denoiser's 8 input channels (BF16) and FP32 base image from the bins
-// (tonemap + features in one pass)
-hipError_t tonemap_features(const void* hist, bool hist64, size_t npix, float scale_constant, float brightness,
-                            float gamma, float vibrancy, void* feat, float* base, hipStream_t s);
+// (tonemap + features in one pass); with `hits` (device) the scale constant
+// comes from the bin call's hit counter instead of scale_constant
+hipError_t tonemap_features(const void* hist, bool hist64, size_t npix, float scale_constant,
+                            const unsigned long long* hits, float brightness, float gamma, float vibrancy, void* feat,
+                            float* base, hipStream_t s);
 hipError_t tonemap(const void* in, int in_kind, size_t npix, void* out, int format, float scale_constant,
                    float brightness, float gamma, float vibrancy, hipStream_t s);
 

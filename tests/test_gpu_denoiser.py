@@ -218,7 +218,8 @@ def test_conv_pair_output_layer(gpu, N, H, W, cout):
 @pytest.mark.parametrize("hist_bits", [32, 64])
 def test_tonemap_features_one_pass(db, gpu, hist_bits):
     """rfk_tonemap_features equals rfk_tonemap (float4) then
-    rfk_denoiser_features bit for bit, feat and base, from 32- and 64-bit bins"""
+    rfk_denoiser_features bit for bit, feat and base, from 32- and 64-bit
+    bins; without a quality it reads the bin call's own on the device"""
     W, H = 320, 180
     f = fl.flame_small(2)
     k = db.compile(f, hist_bits=hist_bits)
@@ -226,7 +227,7 @@ def test_tonemap_features_one_pass(db, gpu, hist_bits):
     st.bin(iters=8192 * 256)
     inf = st.info()
     lib = _lib.lib()
-    args = _lib.TonemapArgs(0.0, f["gamma"], f["brightness"], f["vibrancy"], 0.0)
+    args = _lib.TonemapArgs(inf.quality, f["gamma"], f["brightness"], f["vibrancy"], 0.0)
     npx = W * H
     feat, base = render.DeviceBuffer(npx * 16), render.DeviceBuffer(npx * 12)
     _lib.check(lib.rfk_tonemap_features(st.handle, C.byref(args), feat.ptr, base.ptr))
@@ -239,6 +240,14 @@ def test_tonemap_features_one_pass(db, gpu, hist_bits):
     assert np.array_equal(a, b)
     assert np.array_equal(base.download(np.uint32, (npx, 3)), base2.download(np.uint32, (npx, 3)))
     assert np.count_nonzero(a[:, 5]) > 500  # covered pixels
+    # quality 0: the bin call's quality from its counters on the device (a
+    # scale constant within an ulp or so of the host's)
+    dev = _lib.TonemapArgs(0.0, f["gamma"], f["brightness"], f["vibrancy"], 0.0)
+    feat3, base3 = render.DeviceBuffer(npx * 16), render.DeviceBuffer(npx * 12)
+    _lib.check(lib.rfk_tonemap_features(st.handle, C.byref(dev), feat3.ptr, base3.ptr))
+    _lib.check(lib.rfk_synchronize())
+    np.testing.assert_allclose(base3.download(np.float32, (npx, 3)), base.download(np.float32, (npx, 3)),
+                               rtol=1e-5, atol=1e-6)
     assert lib.rfk_tonemap_features(st.handle, None, feat.ptr, base.ptr) == _lib.RFK_ERR_INVALID
 
 
