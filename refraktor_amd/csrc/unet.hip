@@ -324,7 +324,8 @@ __global__ void reduce_partials_kernel(const float* __restrict__ partial, int gr
     if (g >= count) return;
     float s = 0.f;
     for (int k = 0; k < groups; k++) s += partial[size_t(k) * count + g];
-    out[g] = beta * out[g] + s;
+    // beta 0 overwrites: the output may hold anything, NaN included
+    out[g] = beta == 0.f ? s : beta * out[g] + s;
 }
 
 // dbias[co] = sum over pixels of dY; one block per channel group of 64
@@ -339,7 +340,7 @@ __global__ void bias_grad_kernel(const bf16* __restrict__ dy, size_t npix, int C
         if (int(threadIdx.x) < k) s[threadIdx.x] += s[threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) db[c] = beta * db[c] + s[0];
+    if (threadIdx.x == 0) db[c] = beta == 0.f ? s[0] : beta * db[c] + s[0];
 }
 
 // W'[ci][tap][co] = W[co][8 - tap][ci]: the data gradient is the forward

@@ -281,6 +281,8 @@ def test_conv_gradients_match_oracle(gpu, case):
     gy = nn.to_bf16(rng.normal(size=(N, H, W, Co)).astype(np.float32))
     dx_, dgy, dw_ = _bf16_dev(x), _bf16_dev(gy), _bf16_dev(w)
     gw = render.DeviceBuffer(Co * 9 * (c1 + c2) * 4)
+    # the output's old contents (NaN here) must not leak into an overwrite
+    _lib.check(_lib.lib().rfk_memset(gw.ptr, 0xFF, gw.nbytes))
     _lib.check(_lib.lib().rfk_nn_conv3x3_wgrad(dx_.ptr, c1 + c2, None, 0, dgy.ptr, Co, N, H, W, gw.ptr, None))
     np.testing.assert_allclose(gw.download(np.float32, (Co, 9, c1 + c2)), nn.conv3x3_wgrad(x, gy),
                                rtol=1e-3, atol=1e-3)
