@@ -21,6 +21,7 @@ namespace {
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int KC = 32;       // input channels staged per step
 constexpr int PAD = 8;       // LDS row padding (bf16) against bank conflicts
@@ -39,8 +40,6 @@ __device__ __forceinline__ bf16x8 zero8() {
     return z;
 }
 
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
 // Raw buffer reads: a lane whose byte offset lies past the resource's size
 // reads zeros, so a masked-off lane passes kBufferOOB (beyond any resource,
 // all of which hold fewer than 2^31 bytes) instead of selecting zero after
@@ -51,6 +50,10 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uns
 }
 __device__ __forceinline__ bf16x8 buffer_load8(__amdgpu_buffer_rsrc_t r, unsigned byte_off) {
     return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0));
+}
+// a store at kBufferOOB (or into a resource of size 0) writes nothing
+__device__ __forceinline__ void buffer_store8(bf16x8 v, __amdgpu_buffer_rsrc_t r, unsigned byte_off) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, byte_off, 0, 0);
 }
 
 // act_fwd with the activation's branch taken out of the lanes: relu (act 1)
@@ -644,6 +647,26 @@ __device__ __forceinline__ void static_for(F&& f) {
     }
 }
 
+// COUNT 16-byte fragments into LDS by NT threads: thread t takes fragments
+// t, t + NT, ... = src(e), written by dst(e, value). All of a thread's loads
+// are in flight before its first LDS write, so staging a layer's weights
+// costs one memory latency instead of one per fragment.
+template <int NT, int COUNT, class Src, class Dst>
+__device__ __forceinline__ void stage_fragments(int t, Src&& src, Dst&& dst) {
+    constexpr int PER = (COUNT + NT - 1) / NT;
+    bf16x8 v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int e = t + k * NT;
+        v[k] = (COUNT % NT == 0 || e < COUNT) ? src(e) : zero8();
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int e = t + k * NT;
+        if (COUNT % NT == 0 || e < COUNT) dst(e, v[k]);
+    }
+}
+
 // Every wave walks its own 32-pixel column strip down `rows` output rows for
 // 32 output channels (grid.z = images x 32-channel tiles), input-stationary:
 // each staged input row (one KC-channel chunk at a time, NCH chunks) feeds
@@ -664,7 +687,16 @@ __device__ __forceinline__ void static_for(F&& f) {
 // offsets inside one image are 32-bit (the host checks every image's tensors
 // hold fewer than 2^30 bytes). NW waves per workgroup share the layer's
 // weights in LDS.
-template <int KC, int NCH, int MODE, int NW>
+// OUT is the store: 0 BF16 rows of whole 32-channel tiles through LDS, 1 the
+// same plus their 2x2 average pool, 2 anything else (FP32 output, the rgb
+// output layer, channel counts that end inside a tile). With OUT 0 and 1
+// every wave issues the same global loads and stores in the same order
+// whatever its position: rows past the strip or the image and columns past
+// the image load and store through buffer offsets beyond the resource (reads
+// give zeros, writes are dropped), so the compiler's wait counts before each
+// staged chunk only wait for that chunk's loads, not for the ones fetched
+// after it.
+template <int KC, int NCH, int MODE, int NW, int OUT>
 __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __restrict__ x1, int c1,
                                                             const bf16* __restrict__ x2, int c2, int H, int W,
                                                             const bf16* __restrict__ wgt,
@@ -696,14 +728,6 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
     const __amdgpu_buffer_rsrc_t r1 = buffer_rsrc(x1n, unsigned(H1 * W1 * c1) * 2u);
     const __amdgpu_buffer_rsrc_t r2 = buffer_rsrc(x2n ? x2n : x1n, x2n ? unsigned(H * W * c2) * 2u : 0u);
 
-    for (int e = t; e < 9 * 32 * GW; e += 64 * NW) {
-        const int g = e % GW, tap = (e / GW) % 9, co = e / (9 * GW);
-        bf16x8 v = zero8();
-        if (co0 + co < Cout && g * 8 < Cin) v = load8(wgt + unsigned(((co0 + co) * 9 + tap) * Cin + g * 8));
-        *reinterpret_cast<bf16x8*>(s_w + (co * 9 + tap) * WP + g * 8) = v;
-    }
-    if (t < 32) s_bias[t] = (bias && co0 + t < Cout) ? bias[co0 + t] : 0.f;
-
     // the lane's part of a staged row: channel group g, column (l + 64 q) / G
     const int g = l % G;
     unsigned o1c[NQ], o2c[NQ];
@@ -715,11 +739,12 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
         o1c[q] = unsigned((MODE == 1 ? (gx >> 1) : gx) * c1 + g * 8) * 2u;
         o2c[q] = unsigned(gx * c2 + g * 8 - c1) * 2u;  // x2's channel is c - c1
     }
-    // (input row jj of the strip = image row y0 - 1 + jj, chunk CH) -> registers
-    auto load_chunk = [&](int jj, auto chc, bf16x8* v) __attribute__((always_inline)) {
+    // (input row jj of the strip = image row y0 - 1 + jj, chunk CH) -> registers;
+    // zeros where `live` is false
+    auto load_chunk = [&](int jj, auto chc, bf16x8* v, bool live) __attribute__((always_inline)) {
         constexpr int CH = decltype(chc)::value, CB = CH * KC;
         const int gy = y0 - 1 + jj;
-        const bool row_ok = gy >= 0 && gy < H;
+        const bool row_ok = live && gy >= 0 && gy < H;
         const bool ch_ok = g * 8 < Cin - CB;
         if (MODE == 2) {
 #pragma unroll
@@ -747,54 +772,63 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
     const bool relu = act == 1;
     const float neg_slope = act == 2 ? slope : 1.f;
     const int px = x0 + m;
-    const bool via_lds = y && !yf && !rgb && (Cout & 31) == 0;
-    // with ypool (even rows per strip, even H and W), the 2x2 average pool of
+    // OUT 0 / 1: the image's output (and pool) tensors as buffer resources
+    const unsigned y_img = unsigned(H * W * Cout), p_img = unsigned((H / 2) * (W / 2) * Cout);
+    const __amdgpu_buffer_rsrc_t ry = buffer_rsrc(OUT == 2 ? x1 : y + size_t(n) * y_img, OUT == 2 ? 0u : y_img * 2u);
+    const __amdgpu_buffer_rsrc_t rp = buffer_rsrc(OUT == 1 ? ypool + size_t(n) * p_img : y, OUT == 1 ? p_img * 2u : 0u);
+    // with OUT 1 (even rows per strip, even H and W) the 2x2 average pool of
     // the BF16 output as well, summed in the pooling kernel's order: the even
     // row's pixel pair, then the odd row's
     bf16x8 prev[2];
-    auto store_row = [&](int yy, const f32x16& acc) __attribute__((always_inline)) {
-        if (via_lds) {
-            // the wave's reads of s_x for this row are issued before
-            // these writes, its next chunk's writes after these reads
-            __asm__ volatile("" ::: "memory");
-            bf16x4 b[4];
-            act16_bf16(acc, s_bias, kh, relu, neg_slope, b);
+    // OUT 0 / 1: every row goes through here, `valid` false for the rows
+    // outside the strip or the image
+    auto store_tile_row = [&](int yy, const f32x16& acc, bool valid) __attribute__((always_inline)) {
+        // the wave's reads of s_x for this row are issued before these
+        // writes, its next chunk's writes after these reads
+        __asm__ volatile("" ::: "memory");
+        bf16x4 b[4];
+        act16_bf16(acc, s_bias, kh, relu, neg_slope, b);
 #pragma unroll
-            for (int i = 0; i < 4; i++) *reinterpret_cast<bf16x4*>(s_x + m * OXP + 8 * i + 4 * kh) = b[i];
-            __asm__ volatile("" ::: "memory");
-            const size_t o = (size_t(n) * H + yy) * W;
+        for (int i = 0; i < 4; i++) *reinterpret_cast<bf16x4*>(s_x + m * OXP + 8 * i + 4 * kh) = b[i];
+        __asm__ volatile("" ::: "memory");
+        const bool odd = (yy & 1) != 0;
 #pragma unroll
-            for (int r = 0; r < 2; r++) {
-                const int p = r * 16 + (l >> 2), cg = l & 3;
-                const bf16x8 v = *reinterpret_cast<const bf16x8*>(s_x + p * OXP + cg * 8);
-                if (x0 + p < W) *reinterpret_cast<bf16x8*>(y + (o + x0 + p) * Cout + co0 + cg * 8) = v;
-                if (ypool) {
-                    if ((yy & 1) == 0) {
-                        prev[r] = v;
-                    } else {
-                        // pixel p + 1 of both rows sits four lanes up
-                        const u32x4 pa = __builtin_bit_cast(u32x4, prev[r]), pv = __builtin_bit_cast(u32x4, v);
-                        u32x4 na, nv;
+        for (int r = 0; r < 2; r++) {
+            const int p = r * 16 + (l >> 2), cg = l & 3;
+            const bf16x8 v = *reinterpret_cast<const bf16x8*>(s_x + p * OXP + cg * 8);
+            const bool ok = valid && x0 + p < W;
+            buffer_store8(v, ry, ok ? unsigned(((yy * W + x0 + p) * Cout + co0 + cg * 8) * 2) : kBufferOOB);
+            if constexpr (OUT == 1) {
+                // the pool is computed on odd rows only (a wave-uniform
+                // branch holding no global memory operation); its store is
+                // issued on every row
+                bf16x8 pool = v;
+                if (odd) {
+                    // pixel p + 1 of both rows sits four lanes up
+                    const u32x4 pa = __builtin_bit_cast(u32x4, prev[r]), pv = __builtin_bit_cast(u32x4, v);
+                    u32x4 na, nv;
 #pragma unroll
-                        for (int j = 0; j < 4; j++) {
-                            na[j] = unsigned(__shfl_down(int(pa[j]), 4));
-                            nv[j] = unsigned(__shfl_down(int(pv[j]), 4));
-                        }
-                        const bf16x8 b = __builtin_bit_cast(bf16x8, na), d = __builtin_bit_cast(bf16x8, nv);
-                        bf16x8 pool;
-#pragma unroll
-                        for (int j = 0; j < 8; j++)
-                            pool[j] = bf16(0.25f * (float(prev[r][j]) + float(b[j]) + float(v[j]) + float(d[j])));
-                        if ((p & 1) == 0 && x0 + p < W) {
-                            const size_t po = (size_t(n) * (H / 2) + yy / 2) * (W / 2) + (x0 + p) / 2;
-                            *reinterpret_cast<bf16x8*>(ypool + po * Cout + co0 + cg * 8) = pool;
-                        }
+                    for (int j = 0; j < 4; j++) {
+                        na[j] = unsigned(__shfl_down(int(pa[j]), 4));
+                        nv[j] = unsigned(__shfl_down(int(pv[j]), 4));
                     }
+                    const bf16x8 bq = __builtin_bit_cast(bf16x8, na), d = __builtin_bit_cast(bf16x8, nv);
+#pragma unroll
+                    for (int j = 0; j < 8; j++)
+                        pool[j] = bf16(0.25f * (float(prev[r][j]) + float(bq[j]) + float(v[j]) + float(d[j])));
+                } else {
+                    prev[r] = v;
                 }
+                const bool okp = ok && odd && (p & 1) == 0;
+                buffer_store8(pool, rp,
+                              okp ? unsigned((((yy >> 1) * (W >> 1) + ((x0 + p) >> 1)) * Cout + co0 + cg * 8) * 2)
+                                  : kBufferOOB);
             }
-            __asm__ volatile("" ::: "memory");
-            return;
         }
+        __asm__ volatile("" ::: "memory");
+    };
+    // OUT 2
+    auto store_row = [&](int yy, const f32x16& acc) __attribute__((always_inline)) {
         if (px >= W) return;
         if (rgb) {
             // the network's output layer: rgb = clamp(base + first three channels)
@@ -850,8 +884,22 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
             else if constexpr (S == 1) return pb;
             else return pc;
         }();
-        load_chunk(S / NCH, std::integral_constant<int, S % NCH>{}, buf);
+        load_chunk(S / NCH, std::integral_constant<int, S % NCH>{}, buf, S / NCH < rows + 2);
     });
+    // the tile's weights [32 out][9 taps][WP], their loads issued after the
+    // first three chunks'
+    stage_fragments<64 * NW, 9 * 32 * GW>(
+        t,
+        [&](int e) __attribute__((always_inline)) {
+            const int g = e % GW, tap = (e / GW) % 9, co = e / (9 * GW);
+            return co0 + co < Cout && g * 8 < Cin ? load8(wgt + unsigned(((co0 + co) * 9 + tap) * Cin + g * 8))
+                                                  : zero8();
+        },
+        [&](int e, bf16x8 v) __attribute__((always_inline)) {
+            const int g = e % GW, tap = (e / GW) % 9, co = e / (9 * GW);
+            *reinterpret_cast<bf16x8*>(s_w + (co * 9 + tap) * WP + g * 8) = v;
+        });
+    if (t < 32) s_bias[t] = (bias && co0 + t < Cout) ? bias[co0 + t] : 0.f;
     __syncthreads();  // weights and biases staged; from here on every wave runs on its own
     if (x0 >= W) return;  // a wave wholly right of the image (the last column of workgroups)
 
@@ -867,7 +915,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
     auto row = [&](auto rc, int jj, f32x16& aA, f32x16& aB, f32x16& aC) __attribute__((always_inline)) {
         constexpr int R = decltype(rc)::value;
         const int gy = y0 - 1 + jj;
-        const bool in_img = gy >= 0 && gy < H;
+        const bool in_img = jj < rows + 2 && gy >= 0 && gy < H;
         static_for<0, NCH>([&](auto chc) __attribute__((always_inline)) {
             constexpr int CH = decltype(chc)::value, S = R * NCH + CH;
             auto& buf = [&]() __attribute__((always_inline)) -> bf16x8 (&)[NQ] {
@@ -886,8 +934,12 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
             __asm__ volatile("" ::: "memory");
             // step s + 3
             constexpr int NJ = (CH + 3) / NCH, NC = (CH + 3) % NCH;
-            if (jj + NJ < rows + 2) load_chunk(jj + NJ, std::integral_constant<int, NC>{}, buf);
-            if (!in_img) return;  // rows outside the image add nothing
+            if constexpr (OUT == 2) {
+                if (jj + NJ < rows + 2) load_chunk(jj + NJ, std::integral_constant<int, NC>{}, buf, true);
+            } else {
+                load_chunk(jj + NJ, std::integral_constant<int, NC>{}, buf, jj + NJ < rows + 2);
+            }
+            if (!in_img) return;  // rows outside the image or the strip add nothing
             const bf16* sx = s_x + m * XP + kh * 8;
             const bf16* sw = s_w + m * 9 * WP + CH * KC + kh * 8;
             // operands of step i = (dx = i / KS, ks = i % KS): the pixels are
@@ -919,7 +971,11 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
                 xc = xn;
             }
         });
-        if (jj >= 2 && y0 + jj - 2 < H) store_row(y0 + jj - 2, aC);
+        if constexpr (OUT == 2) {
+            if (jj >= 2 && y0 + jj - 2 < H) store_row(y0 + jj - 2, aC);
+        } else {
+            store_tile_row(y0 + jj - 2, aC, jj >= 2 && jj < rows + 2 && y0 + jj - 2 < H);
+        }
 #pragma unroll
         for (int i = 0; i < 16; i++) aC[i] = 0.f;
     };
@@ -927,7 +983,11 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
         static_for<0, 3>([&](auto rc) __attribute__((always_inline)) {
             constexpr int R = decltype(rc)::value;
             const int jj = jb + R;
-            if (jj >= rows + 2) return;
+            // OUT 0 / 1 run whole groups of three rows, the ones past the
+            // strip as dead rows, so every group issues the same memory ops
+            if constexpr (OUT == 2) {
+                if (jj >= rows + 2) return;
+            }
             if constexpr (R == 0) row(rc, jj, a0, a2, a1);
             else if constexpr (R == 1) row(rc, jj, a1, a0, a2);
             else row(rc, jj, a2, a1, a0);
@@ -977,25 +1037,6 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
     const bf16* xn = x + size_t(n) * H * W * Cin;
     const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(xn, unsigned(H * W * Cin) * 2u);
 
-    for (int e = t; e < 32 * 9 * (WPA / 8); e += 64 * NW) {
-        const int g = e % (WPA / 8), tap = (e / (WPA / 8)) % 9, co = e / (9 * (WPA / 8));
-        bf16x8 v = zero8();
-        if (g * 8 < Cin) v = load8(wa + unsigned((co * 9 + tap) * Cin + g * 8));
-        *reinterpret_cast<bf16x8*>(s_wa + (co * 9 + tap) * WPA + g * 8) = v;
-    }
-    for (int e = t; e < 32 * WB_ROWS * (MP / 8); e += 64 * NW) {
-        const int g = e % (MP / 8), j = (e / (MP / 8)) % WB_ROWS, r = e / (WB_ROWS * (MP / 8));
-        bf16x8 v = zero8();
-        if (g < 4) {
-            if (BKIND == 0) {
-                if (r < cout_b) v = load8(wb + unsigned((r * 9 + j) * 32 + g * 8));
-            } else {
-                const int dy = r >> 3, co = r & 7;
-                if (dy < 3 && co < cout_b) v = load8(wb + unsigned((co * 9 + dy * 3 + j) * 32 + g * 8));
-            }
-        }
-        *reinterpret_cast<bf16x8*>(s_wb + (r * WB_ROWS + j) * MP + g * 8) = v;
-    }
     if (t < 32) {
         s_ba[t] = ba ? ba[t] : 0.f;
         s_bb[t] = (bb && t < cout_b) ? bb[t] : 0.f;
@@ -1027,6 +1068,36 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
     load_row(0, pa);
     load_row(1, pb);
     load_row(2, pc);
+    // both layers' weights, their loads issued after the first three rows'
+    stage_fragments<64 * NW, 32 * 9 * (WPA / 8)>(
+        t,
+        [&](int e) __attribute__((always_inline)) {
+            const int g = e % (WPA / 8), tap = (e / (WPA / 8)) % 9, co = e / (9 * (WPA / 8));
+            return g * 8 < Cin ? load8(wa + unsigned((co * 9 + tap) * Cin + g * 8)) : zero8();
+        },
+        [&](int e, bf16x8 v) __attribute__((always_inline)) {
+            const int g = e % (WPA / 8), tap = (e / (WPA / 8)) % 9, co = e / (9 * (WPA / 8));
+            *reinterpret_cast<bf16x8*>(s_wa + (co * 9 + tap) * WPA + g * 8) = v;
+        });
+    stage_fragments<64 * NW, 32 * WB_ROWS * (MP / 8)>(
+        t,
+        [&](int e) __attribute__((always_inline)) {
+            const int g = e % (MP / 8), j = (e / (MP / 8)) % WB_ROWS, r = e / (WB_ROWS * (MP / 8));
+            bf16x8 v = zero8();
+            if (g < 4) {
+                if (BKIND == 0) {
+                    if (r < cout_b) v = load8(wb + unsigned((r * 9 + j) * 32 + g * 8));
+                } else {
+                    const int dy = r >> 3, co = r & 7;
+                    if (dy < 3 && co < cout_b) v = load8(wb + unsigned((co * 9 + dy * 3 + j) * 32 + g * 8));
+                }
+            }
+            return v;
+        },
+        [&](int e, bf16x8 v) __attribute__((always_inline)) {
+            const int g = e % (MP / 8), j = (e / (MP / 8)) % WB_ROWS, r = e / (WB_ROWS * (MP / 8));
+            *reinterpret_cast<bf16x8*>(s_wb + (r * WB_ROWS + j) * MP + g * 8) = v;
+        });
     __syncthreads();  // weights and biases staged; from here on every wave runs on its own
     if (xs >= W) return;
 
@@ -1621,15 +1692,15 @@ struct strip_args {
     float* rgb;
 };
 
-template <int KC, int NCH, int M, int NW>
+template <int KC, int NCH, int M, int NW, int OUT>
 static hipError_t launch_strip_nch(const strip_args& a, size_t lds, hipStream_t s) {
-    const void* fn = reinterpret_cast<const void*>(&conv3x3_strip_kernel<KC, NCH, M, NW>);
+    const void* fn = reinterpret_cast<const void*>(&conv3x3_strip_kernel<KC, NCH, M, NW, OUT>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
     const int zdim = a.N * ((a.Cout + 31) / 32);
     const int rows = strip_rows(fn, lds, NW, zdim, a.H, a.W, a.ypool != nullptr);
     dim3 grid((a.W + 32 * NW - 1) / (32 * NW), (a.H + rows - 1) / rows, zdim);
-    conv3x3_strip_kernel<KC, NCH, M, NW><<<grid, 64 * NW, lds, s>>>(
+    conv3x3_strip_kernel<KC, NCH, M, NW, OUT><<<grid, 64 * NW, lds, s>>>(
         (const bf16*)a.x1, a.c1, (const bf16*)a.x2, a.c2, a.H, a.W, (const bf16*)a.wgt, a.bias, a.Cout, a.act,
         a.slope, rows, (bf16*)a.y, (bf16*)a.ypool, a.yf, a.rbase, a.rgb);
     return hipGetLastError();
@@ -1637,31 +1708,50 @@ static hipError_t launch_strip_nch(const strip_args& a, size_t lds, hipStream_t 
 
 // the kernel instance for the layer's chunk count (KC 16: one chunk; KC 32:
 // up to seven, 224 input channels, the most whose weights fit in LDS)
-template <int KC, int M, int NW>
+template <int KC, int M, int NW, int OUT>
 static hipError_t launch_strip_mode(const strip_args& a, size_t lds, hipStream_t s) {
     const int nch = (a.c1 + a.c2 + KC - 1) / KC;
     if constexpr (KC == 16) {
-        if (nch == 1) return launch_strip_nch<KC, 1, M, NW>(a, lds, s);
+        if (nch == 1) return launch_strip_nch<KC, 1, M, NW, OUT>(a, lds, s);
     } else {
         switch (nch) {
-            case 1: return launch_strip_nch<KC, 1, M, NW>(a, lds, s);
-            case 2: return launch_strip_nch<KC, 2, M, NW>(a, lds, s);
-            case 3: return launch_strip_nch<KC, 3, M, NW>(a, lds, s);
-            case 4: return launch_strip_nch<KC, 4, M, NW>(a, lds, s);
-            case 5: return launch_strip_nch<KC, 5, M, NW>(a, lds, s);
-            case 6: return launch_strip_nch<KC, 6, M, NW>(a, lds, s);
-            case 7: return launch_strip_nch<KC, 7, M, NW>(a, lds, s);
+            case 1: return launch_strip_nch<KC, 1, M, NW, OUT>(a, lds, s);
+            case 2: return launch_strip_nch<KC, 2, M, NW, OUT>(a, lds, s);
+            case 3: return launch_strip_nch<KC, 3, M, NW, OUT>(a, lds, s);
+            case 4: return launch_strip_nch<KC, 4, M, NW, OUT>(a, lds, s);
+            case 5: return launch_strip_nch<KC, 5, M, NW, OUT>(a, lds, s);
+            case 6: return launch_strip_nch<KC, 6, M, NW, OUT>(a, lds, s);
+            case 7: return launch_strip_nch<KC, 7, M, NW, OUT>(a, lds, s);
             default: break;
         }
     }
     return hipErrorInvalidValue;
 }
 
+// the strip kernel's store (OUT, see conv3x3_strip_kernel) for a layer's outputs
+static int strip_out(const strip_args& a) {
+    if (a.y && !a.yf && !a.rgb && (a.Cout % 32) == 0) return a.ypool ? 1 : 0;
+    return 2;
+}
+
+// the instances built: direct reads with every store; the upsampling and
+// pooling reads with whole BF16 tiles and no pool (the other layers take the
+// tiled kernel). The LDS size was computed for nw waves.
+static bool strip_instance(int mode, int nw, int out) {
+    if (mode == 0) return nw == 8;
+    if (mode == 1) return (nw == 8 || nw == 12) && out == 0;
+    return mode == 2 && nw == 4 && out == 0;
+}
+
 template <int KC>
 static hipError_t launch_strip(const strip_args& a, int mode, int nw, size_t lds, hipStream_t s) {
-    if (mode == 1) return nw == 12 ? launch_strip_mode<KC, 1, 12>(a, lds, s) : launch_strip_mode<KC, 1, 8>(a, lds, s);
-    if (mode == 2) return launch_strip_mode<KC, 2, 4>(a, lds, s);
-    return launch_strip_mode<KC, 0, 8>(a, lds, s);
+    const int out = strip_out(a);
+    if (!strip_instance(mode, nw, out)) return hipErrorInvalidValue;
+    if (mode == 1) return nw == 12 ? launch_strip_mode<KC, 1, 12, 0>(a, lds, s) : launch_strip_mode<KC, 1, 8, 0>(a, lds, s);
+    if (mode == 2) return launch_strip_mode<KC, 2, 4, 0>(a, lds, s);
+    if (out == 0) return launch_strip_mode<KC, 0, 8, 0>(a, lds, s);
+    if (out == 1) return launch_strip_mode<KC, 0, 8, 1>(a, lds, s);
+    return launch_strip_mode<KC, 0, 8, 2>(a, lds, s);
 }
 
 template <int R, int C, int KCH>
@@ -1691,18 +1781,23 @@ hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
     // measured fastest there (96->32 at 1080p 143 us against 192 us
     // weight-stationary, 128->128 at 135x240 against the tiled kernel)
     const int KC = Cin <= 16 ? 16 : 32;
+    // the pool comes out of the store epilogue of whole 32-channel BF16
+    // tiles read directly
+    const bool pool_fused = ypool && !yf && (Cout % 32) == 0 && x1_mode == 0;
+    const strip_args a{x1, c1, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, pool_fused ? ypool : nullptr, yf,
+                       nullptr, nullptr};
+    const int out = strip_out(a);
     const int nw = strip_waves(x1_mode, Cin, KC);
     const size_t lds = strip_lds_bytes(Cin, KC, nw);
     // (the strip kernel reads every KC-channel chunk from one of x1, x2)
     const bool chunks_split = c2 == 0 || c1 % KC == 0;
     if (strip_weight_bytes(Cin, KC) <= 140 * 1024 && lds <= 160 * 1024 && offsets32 && chunks_split) {
-        // the pool comes out of the store epilogue of whole 32-channel BF16 tiles
-        const bool pool_fused = ypool && !yf && (Cout % 32) == 0;
-        const strip_args a{x1, c1, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, pool_fused ? ypool : nullptr, yf,
-                           nullptr, nullptr};
-        hipError_t e = KC == 16 ? launch_strip<16>(a, x1_mode, nw, lds, s) : launch_strip<32>(a, x1_mode, nw, lds, s);
-        if (e != hipSuccess || !ypool || pool_fused) return e;
-        return avgpool2_forward(y, N, H, W, Cout, ypool, s);
+        if (strip_instance(x1_mode, nw, out)) {
+            hipError_t e =
+                KC == 16 ? launch_strip<16>(a, x1_mode, nw, lds, s) : launch_strip<32>(a, x1_mode, nw, lds, s);
+            if (e != hipSuccess || !ypool || pool_fused) return e;
+            return avgpool2_forward(y, N, H, W, Cout, ypool, s);
+        }
     }
     // wider inputs: the weight-stationary strip; the tiled kernel for the rest
     if (Cin >= 96 && Cout <= 96 && offsets32) {
