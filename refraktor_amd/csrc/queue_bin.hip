@@ -44,6 +44,13 @@ constexpr uint32_t kThreads = 1024;
 // memory traffic (2.5 % off a 2^30-iteration bin call against 1024 / 8192)
 constexpr uint32_t kScatterThreads = 512, kScatterPiece = 4096;
 constexpr uint32_t kSegment = 1u << 18;  // records per bin work item (u32 LDS sums cannot wrap)
+// copies of a work item's LDS tile sums: lane l adds into copy l % kCopies,
+// so the lanes of one wave that draw into the same pixel (a flame's hottest
+// pixels take 10-13 % of their tile's records) split over kCopies addresses
+// and their LDS atomics serialise half as deep (the hottest tile's work items
+// of a 2^25-iteration 1080p frame 125 -> 80 us)
+constexpr uint32_t kCopies = 2;
+constexpr size_t kBinLds = size_t(kCopies) * 2 * kTilePix * sizeof(unsigned long long);
 
 __device__ __forceinline__ uint32_t rec_tile(u64 r) { return uint32_t(r >> 44); }
 
@@ -249,9 +256,10 @@ __device__ __forceinline__ void add_bin(HistT* __restrict__ hist, size_t p, uint
     if (w) atomicAdd(h + 3, HistT(w));
 }
 
-// LDS sums of a work item: (r | g << 32) and (b | w << 32) per pixel. A work
-// item holds at most 2^18 records of 8-bit channels, so each 32-bit half
-// stays below 2^26 and never carries into its neighbour. Chunk tile t is
+// LDS sums of a work item: (r | g << 32) and (b | w << 32) per pixel, in
+// kCopies copies (dynamic LDS). A work item holds at most 2^18 records of
+// 8-bit channels, so each 32-bit half stays below 2^26 and never carries into
+// its neighbour. Chunk tile t is
 // tile t % tpf of frame group_frame[t / tpf] (frame 0 without a list), whose
 // bins start frame_elems elements after the previous frame's.
 template <class HistT>
@@ -262,13 +270,12 @@ __global__ void __launch_bounds__(kThreads)
                 const uint32_t* __restrict__ group_frame, size_t frame_elems, HistT* __restrict__ hist,
                 bool packed64) {
     static_assert(kSegment <= (1u << 24), "32-bit halves of the LDS sums must not carry");
-    __shared__ u64 acc[2][kTilePix];
+    extern __shared__ u64 qbin_lds[];
+    u64(*copies)[2][kTilePix] = reinterpret_cast<u64(*)[2][kTilePix]>(qbin_lds);
+    u64(*acc)[kTilePix] = copies[__lane_id() % kCopies];
     if (blockIdx.x >= nseg[0]) return;
     const uint4 sg = segments[blockIdx.x];
-    for (uint32_t p = threadIdx.x; p < kTilePix; p += kThreads) {
-        acc[0][p] = 0ull;
-        acc[1][p] = 0ull;
-    }
+    for (uint32_t p = threadIdx.x; p < kCopies * 2 * kTilePix; p += kThreads) qbin_lds[p] = 0ull;
     __syncthreads();
     // U records per thread in flight before the first LDS add
     constexpr uint32_t U = 8;
@@ -295,7 +302,12 @@ __global__ void __launch_bounds__(kThreads)
     const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
     const bool exclusive = sg.w != 0u;
     for (uint32_t p = threadIdx.x; p < kTilePix; p += kThreads) {
-        const u64 rg = acc[0][p], bw = acc[1][p];
+        u64 rg = 0ull, bw = 0ull;
+#pragma unroll
+        for (uint32_t c = 0; c < kCopies; c++) {
+            rg += copies[c][0][p];
+            bw += copies[c][1][p];
+        }
         if ((rg | bw) == 0ull) continue;
         const uint32_t r = uint32_t(rg), g = uint32_t(rg >> 32), b = uint32_t(bw), w = uint32_t(bw >> 32);
         const uint32_t x = (tx << kTileLog2) + (p & 63u), y = (ty << kTileLog2) + (p >> kTileLog2);
@@ -338,12 +350,16 @@ hipError_t queue_bin_groups(const unsigned long long* records, size_t per_part, 
     uint16_t* sorted_loc = reinterpret_cast<uint16_t*>(sorted_col + n);
     qscatter_kernel<kScatterThreads, kScatterPiece><<<nparts, kScatterThreads, lds, s>>>(
         records, per_part, tpf, ncols, counts, tile_start, sorted_col, sorted_loc);
+    const void* bin_fn = hist64 ? reinterpret_cast<const void*>(&qbin_kernel<u64>)
+                                : reinterpret_cast<const void*>(&qbin_kernel<uint32_t>);
+    e = hipFuncSetAttribute(bin_fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(kBinLds));
+    if (e != hipSuccess) return e;
     if (hist64)
-        qbin_kernel<<<max_segments, kThreads, 0, s>>>(sorted_col, sorted_loc, reinterpret_cast<const uint4*>(segments),
+        qbin_kernel<<<max_segments, kThreads, kBinLds, s>>>(sorted_col, sorted_loc, reinterpret_cast<const uint4*>(segments),
                                                       nseg, width, height, tiles_x, tpf, group_frame, frame_elems,
                                                       reinterpret_cast<u64*>(hist), false);
     else
-        qbin_kernel<<<max_segments, kThreads, 0, s>>>(sorted_col, sorted_loc, reinterpret_cast<const uint4*>(segments),
+        qbin_kernel<<<max_segments, kThreads, kBinLds, s>>>(sorted_col, sorted_loc, reinterpret_cast<const uint4*>(segments),
                                                       nseg, width, height, tiles_x, tpf, group_frame, frame_elems,
                                                       hist, packed64);
     return hipGetLastError();
