@@ -79,11 +79,13 @@ struct rfk_state {
     double* staging = nullptr;  // pinned host copy of the samples being uploaded
     size_t staging_reals = 0;
     hipEvent_t ev_staging = nullptr;
-    // queue binning scratch (kernels compiled with bin_queue)
+    // queue binning scratch (kernels compiled with bin_queue): record
+    // buffers for chunks of at most chunk_passes passes, allocated for
+    // queue_passes passes (grown on demand by ensure_queue)
     unsigned long long *queue = nullptr, *sorted = nullptr;
     uint32_t *qcounts = nullptr, *qnseg = nullptr, *qtiles = nullptr;
     void* qsegments = nullptr;
-    uint32_t chunk_passes = 0, max_segments = 0;
+    uint32_t chunk_passes = 0, queue_passes = 0, max_segments = 0;
     // density estimation: per-level 1/Z table for the last (max, min) radius
     // pair, and the per-pixel level scratch
     float* de_inv_z = nullptr;
@@ -213,13 +215,37 @@ int launch_iterate(rfk_state* s, rfk_iter_args& a) {
     return RFK_OK;
 }
 
-// Queue binning: chunks of `chunk_passes` passes, each followed by the sort
-// and per-tile sum of its records. With an iteration budget every chunk is
-// enqueued at once (a bailout makes the remaining chunks record nothing);
-// without one the host checks the device's stop count after every chunk.
+// Queue binning's record buffers (8 B per record, twice, and the work-item
+// list) hold one chunk of min(chunk_passes, passes) passes: a state whose
+// calls bin 2^25 draws keeps 2 x 256 MiB instead of 2 x 1 GiB. A call with a
+// larger chunk regrows them once the state's earlier work has finished.
+int ensure_queue(rfk_state* s, uint64_t passes) {
+    const uint32_t need = uint32_t(std::min<uint64_t>(s->chunk_passes, passes));
+    if (need <= s->queue_passes) return RFK_OK;
+    RFK_HIP(hipStreamSynchronize(s->stream));
+    for (void* p : {static_cast<void*>(s->queue), static_cast<void*>(s->sorted), s->qsegments})
+        if (p) (void)hipFree(p);
+    s->queue = s->sorted = nullptr;
+    s->qsegments = nullptr;
+    s->queue_passes = 0;
+    const size_t q = size_t(need) * s->walkers;
+    s->max_segments = rfk::sk::queue_max_segments(q, rfk::sk::queue_tiles(s->width, s->height));
+    RFK_HIP(hipMalloc(&s->queue, q * 8));
+    RFK_HIP(hipMalloc(&s->sorted, q * 8));
+    RFK_HIP(hipMalloc(&s->qsegments, size_t(s->max_segments) * 16));
+    s->queue_passes = need;
+    return RFK_OK;
+}
+
+// Queue binning: chunks of at most `queue_passes` passes, each followed by
+// the sort and per-tile sum of its records. With an iteration budget every
+// chunk is enqueued at once (a bailout makes the remaining chunks record
+// nothing); without one the host checks the device's stop count after every
+// chunk.
 int bin_queued(rfk_state* s, rfk_iter_args& a, uint64_t passes, bool bounded) {
+    a.queue = reinterpret_cast<u64*>(s->queue);
     for (uint64_t done = 0; done < passes;) {
-        const uint32_t n = uint32_t(std::min<uint64_t>(s->chunk_passes, passes - done));
+        const uint32_t n = uint32_t(std::min<uint64_t>(s->queue_passes, passes - done));
         a.npasses = n;
         if (int rc = launch_iterate(s, a)) return rc;
         RFK_HIP(rfk::sk::queue_bin(s->queue, size_t(n) * s->tpb, s->blocks, s->sorted, s->qcounts, s->qtiles,
@@ -634,17 +660,14 @@ int rfk_warmup_ex(rfk_kernel* k, void* stream, const double* samples, size_t n_r
     s->hist64 = k->opt.hist64;
     RFK_HIPC(dalloc(&s->hist, s->hist_words()));
     if (k->opt.bin_queue) {
-        // chunks of whole passes of all walkers, about 2^27 records (1 GiB) each
+        // chunks of whole passes of all walkers, at most about 2^27 records
+        // (1 GiB) each; their record buffers come with the first bin call
         s->chunk_passes = std::max<uint32_t>(1u, uint32_t((1ull << 27) / s->walkers));
         const size_t q = size_t(s->chunk_passes) * s->walkers;
         const uint32_t ntiles = rfk::sk::queue_tiles(s->width, s->height);
         if (q >= (1ull << 32)) return cleanup_on_error(fail(RFK_ERR_INVALID, "too many walkers for queue binning"));
-        s->max_segments = rfk::sk::queue_max_segments(q, ntiles);
-        RFK_HIPC(hipMalloc(&s->queue, q * 8));
-        RFK_HIPC(hipMalloc(&s->sorted, q * 8));
         RFK_HIPC(dalloc(&s->qcounts, size_t(ntiles) * s->blocks));
         RFK_HIPC(dalloc(&s->qtiles, size_t(ntiles) * 2));
-        RFK_HIPC(hipMalloc(&s->qsegments, size_t(s->max_segments) * 16));
         RFK_HIPC(dalloc(&s->qnseg, 4));
     }
     RFK_HIPC(dalloc(&s->counters, 8));
@@ -708,8 +731,10 @@ int rfk_bin_async(rfk_kernel* k, rfk_state* s, const rfk_bailout* bo) {
     RFK_HIP(hipEventRecord(s->ev0, s->stream));
     if (!k->opt.bin_queue) {
         if (int rc = launch_iterate(s, a)) return rc;
-    } else if (int rc = bin_queued(s, a, passes, bo->iters != 0)) {
+    } else if (int rc = ensure_queue(s, passes)) {
         return rc;
+    } else if (int rc2 = bin_queued(s, a, passes, bo->iters != 0)) {
+        return rc2;
     }
     RFK_HIP(hipEventRecord(s->ev1, s->stream));
     // density statistics after every bin, as the reference does (flame_compiler.cpp:928-937)
