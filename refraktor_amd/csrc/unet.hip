@@ -1055,10 +1055,10 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
         oc[q] = unsigned(gx * Cin + g * 8) * 2u;
     }
     const bool ch_ok = g * 8 < Cin;
-    // input row jj = image row y0 - 2 + jj
-    auto load_row = [&](int jj, bf16x8* v) __attribute__((always_inline)) {
+    // input row jj = image row y0 - 2 + jj; zeros where `live` is false
+    auto load_row = [&](int jj, bf16x8* v, bool live = true) __attribute__((always_inline)) {
         const int gy = y0 - 2 + jj;
-        const bool row_ok = gy >= 0 && gy < H;
+        const bool row_ok = live && gy >= 0 && gy < H;
         const unsigned srow = unsigned(gy * W * Cin) * 2u;
 #pragma unroll
         for (int q = 0; q < NQ; q++) v[q] = buffer_load8(rx, row_ok && colok[q] && ch_ok ? oc[q] + srow : kBufferOOB);
@@ -1117,48 +1117,82 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
     // the output layer: group 0 of A's two previous rows, group 1 of the previous one
     float q0a[3] = {0.f, 0.f, 0.f}, q0b[3] = {0.f, 0.f, 0.f}, q1a[3] = {0.f, 0.f, 0.f};
     bf16x8 prev[2];
+    // Every row issues the same global loads and stores, dead ones through
+    // buffer offsets beyond their resource (reads give zeros, writes are
+    // dropped; a missing pool or output buffer is a resource of size 0), so
+    // the wait counts before each staged row only wait for that row's loads.
+    const unsigned y_img = unsigned(H * W * 32), p_img = unsigned((H / 2) * (W / 2) * 32), f_img = unsigned(H * W * 3);
+    const void* any = x;  // the base of a resource of size 0
+    const __amdgpu_buffer_rsrc_t ry = buffer_rsrc(y ? static_cast<const void*>(y + size_t(n) * y_img) : any, y ? y_img * 2u : 0u);
+    const __amdgpu_buffer_rsrc_t rp =
+        buffer_rsrc(ypool ? static_cast<const void*>(ypool + size_t(n) * p_img) : any, ypool ? p_img * 2u : 0u);
+    const __amdgpu_buffer_rsrc_t rbr =
+        buffer_rsrc(rbase ? static_cast<const void*>(rbase + size_t(n) * f_img) : any, rbase ? f_img * 4u : 0u);
+    const __amdgpu_buffer_rsrc_t rrgb =
+        buffer_rsrc(rgb ? static_cast<const void*>(rgb + size_t(n) * f_img) : any, rgb ? f_img * 4u : 0u);
 
-    // B's output row yy (lane-transposed through s_x, as the strip kernel stores)
-    auto store_b = [&](int yy, const f32x16& acc) __attribute__((always_inline)) {
+    // B's output row yy (lane-transposed through s_x, as the strip kernel
+    // stores), dropped where `valid` is false
+    auto store_b = [&](int yy, const f32x16& acc, bool valid) __attribute__((always_inline)) {
         __asm__ volatile("" ::: "memory");
         bf16x4 b[4];
         act16_bf16(acc, s_bb, kh, relu_b, neg_b, b);
 #pragma unroll
         for (int i = 0; i < 4; i++) *reinterpret_cast<bf16x4*>(s_x + m * OXP + 8 * i + 4 * kh) = b[i];
         __asm__ volatile("" ::: "memory");
-        const size_t o = (size_t(n) * H + yy) * W;
+        const bool odd = (yy & 1) != 0;
 #pragma unroll
         for (int r = 0; r < 2; r++) {
             const int p = r * 16 + (l >> 2), cg = l & 3;
             const bf16x8 v = *reinterpret_cast<const bf16x8*>(s_x + p * OXP + cg * 8);
-            const bool keep = p < 30 && xs + p < W;
-            if (keep) *reinterpret_cast<bf16x8*>(y + (o + xs + p) * 32 + cg * 8) = v;
-            if (ypool) {
-                if ((yy & 1) == 0) {
-                    prev[r] = v;
-                } else {
-                    const u32x4 pu = __builtin_bit_cast(u32x4, prev[r]), vu = __builtin_bit_cast(u32x4, v);
-                    u32x4 nu, nv;
+            const bool keep = valid && p < 30 && xs + p < W;
+            buffer_store8(v, ry, keep ? unsigned(((yy * W + xs + p) * 32 + cg * 8) * 2) : kBufferOOB);
+            // the pool (computed on odd rows, a branch without global memory
+            // operations; its store issued on every row)
+            bf16x8 pool = v;
+            if (odd) {
+                const u32x4 pu = __builtin_bit_cast(u32x4, prev[r]), vu = __builtin_bit_cast(u32x4, v);
+                u32x4 nu, nv;
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        nu[j] = unsigned(__shfl_down(int(pu[j]), 4));
-                        nv[j] = unsigned(__shfl_down(int(vu[j]), 4));
-                    }
-                    const bf16x8 b = __builtin_bit_cast(bf16x8, nu), d = __builtin_bit_cast(bf16x8, nv);
-                    bf16x8 pool;
-#pragma unroll
-                    for (int j = 0; j < 8; j++)
-                        pool[j] = bf16(0.25f * (float(prev[r][j]) + float(b[j]) + float(v[j]) + float(d[j])));
-                    if (keep && (p & 1) == 0) {
-                        const size_t po = (size_t(n) * (H / 2) + yy / 2) * (W / 2) + (xs + p) / 2;
-                        *reinterpret_cast<bf16x8*>(ypool + po * 32 + cg * 8) = pool;
-                    }
+                for (int j = 0; j < 4; j++) {
+                    nu[j] = unsigned(__shfl_down(int(pu[j]), 4));
+                    nv[j] = unsigned(__shfl_down(int(vu[j]), 4));
                 }
+                const bf16x8 bq = __builtin_bit_cast(bf16x8, nu), d = __builtin_bit_cast(bf16x8, nv);
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    pool[j] = bf16(0.25f * (float(prev[r][j]) + float(bq[j]) + float(v[j]) + float(d[j])));
+            } else {
+                prev[r] = v;
             }
+            const bool keep_pool = keep && odd && (p & 1) == 0;
+            buffer_store8(pool, rp,
+                          keep_pool ? unsigned((((yy >> 1) * (W >> 1) + ((xs + p) >> 1)) * 32 + cg * 8) * 2)
+                                    : kBufferOOB);
         }
         __asm__ volatile("" ::: "memory");
     };
 
+    // the output layer's pixel of input row jj's finished output row (image
+    // row y0 + jj - 4): the byte offset of its three floats in base and rgb,
+    // kBufferOOB where the lane has none
+    auto out_off = [&](int jj) __attribute__((always_inline)) {
+        const int oy = y0 + jj - 4, px = xs + m;
+        const bool ok = BKIND == 1 && jj < NJ && jj >= 4 && oy < H && kh == 0 && m < 30 && px < W;
+        return ok ? unsigned((oy * W + px) * 3) * 4u : kBufferOOB;
+    };
+    // its base pixels, fetched three rows before they are used
+    float rba[3] = {0.f, 0.f, 0.f}, rbb[3] = {0.f, 0.f, 0.f}, rbc[3] = {0.f, 0.f, 0.f};
+    auto load_base = [&](int jj, float (&v)[3]) __attribute__((always_inline)) {
+        const unsigned o = out_off(jj);
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rbr, o + 4u * c, 0, 0));
+    };
+    if constexpr (BKIND == 1) {
+        load_base(0, rba);
+        load_base(1, rbb);
+        load_base(2, rbc);
+    }
     // input row jj: A's accumulators hold A rows jj, jj - 1, jj - 2 (aA, aB,
     // aC); B's hold B rows jj - 2, jj - 3, jj - 4 (bA, bB, bC), in the
     // relative numbering where A row q is image row y0 - 1 + q and B row o
@@ -1171,17 +1205,13 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
             else if constexpr (R == 1) return pb;
             else return pc;
         }();
-        // the output layer's base pixels of this row's output, fetched now
-        // and used after the row's MFMAs
-        const int oy = y0 + jj - 4, px = xs + m;
-        const bool out_px = BKIND == 1 && jj >= 4 && oy < H && kh == 0 && m < 30 && px < W;
-        float rb[3] = {0.f, 0.f, 0.f};
-        if (out_px) {
-            const size_t qo = ((size_t(n) * H + oy) * W + px) * 3;
-            rb[0] = rbase[qo + 0];
-            rb[1] = rbase[qo + 1];
-            rb[2] = rbase[qo + 2];
-        }
+        auto& rb = [&]() __attribute__((always_inline)) -> float (&)[3] {
+            if constexpr (R == 0) return rba;
+            else if constexpr (R == 1) return rbb;
+            else return rbc;
+        }();
+        const bool live = jj < NJ;  // rows past the strip's run as dead ones
+        const int oy = y0 + jj - 4;
         __asm__ volatile("" ::: "memory");
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
@@ -1189,9 +1219,9 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
             if (col < SPX) *reinterpret_cast<bf16x8*>(s_x + col * XP + g * 8) = buf[q];
         }
         __asm__ volatile("" ::: "memory");
-        if (jj + 3 < NJ) load_row(jj + 3, buf);
+        load_row(jj + 3, buf, jj + 3 < NJ);
         const int gy = y0 - 2 + jj;
-        if (gy >= 0 && gy < H) {
+        if (live && gy >= 0 && gy < H) {
             const bf16* sx = s_x + m * XP + kh * 8;
             const bf16* sw = s_wa + m * 9 * WPA + kh * 8;
             auto rxa = [&](int i) __attribute__((always_inline)) {
@@ -1220,7 +1250,8 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                 xc = xn2;
             }
         }
-        if (jj >= 2) {
+        float vout[3] = {0.f, 0.f, 0.f};
+        if (live && jj >= 2) {
             // A row q = jj - 2 (image row y0 - 3 + jj) is complete: activation,
             // BF16, zero outside the image, into s_m for B
             const int qy = y0 - 3 + jj;
@@ -1265,10 +1296,6 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                         xc = xn2;
                     }
                 }
-                // B row o = jj - 4 (image row oy = y0 + jj - 4) is complete
-                if (jj >= 4 && oy < H) store_b(oy, bC);
-#pragma unroll
-                for (int i = 0; i < 16; i++) bC[i] = 0.f;
             } else {
                 // the stacked output layer: one accumulator per dx
                 f32x16 acc[3];
@@ -1295,13 +1322,10 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                     g2[c] = (acc[0][8 + c] + acc[1][8 + c]) + acc[2][8 + c];
                 }
                 // B's output row image qy - 1 = y0 + jj - 4 is complete
-                if (out_px) {
-                    const size_t qo = ((size_t(n) * H + oy) * W + px) * 3;
-                    const float v0 = q0b[0] + q1a[0] + g2[0], v1 = q0b[1] + q1a[1] + g2[1],
-                                v2 = q0b[2] + q1a[2] + g2[2];
-                    rgb[qo + 0] = fminf(fmaxf(rb[0] + act_fwd(v0 + s_bb[0], act_b, 0.f), 0.f), 1.f);
-                    rgb[qo + 1] = fminf(fmaxf(rb[1] + act_fwd(v1 + s_bb[1], act_b, 0.f), 0.f), 1.f);
-                    rgb[qo + 2] = fminf(fmaxf(rb[2] + act_fwd(v2 + s_bb[2], act_b, 0.f), 0.f), 1.f);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const float v = q0b[c] + q1a[c] + g2[c];
+                    vout[c] = fminf(fmaxf(rb[c] + act_fwd(v + s_bb[c], act_b, 0.f), 0.f), 1.f);
                 }
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
@@ -1310,6 +1334,19 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                     q1a[c] = g1[c];
                 }
             }
+        }
+        if constexpr (BKIND == 0) {
+            // B row o = jj - 4 (image row oy = y0 + jj - 4) is complete
+            store_b(oy, bC, live && jj >= 4 && oy < H);
+#pragma unroll
+            for (int i = 0; i < 16; i++) bC[i] = 0.f;
+        } else {
+            const unsigned qo = out_off(jj);
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, vout[c]), rrgb, qo + 4u * c, 0, 0);
+            // this set's next row, three rows on
+            load_base(jj + 3, rb);
         }
         // A row jj - 2 is done (or, before row 2, outside the strip): aC
         // starts over as row jj + 1
@@ -1320,7 +1357,6 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
         static_for<0, 3>([&](auto rc) __attribute__((always_inline)) {
             constexpr int R = decltype(rc)::value;
             const int jj = jb + R;
-            if (jj >= NJ) return;
             if constexpr (R == 0) row(rc, jj, a0, a2, a1, b0, b2, b1);
             else if constexpr (R == 1) row(rc, jj, a1, a0, a2, b1, b0, b2);
             else row(rc, jj, a2, a1, a0, b2, b1, b0);
