@@ -916,6 +916,9 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
         constexpr int R = decltype(rc)::value;
         const int gy = y0 - 1 + jj;
         const bool in_img = jj < rows + 2 && gy >= 0 && gy < H;
+        // the halo rows feed only some of the three output rows: the MFMAs
+        // of an accumulator whose row lies outside the strip are skipped
+        const bool uA = jj < rows, uB = jj >= 1 && jj <= rows, uC = jj >= 2;
         static_for<0, NCH>([&](auto chc) __attribute__((always_inline)) {
             constexpr int CH = decltype(chc)::value, S = R * NCH + CH;
             auto& buf = [&]() __attribute__((always_inline)) -> bf16x8 (&)[NQ] {
@@ -957,15 +960,15 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
                 const bool more = i + 1 < NS;
                 const bf16x8 xn = more ? rx(i + 1) : xc;
                 __builtin_amdgcn_sched_barrier(0);
-                aA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, xc, aA, 0, 0, 0);
+                if (uA) aA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, xc, aA, 0, 0, 0);
                 __builtin_amdgcn_sched_barrier(0);
                 if (more) w0 = rw(i + 1, 0);
                 __builtin_amdgcn_sched_barrier(0);
-                aB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, xc, aB, 0, 0, 0);
+                if (uB) aB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, xc, aB, 0, 0, 0);
                 __builtin_amdgcn_sched_barrier(0);
                 if (more) w1 = rw(i + 1, 1);
                 __builtin_amdgcn_sched_barrier(0);
-                aC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2, xc, aC, 0, 0, 0);
+                if (uC) aC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2, xc, aC, 0, 0, 0);
                 __builtin_amdgcn_sched_barrier(0);
                 if (more) w2 = rw(i + 1, 2);
                 xc = xn;
@@ -1222,6 +1225,8 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
         load_row(jj + 3, buf, jj + 3 < NJ);
         const int gy = y0 - 2 + jj;
         if (live && gy >= 0 && gy < H) {
+            // A rows outside 0 .. rows + 1 feed no B row of the strip
+            const bool uA = jj < rows + 2, uB = jj >= 1 && jj <= rows + 2, uC = jj >= 2;
             const bf16* sx = s_x + m * XP + kh * 8;
             const bf16* sw = s_wa + m * 9 * WPA + kh * 8;
             auto rxa = [&](int i) __attribute__((always_inline)) {
@@ -1236,15 +1241,15 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                 const bool more = i + 1 < NSA;
                 const bf16x8 xn2 = more ? rxa(i + 1) : xc;
                 __builtin_amdgcn_sched_barrier(0);
-                aA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, xc, aA, 0, 0, 0);
+                if (uA) aA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, xc, aA, 0, 0, 0);
                 __builtin_amdgcn_sched_barrier(0);
                 if (more) w0 = rwa(i + 1, 0);
                 __builtin_amdgcn_sched_barrier(0);
-                aB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, xc, aB, 0, 0, 0);
+                if (uB) aB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, xc, aB, 0, 0, 0);
                 __builtin_amdgcn_sched_barrier(0);
                 if (more) w1 = rwa(i + 1, 1);
                 __builtin_amdgcn_sched_barrier(0);
-                aC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2, xc, aC, 0, 0, 0);
+                if (uC) aC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2, xc, aC, 0, 0, 0);
                 __builtin_amdgcn_sched_barrier(0);
                 if (more) w2 = rwa(i + 1, 2);
                 xc = xn2;
@@ -1272,6 +1277,8 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
             };
             if constexpr (BKIND == 0) {
                 if (q_in) {
+                    // B rows outside the strip are skipped
+                    const bool uA = jj < rows + 2, uB = jj >= 3 && jj < rows + 3, uC = jj >= 4;
                     const bf16* sw = s_wb + m * 9 * MP + kh * 8;
                     auto rwb = [&](int i, int dy) __attribute__((always_inline)) {
                         return *reinterpret_cast<const bf16x8*>(sw + (3 * dy + i / 2) * MP + (i % 2) * 16);
@@ -1282,15 +1289,15 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                         const bool more = i + 1 < 6;
                         const bf16x8 xn2 = more ? rxb(i + 1) : xc;
                         __builtin_amdgcn_sched_barrier(0);
-                        bA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, xc, bA, 0, 0, 0);
+                        if (uA) bA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, xc, bA, 0, 0, 0);
                         __builtin_amdgcn_sched_barrier(0);
                         if (more) w0 = rwb(i + 1, 0);
                         __builtin_amdgcn_sched_barrier(0);
-                        bB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, xc, bB, 0, 0, 0);
+                        if (uB) bB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, xc, bB, 0, 0, 0);
                         __builtin_amdgcn_sched_barrier(0);
                         if (more) w1 = rwb(i + 1, 1);
                         __builtin_amdgcn_sched_barrier(0);
-                        bC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2, xc, bC, 0, 0, 0);
+                        if (uC) bC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2, xc, bC, 0, 0, 0);
                         __builtin_amdgcn_sched_barrier(0);
                         if (more) w2 = rwb(i + 1, 2);
                         xc = xn2;
