@@ -667,6 +667,28 @@ __device__ __forceinline__ void stage_fragments(int t, Src&& src, Dst&& dst) {
     }
 }
 
+// The column of a staged pixel row that lane l fills in round q, G lanes per
+// pixel (one 16-byte channel group each) and 64 / G columns per round. LDS
+// serves a wave's 16-byte accesses sixteen lanes at a time; through rows of
+// 40 BF16 (G = 4) lanes 0-15 in column order would take columns 0-3, whose
+// bank ranges overlap, so consecutive lane quads take columns four apart
+// (0, 4, 8, 12 | 1, 5, 9, 13 | ...) and every group covers all banks once;
+// with G = 2 (rows of 24 BF16) the groups take every other column. Measured
+// on one MI355X, 16-byte writes in this order run at the rate of linear ones,
+// 1.2x faster than in column order, and reads 1.4x faster.
+template <int G>
+__device__ __forceinline__ int stage_col(int q, int l) {
+    const int k = l / G;
+    if constexpr (G == 4) {
+        return 16 * q + 4 * (k & 3) + (k >> 2);
+    } else if constexpr (G == 2) {
+        const int j = k >> 3;
+        return 32 * q + (j >> 1) * 16 + (j & 1) + 2 * (k & 7);
+    } else {
+        return (l + 64 * q) / G;
+    }
+}
+
 // Every wave walks its own 32-pixel column strip down `rows` output rows for
 // 32 output channels (grid.z = images x 32-channel tiles), input-stationary:
 // each staged input row (one KC-channel chunk at a time, NCH chunks) feeds
@@ -734,7 +756,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
     bool colok[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
-        const int col = (l + 64 * q) / G, gx = x0 - 1 + col;
+        const int col = stage_col<G>(q, l), gx = x0 - 1 + col;
         colok[q] = col < SPX && gx >= 0 && gx < W;
         o1c[q] = unsigned((MODE == 1 ? (gx >> 1) : gx) * c1 + g * 8) * 2u;
         o2c[q] = unsigned(gx * c2 + g * 8 - c1) * 2u;  // x2's channel is c - c1
@@ -749,7 +771,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
         if (MODE == 2) {
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
-                const int col = (l + 64 * q) / G;
+                const int col = stage_col<G>(q, l);
                 v[q] = load_px8<2>(x1n, c1, x2n, c2, H, W, gy, x0 - 1 + col, CB + g * 8, row_ok && colok[q] && ch_ok);
             }
             return;
@@ -794,7 +816,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
         const bool odd = (yy & 1) != 0;
 #pragma unroll
         for (int r = 0; r < 2; r++) {
-            const int p = r * 16 + (l >> 2), cg = l & 3;
+            const int p = stage_col<4>(r, l), cg = l & 3;
             const bf16x8 v = *reinterpret_cast<const bf16x8*>(s_x + p * OXP + cg * 8);
             const bool ok = valid && x0 + p < W;
             buffer_store8(v, ry, ok ? unsigned(((yy * W + x0 + p) * Cout + co0 + cg * 8) * 2) : kBufferOOB);
@@ -804,13 +826,13 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
                 // issued on every row
                 bf16x8 pool = v;
                 if (odd) {
-                    // pixel p + 1 of both rows sits four lanes up
+                    // pixel p + 1 of both rows sits sixteen lanes up
                     const u32x4 pa = __builtin_bit_cast(u32x4, prev[r]), pv = __builtin_bit_cast(u32x4, v);
                     u32x4 na, nv;
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
-                        na[j] = unsigned(__shfl_down(int(pa[j]), 4));
-                        nv[j] = unsigned(__shfl_down(int(pv[j]), 4));
+                        na[j] = unsigned(__shfl_down(int(pa[j]), 16));
+                        nv[j] = unsigned(__shfl_down(int(pv[j]), 16));
                     }
                     const bf16x8 bq = __builtin_bit_cast(bf16x8, na), d = __builtin_bit_cast(bf16x8, nv);
 #pragma unroll
@@ -931,7 +953,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
             __asm__ volatile("" ::: "memory");
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
-                const int col = (l + 64 * q) / G;
+                const int col = stage_col<G>(q, l);
                 if (col < SPX) *reinterpret_cast<bf16x8*>(s_x + col * XP + g * 8) = buf[q];
             }
             __asm__ volatile("" ::: "memory");
@@ -1053,7 +1075,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
     bool colok[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
-        const int col = (l + 64 * q) / G, gx = xs - 2 + col;
+        const int col = stage_col<G>(q, l), gx = xs - 2 + col;
         colok[q] = col < SPX && gx >= 0 && gx < W;
         oc[q] = unsigned(gx * Cin + g * 8) * 2u;
     }
@@ -1146,7 +1168,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
         const bool odd = (yy & 1) != 0;
 #pragma unroll
         for (int r = 0; r < 2; r++) {
-            const int p = r * 16 + (l >> 2), cg = l & 3;
+            const int p = stage_col<4>(r, l), cg = l & 3;
             const bf16x8 v = *reinterpret_cast<const bf16x8*>(s_x + p * OXP + cg * 8);
             const bool keep = valid && p < 30 && xs + p < W;
             buffer_store8(v, ry, keep ? unsigned(((yy * W + xs + p) * 32 + cg * 8) * 2) : kBufferOOB);
@@ -1158,8 +1180,8 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                 u32x4 nu, nv;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    nu[j] = unsigned(__shfl_down(int(pu[j]), 4));
-                    nv[j] = unsigned(__shfl_down(int(vu[j]), 4));
+                    nu[j] = unsigned(__shfl_down(int(pu[j]), 16));
+                    nv[j] = unsigned(__shfl_down(int(vu[j]), 16));
                 }
                 const bf16x8 bq = __builtin_bit_cast(bf16x8, nu), d = __builtin_bit_cast(bf16x8, nv);
 #pragma unroll
@@ -1218,7 +1240,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
         __asm__ volatile("" ::: "memory");
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
-            const int col = (l + 64 * q) / G;
+            const int col = stage_col<G>(q, l);
             if (col < SPX) *reinterpret_cast<bf16x8*>(s_x + col * XP + g * 8) = buf[q];
         }
         __asm__ volatile("" ::: "memory");
