@@ -59,7 +59,7 @@ struct rfk_state {
     uint32_t* hist = nullptr;
     unsigned long long* counters = nullptr;        // device [8]
     unsigned long long* counters_host = nullptr;   // pinned [8 + 98]
-    unsigned long long* dstats = nullptr;          // device [97 + 1]
+    unsigned long long* dstats = nullptr;          // device [97 + 1], = counters + 8 (one allocation)
     uint32_t* stop_host = nullptr;                 // pinned, mapped
     uint32_t* stop_dev = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -170,7 +170,7 @@ void free_state_buffers(rfk_state* s) {
     if (s->ev_staging) (void)hipEventSynchronize(s->ev_staging);
     if (s->staging) (void)hipHostFree(s->staging);
     if (s->ev_staging) (void)hipEventDestroy(s->ev_staging);
-    f(s->counters); f(s->dstats);
+    f(s->counters);  // dstats lies inside it
     if (s->counters_host) (void)hipHostFree(s->counters_host);
     if (s->stop_host) (void)hipHostFree(s->stop_host);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -262,12 +262,12 @@ int bin_queued(rfk_state* s, rfk_iter_args& a, uint64_t passes, bool bounded) {
     return RFK_OK;
 }
 
-// zero the counters, the running minimum start clock at ~0 (fill kernels: no
-// host buffer is read after this returns)
+// zero the counters and the density statistics behind them, the running
+// minimum start clock at ~0 (fill kernels: no host buffer is read after this
+// returns)
 int reset_counters(rfk_state* s) {
     uint32_t* c = reinterpret_cast<uint32_t*>(s->counters);
-    RFK_HIP(rfk::sk::fill_u32(c, 0u, 16, s->stream));
-    RFK_HIP(rfk::sk::fill_u32(c + 2 * RFK_CNT_START, 0xFFFFFFFFu, 2, s->stream));
+    RFK_HIP(rfk::sk::fill_u32_except(c, 0u, 2 * (8 + 98), 2 * RFK_CNT_START, 2, 0xFFFFFFFFu, s->stream));
     return RFK_OK;
 }
 
@@ -670,8 +670,10 @@ int rfk_warmup_ex(rfk_kernel* k, void* stream, const double* samples, size_t n_r
         RFK_HIPC(dalloc(&s->qtiles, size_t(ntiles) * 2));
         RFK_HIPC(dalloc(&s->qnseg, 4));
     }
-    RFK_HIPC(dalloc(&s->counters, 8));
-    RFK_HIPC(dalloc(&s->dstats, 98));
+    // the call's counters and its density statistics in one allocation, so
+    // that one copy brings both to the host
+    RFK_HIPC(dalloc(&s->counters, 8 + 98));
+    s->dstats = s->counters + 8;
     RFK_HIPC(hipHostMalloc(reinterpret_cast<void**>(&s->counters_host), 128 * sizeof(unsigned long long),
                            hipHostMallocDefault));
     RFK_HIPC(hipHostMalloc(reinterpret_cast<void**>(&s->stop_host), 64, hipHostMallocMapped | hipHostMallocCoherent));
@@ -737,14 +739,13 @@ int rfk_bin_async(rfk_kernel* k, rfk_state* s, const rfk_bailout* bo) {
         return rc2;
     }
     RFK_HIP(hipEventRecord(s->ev1, s->stream));
-    // density statistics after every bin, as the reference does (flame_compiler.cpp:928-937)
-    RFK_HIP(rfk::sk::fill_u32(reinterpret_cast<uint32_t*>(s->dstats), 0u, 2 * 98, s->stream));
+    // density statistics after every bin, as the reference does
+    // (flame_compiler.cpp:928-937), into the bins reset_counters cleared;
+    // counters and statistics to the host in one copy
     RFK_HIP(rfk::sk::density_histogram(s->hist, s->hist64, size_t(s->width) * s->height, s->dstats, s->dstats + 97,
                                        s->stream));
-    RFK_HIP(hipMemcpyAsync(s->counters_host, s->counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                           s->stream));
-    RFK_HIP(hipMemcpyAsync(s->counters_host + 8, s->dstats, 98 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                           s->stream));
+    RFK_HIP(hipMemcpyAsync(s->counters_host, s->counters, (8 + 98) * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s->stream));
     RFK_HIP(hipEventRecord(s->ev_result, s->stream));
     s->pending = true;
     s->pending_passes = passes;

@@ -144,10 +144,11 @@ __device__ __forceinline__ float4 bin_units(const bin64& h) {
 }
 
 // ------------------------------------------------------- density histogram
+constexpr unsigned kHistoBlock = 1024;
 template <class BinT>
-__global__ void density_histogram_kernel(const BinT* __restrict__ hist, size_t npix,
-                                         unsigned long long* __restrict__ bins97,
-                                         unsigned long long* __restrict__ max_density) {
+__global__ __launch_bounds__(kHistoBlock) void density_histogram_kernel(const BinT* __restrict__ hist, size_t npix,
+                                                                        unsigned long long* __restrict__ bins97,
+                                                                        unsigned long long* __restrict__ max_density) {
     constexpr unsigned kBins = 4 * 24 + 1;
     // a workgroup counts fewer than 2^32 pixels: 32-bit LDS counters, added
     // into the 64-bit global bins once
@@ -161,15 +162,28 @@ __global__ void density_histogram_kernel(const BinT* __restrict__ hist, size_t n
     // are kept per thread and added once: one LDS address for every lane
     // would serialise the atomics
     unsigned long long low = 0, mx = 0;
-    for (size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x; g < npix; g += size_t(gridDim.x) * blockDim.x) {
-        const float value = bin_units(hist[g]).w;
-        if (value >= 1.0f) {
-            unsigned long long b = (unsigned long long)(logf(value) * log_div) + 1ull;
-            if (b >= kBins) b = kBins - 1;
-            atomicAdd(&s_bins[b], 1u);
-            mx = max(mx, (unsigned long long)value);
-        } else {
-            low++;
+    // four pixels' loads in flight per thread before any is used
+    constexpr unsigned U = 4;
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x; g < npix; g += U * stride) {
+        float v[U];
+#pragma unroll
+        for (unsigned u = 0; u < U; u++) {
+            const size_t gi = g + u * stride;
+            v[u] = gi < npix ? bin_units(hist[gi]).w : 0.0f;
+        }
+#pragma unroll
+        for (unsigned u = 0; u < U; u++) {
+            if (g + u * stride >= npix) break;
+            const float value = v[u];
+            if (value >= 1.0f) {
+                unsigned long long b = (unsigned long long)(logf(value) * log_div) + 1ull;
+                if (b >= kBins) b = kBins - 1;
+                atomicAdd(&s_bins[b], 1u);
+                mx = max(mx, (unsigned long long)value);
+            } else {
+                low++;
+            }
         }
     }
     if (low) atomicAdd(&s_bins[0], unsigned(low));
@@ -523,14 +537,18 @@ hipError_t eval_samples(const double* segments, uint32_t sample_size, uint32_t s
     return hipGetLastError();
 }
 
-__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, size_t n) {
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+// p[i] = v, except w for i in [at, at + m)
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, size_t n, size_t at, size_t m, uint32_t w) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        p[i] = (i - at < m) ? w : v;
 }
 
-hipError_t fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
+hipError_t fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) { return fill_u32_except(p, v, n, 0, 0, v, s); }
+
+hipError_t fill_u32_except(uint32_t* p, uint32_t v, size_t n, size_t at, size_t m, uint32_t w, hipStream_t s) {
     if (!n) return hipSuccess;
     unsigned g = std::min(nblocks(n), 8192u);
-    fill_u32_kernel<<<g, kBlock, 0, s>>>(p, v, n);
+    fill_u32_kernel<<<g, kBlock, 0, s>>>(p, v, n, at, m, w);
     return hipGetLastError();
 }
 
@@ -548,17 +566,17 @@ hipError_t bin_points(const float* screen6, const uint32_t* palette, const float
 
 hipError_t density_histogram(const uint32_t* hist, bool hist64, size_t npix, unsigned long long* bins97,
                              unsigned long long* max_density, hipStream_t s) {
-    // one workgroup per CU: every workgroup ends with one global atomic per
-    // non-empty bin, on the same 98 addresses (2048 workgroups: 33.5 us at
-    // 1080p, 256: 19.6 us)
-    unsigned grid = std::min(nblocks(npix), 256u);
+    // one workgroup per CU (every workgroup ends with one global atomic per
+    // non-empty bin, on the same 98 addresses: 2048 workgroups of 256 took
+    // 33.5 us at 1080p, 256 of them 19.6 us), 16 waves each
+    unsigned grid = std::min(nblocks(npix, kHistoBlock), 256u);
     if (grid == 0) grid = 1;
     if (hist64)
-        density_histogram_kernel<<<grid, kBlock, 0, s>>>(reinterpret_cast<const bin64*>(hist), npix, bins97,
-                                                         max_density);
+        density_histogram_kernel<<<grid, kHistoBlock, 0, s>>>(reinterpret_cast<const bin64*>(hist), npix, bins97,
+                                                              max_density);
     else
-        density_histogram_kernel<<<grid, kBlock, 0, s>>>(reinterpret_cast<const uint4*>(hist), npix, bins97,
-                                                         max_density);
+        density_histogram_kernel<<<grid, kHistoBlock, 0, s>>>(reinterpret_cast<const uint4*>(hist), npix, bins97,
+                                                              max_density);
     return hipGetLastError();
 }
 

@@ -19,8 +19,10 @@ hipError_t eval_samples(const double* segments, uint32_t sample_size, uint32_t s
                         uint32_t nsamples, const int8_t* kind, int precision, void* params, uint32_t param_stride,
                         uint32_t* palettes, hipStream_t s, const double* times = nullptr);
 
-// graph-safe clear / fill (a kernel instead of hipMemsetAsync)
+// graph-safe clear / fill (a kernel instead of hipMemsetAsync); the second
+// form writes w instead of v into the m words from index at
 hipError_t fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+hipError_t fill_u32_except(uint32_t* p, uint32_t v, size_t n, size_t at, size_t m, uint32_t w, hipStream_t s);
 
 // Bin n plane-space points through a screen transform / palette.
 // (hist64: the histogram holds u64 RGBA bins, else u32 RGBA)
