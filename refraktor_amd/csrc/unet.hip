@@ -977,6 +977,9 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
                 return *reinterpret_cast<const bf16x8*>(sw + (3 * dy + i / KS) * WP + (i % KS) * 16);
             };
             bf16x8 xc = rx(0), w0 = rw(0, 0), w1 = rw(0, 1), w2 = rw(0, 2);
+            // a wave feeding the matrix core issues ahead of the waves
+            // staging or storing (measured 3-6 % off every strip layer)
+            __builtin_amdgcn_s_setprio(1);
 #pragma unroll
             for (int i = 0; i < NS; i++) {
                 const bool more = i + 1 < NS;
@@ -995,6 +998,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
                 if (more) w2 = rw(i + 1, 2);
                 xc = xn;
             }
+            __builtin_amdgcn_s_setprio(0);
         });
         if constexpr (OUT == 2) {
             if (jj >= 2 && y0 + jj - 2 < H) store_row(y0 + jj - 2, aC);
@@ -1258,6 +1262,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                 return *reinterpret_cast<const bf16x8*>(sw + (3 * dy + i / KSA) * WPA + (i % KSA) * 16);
             };
             bf16x8 xc = rxa(0), w0 = rwa(0, 0), w1 = rwa(0, 1), w2 = rwa(0, 2);
+            __builtin_amdgcn_s_setprio(1);  // as in the strip kernel
 #pragma unroll
             for (int i = 0; i < NSA; i++) {
                 const bool more = i + 1 < NSA;
@@ -1276,6 +1281,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                 if (more) w2 = rwa(i + 1, 2);
                 xc = xn2;
             }
+            __builtin_amdgcn_s_setprio(0);
         }
         float vout[3] = {0.f, 0.f, 0.f};
         if (live && jj >= 2) {
@@ -1306,6 +1312,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                         return *reinterpret_cast<const bf16x8*>(sw + (3 * dy + i / 2) * MP + (i % 2) * 16);
                     };
                     bf16x8 xc = rxb(0), w0 = rwb(0, 0), w1 = rwb(0, 1), w2 = rwb(0, 2);
+                    __builtin_amdgcn_s_setprio(1);
 #pragma unroll
                     for (int i = 0; i < 6; i++) {
                         const bool more = i + 1 < 6;
@@ -1324,6 +1331,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                         if (more) w2 = rwb(i + 1, 2);
                         xc = xn2;
                     }
+                    __builtin_amdgcn_s_setprio(0);
                 }
             } else {
                 // the stacked output layer: one accumulator per dx
@@ -1334,6 +1342,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                     for (int i = 0; i < 16; i++) acc[d][i] = 0.f;
                 if (q_in) {
                     const bf16* sw = s_wb + m * 3 * MP + kh * 8;
+                    __builtin_amdgcn_s_setprio(1);
 #pragma unroll
                     for (int dx = 0; dx < 3; dx++)
 #pragma unroll
@@ -1342,6 +1351,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
                             const bf16x8 wv8 = *reinterpret_cast<const bf16x8*>(sw + dx * MP + ks * 16);
                             acc[dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv8, xb, acc[dx], 0, 0, 0);
                         }
+                    __builtin_amdgcn_s_setprio(0);
                 }
                 float g0[3], g1[3], g2[3];
 #pragma unroll
