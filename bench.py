@@ -98,8 +98,8 @@ def unet_roofline(lane, h: int, w: int, reps: int = 10) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tpb", type=int, default=512, help="plain path walkers per workgroup (without autotuning)")
     ap.add_argument("--den-tpb", type=int, default=512, help="denoised path walkers per workgroup (without autotuning)")
