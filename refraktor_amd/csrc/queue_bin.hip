@@ -7,7 +7,7 @@
 //   1. qscan_col  per tile: exclusive prefix of its counts over the blocks
 //                 (in place) and the tile's total
 //   2. qseg       exclusive scan of the tile totals -> tile starts, and the
-//                 (tile, segment of <= 2^18 records) work items of step 4
+//                 (tile, segment of <= 2^17 or 2^18 records) work items of step 4
 //   3. qscatter   counting sort by tile: each block's records, 4096 at a time,
 //                 ranked in LDS and written as tile runs at their offsets,
 //                 colour (u32) and pixel in tile (u16) in separate arrays
@@ -43,7 +43,16 @@ constexpr uint32_t kThreads = 1024;
 // several sorting workgroups share a CU and one's barriers overlap another's
 // memory traffic (2.5 % off a 2^30-iteration bin call against 1024 / 8192)
 constexpr uint32_t kScatterThreads = 512, kScatterPiece = 4096;
-constexpr uint32_t kSegment = 1u << 18;  // records per bin work item (u32 LDS sums cannot wrap)
+// records per bin work item (u32 LDS sums cannot wrap): 2^18 for chunks
+// of more than 2^26 records, 2^17 below, where the hottest tiles' work
+// items would otherwise leave most CUs idle at the end (a 2^25-record
+// denoised frame's tile sums 107 -> 90 us; 2^17 at 2^27-record chunks
+// costs more in shared-tile atomics than it balances: -0.8 % plain frames)
+constexpr uint32_t kSegmentLog2Max = 18, kSegmentLog2Min = 17;
+constexpr uint64_t kSmallChunk = 1ull << 26;
+__host__ __device__ constexpr uint32_t segment_log2(uint64_t records) {
+    return records <= kSmallChunk ? kSegmentLog2Min : kSegmentLog2Max;
+}
 // copies of a work item's LDS tile sums: lane l adds into copy l % kCopies,
 // so the lanes of one wave that draw into the same pixel (a flame's hottest
 // pixels take 10-13 % of their tile's records) split over kCopies addresses
@@ -119,20 +128,23 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // One workgroup: tile_start = exclusive scan of tile_total; work items
-// (tile, begin, end, exclusive), nseg[0] = their count. The full 2^18-record
-// items come first and the tiles' shorter remainders after them, so the
-// workgroups that start last are the short ones (longest first).
+// (tile, begin, end, exclusive) of at most 2^seg_log2 records, nseg[0] =
+// their count. The full items come first and the tiles' shorter remainders
+// after them, so the workgroups that start last are the short ones (longest
+// first).
 __global__ void __launch_bounds__(kThreads)
     qseg_kernel(const uint32_t* __restrict__ tile_total, uint32_t ntiles, uint32_t* __restrict__ tile_start,
-                uint4* __restrict__ segments, uint32_t max_segments, uint32_t* __restrict__ nseg) {
+                uint4* __restrict__ segments, uint32_t max_segments, uint32_t* __restrict__ nseg,
+                uint32_t seg_log2) {
     __shared__ uint32_t s_wave[kThreads / 64];
+    const uint32_t seg = 1u << seg_log2;
     const uint32_t per = (ntiles + kThreads - 1) / kThreads;
     const uint32_t lo = threadIdx.x * per, hi = std::min(ntiles, lo + per);
     uint32_t sum = 0, nfull = 0, nrem = 0;
     for (uint32_t t = lo; t < hi; t++) {
         sum += tile_total[t];
-        nfull += tile_total[t] / kSegment;
-        nrem += (tile_total[t] % kSegment) != 0u;
+        nfull += tile_total[t] / seg;
+        nrem += (tile_total[t] % seg) != 0u;
     }
     uint32_t run = block_exclusive_scan(sum, s_wave, nullptr);
     uint32_t full_all = 0, rem_all = 0;
@@ -141,11 +153,11 @@ __global__ void __launch_bounds__(kThreads)
     for (uint32_t t = lo; t < hi; t++) {
         const uint32_t n = tile_total[t];
         tile_start[t] = run;
-        const uint32_t nf = n / kSegment, exclusive = (n <= kSegment) ? 1u : 0u;
+        const uint32_t nf = n / seg, exclusive = (n <= seg) ? 1u : 0u;
         for (uint32_t s = 0; s < nf; s++, kf++)
-            if (kf < max_segments) segments[kf] = make_uint4(t, run + s * kSegment, run + (s + 1) * kSegment, exclusive);
-        if (n % kSegment) {
-            if (kr < max_segments) segments[kr] = make_uint4(t, run + nf * kSegment, run + n, exclusive);
+            if (kf < max_segments) segments[kf] = make_uint4(t, run + s * seg, run + (s + 1) * seg, exclusive);
+        if (n % seg) {
+            if (kr < max_segments) segments[kr] = make_uint4(t, run + nf * seg, run + n, exclusive);
             kr++;
         }
         run += n;
@@ -269,7 +281,7 @@ __global__ void __launch_bounds__(kThreads)
                 uint32_t width, uint32_t height, uint32_t tiles_x, uint32_t tpf,
                 const uint32_t* __restrict__ group_frame, size_t frame_elems, HistT* __restrict__ hist,
                 bool packed64) {
-    static_assert(kSegment <= (1u << 24), "32-bit halves of the LDS sums must not carry");
+    static_assert(kSegmentLog2Max <= 24, "32-bit halves of the LDS sums must not carry");
     extern __shared__ u64 qbin_lds[];
     u64(*copies)[2][kTilePix] = reinterpret_cast<u64(*)[2][kTilePix]>(qbin_lds);
     u64(*acc)[kTilePix] = copies[__lane_id() % kCopies];
@@ -319,8 +331,11 @@ __global__ void __launch_bounds__(kThreads)
 
 uint32_t queue_tiles_x(uint32_t width) { return (width + 63u) >> kTileLog2; }
 uint32_t queue_tiles(uint32_t width, uint32_t height) { return queue_tiles_x(width) * ((height + 63u) >> kTileLog2); }
+// work items of the smallest size, whatever a call of up to `records`
+// records uses
 uint32_t queue_max_segments(size_t records, uint32_t ntiles) {
-    return uint32_t((records + kSegment - 1) / kSegment) + ntiles;
+    constexpr size_t seg = size_t(1) << kSegmentLog2Min;
+    return uint32_t((records + seg - 1) / seg) + ntiles;
 }
 
 hipError_t queue_bin_groups(const unsigned long long* records, size_t per_part, uint32_t nparts, uint32_t ncols,
@@ -337,7 +352,7 @@ hipError_t queue_bin_groups(const unsigned long long* records, size_t per_part, 
     uint32_t* tile_start = tile_scratch + ntiles;
     qscan_col_kernel<<<uint32_t(ntiles), kThreads, 0, s>>>(counts, ncols, tile_total);
     qseg_kernel<<<1, kThreads, 0, s>>>(tile_total, uint32_t(ntiles), tile_start, reinterpret_cast<uint4*>(segments),
-                                       max_segments, nseg);
+                                       max_segments, nseg, segment_log2(uint64_t(per_part) * nparts));
     const size_t lds = size_t(3 * tpf + 1) * sizeof(uint32_t) + kScatterPiece * sizeof(u64) + 8;
     // more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per workgroup)
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&qscatter_kernel<kScatterThreads, kScatterPiece>),
