@@ -308,6 +308,55 @@ RFK_INLINE rfk_draw_t rfk_make_draw(const float* __restrict__ S, const u32* pal,
     return d;
 }
 
+// The palette as the iterate kernels' draws read it (in LDS): entry i's
+// channels as floats and their step to entry i + 1 (0 for the last entry).
+// The small integers and their differences are exact in float, and at a
+// whole index (mix 0) fmaf(0, step, lo) is lo whatever the step, so the draw
+// below gives rfk_make_draw's bits with one entry read and no unpacking.
+struct alignas(16) rfk_pal_lut {
+    float lo[4], step[4];
+};
+
+RFK_INLINE rfk_pal_lut rfk_pal_lut_entry(const u32* pal, u32 i) {
+    const u32 lo = pal[i], hi = pal[i < 255u ? i + 1u : i];
+    rfk_pal_lut e;
+#pragma unroll
+    for (u32 c = 0; c < 3u; c++) {
+        const float l = float((lo >> (8u * c)) & 0xFFu), h = float((hi >> (8u * c)) & 0xFFu);
+        e.lo[c] = l;
+        e.step[c] = h - l;
+    }
+    e.lo[3] = 0.0f;
+    e.step[3] = 0.0f;
+    return e;
+}
+
+RFK_INLINE rfk_draw_t rfk_make_draw(const float* __restrict__ S, const rfk_pal_lut* lut, u32 width, u32 height,
+                                    float x, float y, float z, float opacity) {
+    RFK_EXACT_FP
+    rfk_draw_t d;
+    d.hit = false;
+    float sx = truncf(fmaf(S[0], x, fmaf(S[2], y, S[4])));
+    float sy = truncf(fmaf(S[1], x, fmaf(S[3], y, S[5])));
+    if (sx >= 0.0f && sy >= 0.0f && sx < float(width) && sy < float(height) && opacity > 0.0f) {
+        d.hit = true;
+        d.px = u32(sx);
+        d.py = u32(sy);
+        d.pixel = d.py * width + d.px;
+        float zc = fminf(fmaxf(z, 0.0f), 1.0f);
+        float idx = __fmul_rn(zc, 255.0f);
+        float fi = floorf(idx);  // idx >= 0: floor = trunc
+        float mix = idx - fi;
+        const rfk_pal_lut e = lut[u32(fi)];
+        float r = fmaf(mix, e.step[0], e.lo[0]), g = fmaf(mix, e.step[1], e.lo[1]), b = fmaf(mix, e.step[2], e.lo[2]);
+        d.r = u32(fmaf(r, opacity, 0.5f));
+        d.g = u32(fmaf(g, opacity, 0.5f));
+        d.b = u32(fmaf(b, opacity, 0.5f));
+        d.w = u32(fmaf(opacity, 255.0f, 0.5f));
+    }
+    return d;
+}
+
 // 64-bit bins: four u64 atomics, no wrap at any budget
 RFK_INLINE void rfk_commit_draw(u64* __restrict__ hist, const rfk_draw_t& d) {
     u64* h = hist + 4ull * d.pixel;

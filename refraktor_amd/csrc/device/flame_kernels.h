@@ -114,7 +114,7 @@ typedef u32 hist_t;
 #define RFK_PICK_ROWS 16u
 #define RFK_PERM_ROWS 32u
 struct rfk_shared {
-    u32 pal[256];
+    rfk_pal_lut pal[256];
     Real pts[2][3][RFK_TPB];
     u32x4 rng[RFK_TPB / 64][RFK_PICK_ROWS + RFK_PERM_ROWS];
     u64 hits, draws, bad, start;
@@ -197,7 +197,7 @@ RFK_INLINE u32 rfk_block(const rfk_iter_args& a, rfk_shared& sm, param_ptr P, co
     const bool trace = (flags & RFK_FLAG_TRACE) != 0u;
     const bool bailout = (flags & RFK_FLAG_BAILOUT) != 0u;
 
-    for (u32 i = t; i < 256u; i += RFK_TPB) sm.pal[i] = pal[i];
+    for (u32 i = t; i < 256u; i += RFK_TPB) sm.pal[i] = rfk_pal_lut_entry(pal, i);
     if (t == 0) {
         sm.hits = 0;
         sm.published = 0;

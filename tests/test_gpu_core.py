@@ -120,6 +120,33 @@ def _bin_compare(st, pts, label):
                     f"{o.sum(axis=(0, 1))}\n" + "\n".join(lines))
 
 
+@pytest.mark.parametrize("queue", [False, True])
+def test_iterate_draws_equal_bin_points(small, db, queue):
+    """the iterate kernel's own draws (palette lerp from its LDS table) add
+    exactly what the oracle's bin_points adds for the same points: a traced
+    run and a binned run of the same walkers from the same warmup see the
+    same points (the small flame has no final xform), binned per block with
+    the block's screen transform and palette"""
+    f, k = small
+    if queue:
+        k = db.compile(f, bin_queue=True)
+    s = k.pack_samples(W, H)
+    passes, walkers = 8, 4096
+    st = k.warmup(s, W, H, seed=4, fuse=2, walkers=walkers)
+    tr, _ = st.trace(passes)
+    st2 = k.warmup(s, W, H, seed=4, fuse=2, walkers=walkers)
+    r = st2.bin(iters=walkers * passes)
+    assert r["total_passes"] == walkers * passes
+    got = st2.histogram()
+    inf = st.info()
+    P, pal, tpb = st.params(), st.palettes(), inf.threads_per_block
+    want = np.zeros_like(got)
+    for b in range(inf.blocks):
+        want += oracle.bin_points(P[b, :6], pal[b], tr[:, b * tpb:(b + 1) * tpb].reshape(-1, 4), W, H)
+    assert int(want[..., 3].sum()) > 0
+    np.testing.assert_array_equal(got, want)
+
+
 def test_bin_points_bit_exact(small):
     f, k = small
     st = k.warmup(k.pack_samples(W, H), W, H, seed=2, fuse=0, walkers=1024)
