@@ -750,7 +750,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_strip_kernel(const bf16* __re
     const __amdgpu_buffer_rsrc_t r1 = buffer_rsrc(x1n, unsigned(H1 * W1 * c1) * 2u);
     const __amdgpu_buffer_rsrc_t r2 = buffer_rsrc(x2n ? x2n : x1n, x2n ? unsigned(H * W * c2) * 2u : 0u);
 
-    // the lane's part of a staged row: channel group g, column (l + 64 q) / G
+    // the lane's part of a staged row: channel group g, column stage_col(q, l) in round q
     const int g = l % G;
     unsigned o1c[NQ], o2c[NQ];
     bool colok[NQ];
@@ -1073,7 +1073,7 @@ __global__ __launch_bounds__(64 * NW) void conv3x3_pair_kernel(const bf16* __res
     // A's row columns 32 and 33 are B's operands only for the two unused lanes
     if (l < 2 * (MP / 8)) *reinterpret_cast<bf16x8*>(s_m + (32 + l / (MP / 8)) * MP + (l % (MP / 8)) * 8) = zero8();
 
-    // the lane's part of a staged input row: channel group g, columns (l + 64 q) / G
+    // the lane's part of a staged input row: channel group g, column stage_col(q, l) in round q
     const int g = l % G;
     unsigned oc[NQ];
     bool colok[NQ];
