@@ -149,12 +149,15 @@ template <class BinT>
 __global__ __launch_bounds__(kHistoBlock) void density_histogram_kernel(const BinT* __restrict__ hist, size_t npix,
                                                                         unsigned long long* __restrict__ bins97,
                                                                         unsigned long long* __restrict__ max_density) {
-    constexpr unsigned kBins = 4 * 24 + 1;
-    // a workgroup counts fewer than 2^32 pixels: 32-bit LDS counters, added
-    // into the 64-bit global bins once
-    __shared__ unsigned s_bins[kBins];
+    constexpr unsigned kBins = 4 * 24 + 1, kWaves = kHistoBlock / 64;
+    // a workgroup counts fewer than 2^32 pixels: 32-bit LDS counters, one
+    // set per wave (the lanes of one wave that share a bin serialise on it,
+    // the waves of the workgroup no longer do), summed and added into the
+    // 64-bit global bins once
+    __shared__ unsigned s_wbins[kWaves][kBins];
     __shared__ unsigned long long s_max;
-    for (unsigned i = threadIdx.x; i < kBins; i += blockDim.x) s_bins[i] = 0;
+    for (unsigned i = threadIdx.x; i < kWaves * kBins; i += blockDim.x) (&s_wbins[0][0])[i] = 0;
+    unsigned* s_bins = s_wbins[threadIdx.x >> 6];
     if (threadIdx.x == 0) s_max = 0;
     __syncthreads();
     const float log_div = 4.0f / logf(2.0f);
@@ -189,8 +192,12 @@ __global__ __launch_bounds__(kHistoBlock) void density_histogram_kernel(const Bi
     if (low) atomicAdd(&s_bins[0], unsigned(low));
     if (mx) atomicMax(&s_max, mx);
     __syncthreads();
-    for (unsigned i = threadIdx.x; i < kBins; i += blockDim.x)
-        if (s_bins[i]) atomicAdd(&bins97[i], (unsigned long long)s_bins[i]);
+    for (unsigned i = threadIdx.x; i < kBins; i += blockDim.x) {
+        unsigned long long sum = 0;
+#pragma unroll
+        for (unsigned w = 0; w < kWaves; w++) sum += s_wbins[w][i];
+        if (sum) atomicAdd(&bins97[i], sum);
+    }
     if (threadIdx.x == 0) atomicMax(max_density, s_max);
 }
 
