@@ -1711,9 +1711,16 @@ static size_t strip_lds_bytes(int Cin, int KC, int nw) {
 
 // waves per strip workgroup (one workgroup per CU either way, its LDS being
 // the layer's weights): direct reads 8, the upsampling read 12 where the
-// weights leave room (96->32 at 1080p 153 -> 143 us), the pooling read 4
-static int strip_waves(int mode, int Cin, int KC) {
-    if (mode == 1) return strip_lds_bytes(Cin, KC, 12) <= 160 * 1024 ? 12 : 8;
+// weights leave room (96->32 at 1080p 153 -> 143 us), 10 where the row's
+// 32-pixel strips come in tens but not in twelves (960 wide: three full
+// workgroups instead of two and a half-empty one, 114 -> 110 us for
+// 160->64 at 540x960), the pooling read 4
+static int strip_waves(int mode, int Cin, int KC, int W = 0) {
+    if (mode == 1) {
+        const int strips = (W + 31) / 32;
+        if (W > 0 && strips % 10 == 0 && strips % 12 != 0 && strip_lds_bytes(Cin, KC, 10) <= 160 * 1024) return 10;
+        return strip_lds_bytes(Cin, KC, 12) <= 160 * 1024 ? 12 : 8;
+    }
     return mode == 2 ? 4 : 8;
 }
 
@@ -1814,7 +1821,7 @@ static int strip_out(const strip_args& a) {
 // tiled kernel). The LDS size was computed for nw waves.
 static bool strip_instance(int mode, int nw, int out) {
     if (mode == 0) return nw == 8;
-    if (mode == 1) return (nw == 8 || nw == 12) && out == 0;
+    if (mode == 1) return (nw == 8 || nw == 10 || nw == 12) && out == 0;
     return mode == 2 && nw == 4 && out == 0;
 }
 
@@ -1822,6 +1829,7 @@ template <int KC>
 static hipError_t launch_strip(const strip_args& a, int mode, int nw, size_t lds, hipStream_t s) {
     const int out = strip_out(a);
     if (!strip_instance(mode, nw, out)) return hipErrorInvalidValue;
+    if (mode == 1 && nw == 10) return launch_strip_mode<KC, 1, 10, 0>(a, lds, s);
     if (mode == 1) return nw == 12 ? launch_strip_mode<KC, 1, 12, 0>(a, lds, s) : launch_strip_mode<KC, 1, 8, 0>(a, lds, s);
     if (mode == 2) return launch_strip_mode<KC, 2, 4, 0>(a, lds, s);
     if (out == 0) return launch_strip_mode<KC, 0, 8, 0>(a, lds, s);
@@ -1862,7 +1870,7 @@ hipError_t conv3x3_forward(const void* x1, int c1, const void* x2, int c2, int N
     const strip_args a{x1, c1, x2, c2, N, H, W, wgt, bias, Cout, act, slope, y, pool_fused ? ypool : nullptr, yf,
                        nullptr, nullptr};
     const int out = strip_out(a);
-    const int nw = strip_waves(x1_mode, Cin, KC);
+    const int nw = strip_waves(x1_mode, Cin, KC, W);
     const size_t lds = strip_lds_bytes(Cin, KC, nw);
     // (the strip kernel reads every KC-channel chunk from one of x1, x2)
     const bool chunks_split = c2 == 0 || c1 % KC == 0;
