@@ -63,29 +63,6 @@ constexpr size_t kBinLds = size_t(kCopies) * 2 * kTilePix * sizeof(unsigned long
 
 __device__ __forceinline__ uint32_t rec_tile(u64 r) { return uint32_t(r >> 44); }
 
-// Lanes of a wave that share the first valid lane's key add once, by the
-// first of them; the others add on their own. Returns the rank of the lane's
-// record among those counted for its key in this call.
-__device__ __forceinline__ uint32_t wave_count(uint32_t* counters, uint32_t key, bool valid) {
-    const u64 vm = __ballot(valid);
-    if (vm == 0ull) return 0u;
-    const uint32_t lead = uint32_t(__shfl(int(key), int(__builtin_ctzll(vm))));
-    const u64 same = __ballot(valid && key == lead);
-    const uint32_t lane = __lane_id();
-    uint32_t rank = 0;
-    if (valid && key == lead) {
-        const uint32_t first = uint32_t(__builtin_ctzll(same));
-        const uint32_t below = uint32_t(__popcll(same & ((1ull << lane) - 1ull)));
-        uint32_t base = 0;
-        if (lane == first) base = atomicAdd(&counters[key], uint32_t(__popcll(same)));
-        base = __shfl(base, int(first));
-        rank = base + below;
-    } else if (valid) {
-        rank = atomicAdd(&counters[key], 1u);
-    }
-    return rank;
-}
-
 // Block-wide exclusive scan of one value per thread (blockDim.x = T).
 template <uint32_t T = kThreads>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t* total) {
@@ -207,11 +184,14 @@ __global__ void __launch_bounds__(T)
         for (uint32_t j = 0; j < kPer; j++) r[j] = nxt[j];
         if (base + PIECE < b1) load_piece(base + PIECE);
         __syncthreads();
+        // a record's rank among its tile's records in this piece, one LDS
+        // add with return each (which of a tile's records comes first does
+        // not matter: the tile's sums are integer)
         uint32_t rank[kPer];
 #pragma unroll
         for (uint32_t j = 0; j < kPer; j++) {
             const uint32_t tile = rec_tile(r[j]);
-            rank[j] = wave_count(lcount, tile, tile < ntiles);
+            rank[j] = tile < ntiles ? atomicAdd(&lcount[tile], 1u) : 0u;
         }
         __syncthreads();
         uint32_t sum = 0;
