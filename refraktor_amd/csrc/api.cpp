@@ -622,7 +622,7 @@ int rfk_warmup_ex(rfk_kernel* k, void* stream, const double* samples, size_t n_r
     s->size_reals = uint32_t(k->L.size_reals);
     s->precision = k->opt.precision;
     const uint32_t align = s->precision ? 8u : 16u;  // 64-byte rows
-    s->param_stride = (s->size_reals + align - 1) / align * align;
+    s->param_stride = (uint32_t(k->L.device_reals()) + align - 1) / align * align;
     s->seed = seed;
     s->stream_id = stream_id;
     if (stream) {
@@ -993,7 +993,7 @@ int rfk_batch_create(rfk_kernel* k, void* stream, const double* samples, size_t 
     if (total > (1u << 24)) return fail(RFK_ERR_INVALID, "too many blocks in the batch");
     B->total_blocks = uint32_t(total);
     const uint32_t align = k->opt.precision ? 8u : 16u;
-    B->param_stride = (uint32_t(k->L.size_reals) + align - 1) / align * align;
+    B->param_stride = (uint32_t(k->L.device_reals()) + align - 1) / align * align;
     B->seed = seed;
     B->fuse = fuse;
     const uint64_t walkers_f = uint64_t(B->blocks_per_frame) * B->tpb;

@@ -132,6 +132,27 @@ std::string make_flame_functions(const variation_db& db, const flame& f, const f
     os << "    RFK_EXACT_FP\n";
     // IEEE-exact products and sums so the choice is reproducible off the GPU
     os << "    ratio = RFK_MUL_RN(ratio, " << P(L.weight_sum) << ");\n";
+    if (L.chaos_base < 0 && n > 0) {
+        // the same choice from the sample's table (rfk_precalc_sample): the
+        // running sums' bits order like the sums themselves where the flag
+        // says every weight and their total are finite and not negative, so
+        // the wave-uniform choice is scalar loads and integer compares
+        const int base = L.size_reals;
+        os << "#ifndef RFK_DOUBLE\n";
+        os << "    {\n";
+        os << "        typedef const __attribute__((address_space(4))) int* table_ptr;\n";
+        os << "        const table_ptr S = (table_ptr)(P + " << base << ");\n";
+        // every slot read before the first compare: one wide scalar load
+        os << "        int s[" << (n + 1) << "];\n";
+        os << "        for (int i = 0; i < " << (n + 1) << "; i++) s[i] = S[i];\n";
+        os << "        if (s[" << n << "] != 0) {\n";
+        os << "            const int rb = __builtin_amdgcn_readfirstlane(__float_as_int(ratio));\n";
+        for (size_t i = 0; i + 1 < n; i++) os << "            if (s[" << i << "] >= rb) return " << i << "u;\n";
+        os << "            return u32(s[" << (n - 1) << "]);\n";
+        os << "        }\n";
+        os << "    }\n";
+        os << "#endif\n";
+    }
     os << "    Real rsum = Real(0);\n    u32 last_nonzero = 0;\n";
     for (size_t i = 0; i < n; i++) {
         const std::string w = P(L.xforms[i].weight);
@@ -172,6 +193,30 @@ std::string make_flame_functions(const variation_db& db, const flame& f, const f
     os << "    " << P(L.weight_sum) << " = ";
     for (size_t i = 0; i < n; i++) os << (i ? " + " : "") << P(L.xforms[i].weight);
     os << ";\n";
+    if (L.chaos_base < 0 && n > 0) {
+        // rfk_select_xform's table: slot i < n-1 the bits of the running sum
+        // through weight i (-1 for a zero weight, which is never chosen),
+        // slot n-1 the choice when no sum reaches the ratio, slot n the flag
+        const int base = L.size_reals;
+        os << "#ifndef RFK_DOUBLE\n    {\n";
+        os << "        int* S = (int*)(P + " << base << ");\n";
+        os << "        Real rsum = Real(0);\n        int last_nonzero = 0;\n        bool ok = true;\n";
+        for (size_t i = 0; i < n; i++) {
+            const std::string w = P(L.xforms[i].weight);
+            // finite and not negative (-0 included), read from the bits
+            os << "        { const u32 b = __float_as_uint(" << w << "); ok = ok && (b < 0x7F800000u || b == 0x80000000u); }\n";
+            if (i + 1 < n) {
+                os << "        rsum = RFK_ADD_RN(rsum, " << w << ");\n";
+                os << "        S[" << i << "] = " << w << " != Real(0) ? __float_as_int(rsum) : -1;\n";
+                os << "        if (" << w << " != Real(0)) last_nonzero = " << i << ";\n";
+            } else {
+                os << "        S[" << i << "] = " << w << " != Real(0) ? " << i << " : last_nonzero;\n";
+            }
+        }
+        os << "        { const u32 b = __float_as_uint(" << P(L.weight_sum) << "); ok = ok && b < 0x7F800000u; }\n";
+        os << "        S[" << n << "] = ok ? 1 : 0;\n";
+        os << "    }\n#endif\n";
+    }
     if (L.chaos_base >= 0) {
         for (size_t i = 0; i < n; i++) {
             int row = L.chaos_base + int(i * (n + 1));

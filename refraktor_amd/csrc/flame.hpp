@@ -117,6 +117,10 @@ struct flame_layout {
     std::vector<xform_slot> xforms;
     std::optional<xform_slot> final_xform;
     std::vector<size_t> affine_indices;
+    // reals per block on the device: the layout, then (without xaos) the
+    // xform selection table that the per-sample precalc fills: n slots from
+    // size_reals on and a flag, none of them part of the reference's layout
+    int device_reals() const { return size_reals + (chaos_base < 0 ? int(xforms.size()) + 1 : 0); }
 };
 
 flame_layout make_layout(const flame& f, const variation_db& db);
