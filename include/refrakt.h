@@ -196,9 +196,10 @@ typedef struct rfk_batch_frame_info {
     uint32_t hist_bits;       /* 32 or 64                                     */
     /* what renders the frame: its temporal samples' parameter rows (device,
      * `blocks` rows of param_stride reals of the kernel's precision, after
-     * precalc; a row's first size_reals are the flame layout, the reals after
-     * them the kernel's own xform-choice table) and palettes (device, blocks x 256 RGBA8); its walkers run the
-     * streams (seed, stream_id) with `fuse` passes before binning          */
+     * precalc; a row's first size_reals are the flame layout, the reals
+     * after them the kernel's own xform-choice table) and palettes (device,
+     * blocks x 256 RGBA8); its walkers run the streams (seed, stream_id)
+     * with `fuse` passes before binning                                     */
     const void* params;
     const uint32_t* palettes;
     uint32_t param_stride, blocks, seed, stream_id, fuse;
